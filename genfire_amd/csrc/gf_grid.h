@@ -1,0 +1,110 @@
+// gf_grid.h -- internal launch interface of the gridding, FFT, iteration and
+// projector kernels.  Every function enqueues on the given stream and returns.
+#pragma once
+
+#include "gf_common.h"
+
+namespace gf {
+
+// ---- index pass (gf_grid_index.hip) ----
+int index_threads();
+void grid_index_count(const PlaneConst* d_planes, int P, const HalfGrid& hg, double cutoff, int n1,
+                      int n2, int nblk, uint32_t* d_blk_counts, hipStream_t st);
+void grid_index_scan(const uint32_t* d_counts, int64_t n, int64_t* d_offsets, int64_t* d_total,
+                     hipStream_t st);
+void grid_index_emit(const PlaneConst* d_planes, int P, const HalfGrid& hg, double cutoff, int n1,
+                     int n2, int nblk, const int64_t* d_blk_offsets, uint32_t* ind, float* dist,
+                     float* un, float* vn, hipStream_t st);
+
+// ---- DFT values, scatter, finalize (gf_grid_values.hip) ----
+// Tile of the value kernels: points [begin, begin+count) of projection p.
+struct PointTile {
+    int64_t begin;
+    int32_t count;
+    int32_t proj;
+};
+// FP32 direct DFT of every point (fillInFourierGrid_DFT.m:236-248).
+void grid_values_fp32(const float* d_proj, int dim1, int dim2, const float* d_un, const float* d_vn,
+                      const PointTile* d_tiles, int ntiles, float2* d_F, hipStream_t st);
+// Weighted-average accumulation of one projection's points (:267-291), in order.
+void grid_scatter(const uint32_t* d_ind, const float* d_dist, const float2* d_F, int64_t begin,
+                  int64_t count, double2* d_acc, double* d_sw, uint32_t* d_hits, float2* d_exact,
+                  hipStream_t st);
+// FS -> measuredK in FFT (ifftshifted) order, with CentroSymmetricity's mirror
+// rule when centro != 0, and the single cast of runGridding.
+void grid_finalize(const HalfGrid& hg, int n1, int n2, int centro, const double2* d_acc,
+                   const double* d_sw, const uint32_t* d_hits, const float2* d_exact, float2* d_M,
+                   hipStream_t st);
+constexpr uint32_t kExactBit = 0x80000000u;
+
+// ---- layout helpers (gf_grid_values.hip) ----
+// Centred <-> FFT-order conversion of an n1 x n2 x n3 complex grid
+// (to_centred = 1: out = fftshift(in), 0: out = ifftshift(in)).
+void shift_complex(const float2* d_in, float2* d_out, int n1, int n2, int n3, int to_centred,
+                   hipStream_t st);
+// Real crop of the centred d1 x d2 x d3 box out of an FFT-order n1 x n2 x n3
+// grid (My_stripzero(fftshift(x), ...)); src_complex selects float2 input (real part).
+void crop_centre(const void* d_src, int src_complex, int n1, int n2, int n3, float* d_dst, int d1,
+                 int d2, int d3, hipStream_t st);
+// Inverse of crop_centre for real data: zero-pad d1 x d2 x d3 into n1 x n2 x n3
+// FFT order (ifftshift(My_paddzero(v, n))).
+void pad_centre(const float* d_src, int d1, int d2, int d3, float* d_dst, int n1, int n2, int n3,
+                hipStream_t st);
+void pad_centre_u8(const uint8_t* d_src, int d1, int d2, int d3, uint8_t* d_dst, int n1, int n2,
+                   int n3, hipStream_t st);
+
+// ---- 3D FFT (gf_fft.hip) ----
+bool fft_supported(int n);
+// Complex-to-complex pass along one axis of an n1 x n2 x n3 grid, in place.
+// inverse passes scale by 1/n_axis.
+void fft_axis(float2* d, int n1, int n2, int n3, int axis, int inverse, hipStream_t st);
+void fft3(float2* d, int n1, int n2, int n3, int inverse, hipStream_t st);
+
+// ---- iteration passes (gf_fft.hip) ----
+struct IterArgs {
+    float* x;            // iterate, real, FFT order
+    float2* K;           // spectrum workspace
+    const uint8_t* S;    // padded support, FFT order (0/1)
+    const float2* M;     // measuredK, FFT order
+    const uint8_t* flags;  // bit0 measured (errK set), bit1 enforced (replacement set)
+    float* best;         // P_R(x) output (nullable)
+    float* pf_out;       // P_F output copy (nullable)
+    double* err_part;    // per-workgroup partial errK numerators
+    int n1, n2, n3;
+    int positivity, support, dr;
+};
+constexpr uint8_t kFlagMeasured = 1;
+constexpr uint8_t kFlagEnforced = 2;
+// Pass 1: P_R (or the DR reflection) fused into the axis-0 forward FFT.
+void iter_first(const IterArgs& a, hipStream_t st);
+// Axis-2 forward FFT, errK partials, replacement, axis-2 inverse FFT.
+int iter_fused_blocks(const IterArgs& a);
+void iter_fused(const IterArgs& a, hipStream_t st);
+// Axis-0 inverse FFT fused with real part and the GENFIRE / DR update.
+void iter_last(const IterArgs& a, hipStream_t st);
+// errK[it] = sum(partials) / den.
+void iter_errk_reduce(const double* d_part, int n, double den, float* d_errK_it, hipStream_t st);
+// flags from M, mask and the enforcement threshold (1-Q > delay), reconstruct.m:42,120.
+void iter_flags(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, double delay,
+                uint8_t* d_flags, hipStream_t st);
+// sum |M| over measured voxels (errK denominator), into d_out (double).
+void iter_den(const float2* d_M, const uint8_t* d_flags, int64_t n, double* d_tmp, int ntmp,
+              double* d_out, hipStream_t st);
+
+// ---- projector (gf_project.hip) ----
+struct ProjArgs {
+    const float* vol;  // d1 x d2 x d3 column-major
+    int d1, d2, d3;
+    const float* R;    // P x 9 (column-major 3x3 each), single
+    int P;
+    float* out;        // d1 x d2 x P back projections (nullable)
+    const float* meas; // measured projections m1 x m2 x P (nullable)
+    int m1, m2;        // measured size (crop of the back projection centre)
+    double* sums;      // P x 4 x nblk partials: sum|b-m|, sum (b-m)^2, sum|m|, sum m^2
+    int post_hoc;      // 1: sum ||b|-|m|| instead of |b-m| (Tian_calc_Rfactor_realspace)
+};
+int project_blocks(int m1, int m2);
+void project(const ProjArgs& a, hipStream_t st);
+void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st);
+
+}  // namespace gf

@@ -1,0 +1,286 @@
+// gf_grid_values.hip -- DFT values, weighted average and centrosymmetric fill of
+// fillInFourierGrid_DFT.m:236-300, plus the centred/FFT-order layout helpers.
+#include "gf_grid.h"
+
+namespace gf {
+
+namespace {
+
+constexpr int kVT = 256;  // points per workgroup of the value kernel (one per lane)
+constexpr int kCB = 16;   // projection columns staged in LDS per step
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// exp(-2 pi i k t) for an integer k and an FP32 t.  The phase in turns, k*t, is
+// exact in FP64 (|k| < 2^12, t has 24 mantissa bits); its fractional part is
+// exact; only the final FP32 sincospi of the reduced turn rounds, so the
+// argument error is 2^-25 turn whatever the size of k*t.
+__device__ __forceinline__ float2 cis_turns(double k, float t)
+{
+    const double ph = k * (double)t;
+    const float r = (float)(ph - rint(ph));
+    float s, c;
+    sincospif(2.0f * r, &s, &c);
+    return make_float2(c, -s);
+}
+
+// F_j = sum_b exp(-2 pi i k2o[b] v_j) * sum_a exp(-2 pi i k1o[a] u_j) P[a,b]
+// (the separable form of the exponent K10*u + K20*v at :243).  One lane per
+// point, 256 points of one projection per workgroup.  The inner sum runs over
+// 16-row blocks: exp(-2 pi i (16h + l - c1) u) = E_hi(h) * E_lo(l) with both
+// factors from cis_turns, so every factor carries 2^-25-turn phase accuracy.
+// Projection columns stream through LDS 16 at a time and are read as broadcasts.
+__global__ __launch_bounds__(kVT) void k_dft_fp32(const float* __restrict__ proj_all, int dim1,
+                                                  int dim2, const float* __restrict__ un,
+                                                  const float* __restrict__ vn,
+                                                  const PointTile* __restrict__ tiles,
+                                                  float2* __restrict__ F)
+{
+    extern __shared__ float sP[];  // (16*nh1) rows x kCB columns
+    const PointTile tl = tiles[blockIdx.x];
+    const int i = threadIdx.x;
+    const bool valid = i < tl.count;
+    const int64_t j = tl.begin + (valid ? i : 0);
+    const float t1 = valid ? un[j] : 0.0f;
+    const float t2 = valid ? vn[j] : 0.0f;
+    const float* P = proj_all + (int64_t)tl.proj * dim1 * dim2;
+    const int c1 = dim1 / 2, c2 = dim2 / 2;
+    const int nh1 = (dim1 + 15) / 16;
+    const int rows = nh1 * 16;
+
+    float2 elo1[16], elo2[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+        elo1[l] = cis_turns((double)l, t1);
+        elo2[l] = cis_turns((double)l, t2);
+    }
+    float Fr = 0.0f, Fi = 0.0f;
+    for (int b0 = 0; b0 < dim2; b0 += kCB) {
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < rows * kCB; idx += kVT) {
+            const int a = idx % rows, bb = idx / rows;
+            sP[a * kCB + bb] = (a < dim1 && b0 + bb < dim2) ? P[a + (int64_t)dim1 * (b0 + bb)] : 0.0f;
+        }
+        __syncthreads();
+        float gr[kCB], gi[kCB];
+#pragma unroll
+        for (int bb = 0; bb < kCB; ++bb) { gr[bb] = 0.0f; gi[bb] = 0.0f; }
+        for (int h = 0; h < nh1; ++h) {
+            const float2 ehi = cis_turns((double)(16 * h - c1), t1);
+#pragma unroll
+            for (int l = 0; l < 16; ++l) {
+                const float2 e = cmul(ehi, elo1[l]);
+                const float4* row = reinterpret_cast<const float4*>(sP + (16 * h + l) * kCB);
+#pragma unroll
+                for (int q = 0; q < kCB / 4; ++q) {
+                    const float4 r = row[q];
+                    gr[4 * q + 0] += e.x * r.x; gi[4 * q + 0] += e.y * r.x;
+                    gr[4 * q + 1] += e.x * r.y; gi[4 * q + 1] += e.y * r.y;
+                    gr[4 * q + 2] += e.x * r.z; gi[4 * q + 2] += e.y * r.z;
+                    gr[4 * q + 3] += e.x * r.w; gi[4 * q + 3] += e.y * r.w;
+                }
+            }
+        }
+        const float2 e2hi = cis_turns((double)(b0 - c2), t2);
+#pragma unroll
+        for (int bb = 0; bb < kCB; ++bb) {
+            const float2 e = cmul(e2hi, elo2[bb]);
+            Fr += gr[bb] * e.x - gi[bb] * e.y;
+            Fi += gr[bb] * e.y + gi[bb] * e.x;
+        }
+    }
+    if (valid) F[j] = make_float2(Fr, Fi);
+}
+
+// :271-289 for one projection (each half-grid voxel appears at most once per
+// projection, and projections are launched in order, so the sums are formed in
+// the reference's list order without atomics):
+//   exact (D < 1e-7): value overwrite, flag set (last projection wins, :291);
+//   else w = 1/D in single (:275), accumulated in FP64.
+__global__ void k_scatter(const uint32_t* __restrict__ ind, const float* __restrict__ dist,
+                          const float2* __restrict__ F, int64_t begin, int64_t count,
+                          double2* __restrict__ acc, double* __restrict__ sw,
+                          uint32_t* __restrict__ hits, float2* __restrict__ exact)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t i = begin + t;
+    const uint32_t h = ind[i];
+    const float d = dist[i];
+    const float2 f = F[i];
+    const uint32_t hh = hits[h];
+    if ((double)d < 1e-7) {
+        exact[h] = f;
+        hits[h] = (hh + 1u) | kExactBit;
+    } else {
+        const float w = 1.0f / d;
+        const double wd = (double)w;
+        double2 a = acc[h];
+        a.x += wd * (double)f.x;
+        a.y += wd * (double)f.y;
+        acc[h] = a;
+        sw[h] += wd;
+        hits[h] = hh + 1u;
+    }
+}
+
+__device__ __forceinline__ int signed_freq(int s, int n) { return s < (n + 1) / 2 ? s : s - n; }
+
+// FS(index_interp) = sum(wF)/sum(w); FS(index_k) = exact (:287-291); single cast
+// (GENFIRE_Reconstructor.m:182); CentroSymmetricity.m: k1 <= 0 from FS(k),
+// k1 > 0 from conj(FS(-k)) (all four parity branches reduce to this rule).
+// Written directly in FFT order: voxel s holds frequency k = s (mod n).
+__global__ void k_finalize(HalfGrid hg, int n1, int n2, int centro, const double2* __restrict__ acc,
+                           const double* __restrict__ sw, const uint32_t* __restrict__ hits,
+                           const float2* __restrict__ exact, float2* __restrict__ M)
+{
+    const int64_t total = (int64_t)n1 * n2 * n1;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int s1 = (int)(s % n1);
+        const int s2 = (int)((s / n1) % n2);
+        const int s3 = (int)(s / ((int64_t)n1 * n2));
+        int k1 = signed_freq(s1, n1), k2 = signed_freq(s2, n2), k3 = signed_freq(s3, n1);
+        bool cj = false;
+        if (centro && k1 > 0) { k1 = -k1; k2 = -k2; k3 = -k3; cj = true; }
+        const int64_t h = (int64_t)(k1 - hg.lo[0]) +
+                          (int64_t)hg.len[0] * ((k2 - hg.lo[1]) + (int64_t)hg.len[1] * (k3 - hg.lo[2]));
+        float2 v = make_float2(0.0f, 0.0f);
+        const uint32_t hh = hits[h];
+        if (hh & kExactBit) {
+            v = exact[h];
+        } else if (sw[h] > 0.0) {
+            const double2 a = acc[h];
+            const double w = sw[h];
+            v = make_float2((float)(a.x / w), (float)(a.y / w));
+        }
+        if (cj) v.y = -v.y;
+        M[s] = v;
+    }
+}
+
+__global__ void k_shift(const float2* __restrict__ in, float2* __restrict__ out, int n1, int n2,
+                        int n3, int to_centred)
+{
+    const int64_t total = (int64_t)n1 * n2 * n3;
+    const int o1 = to_centred ? (n1 + 1) / 2 : n1 / 2;
+    const int o2 = to_centred ? (n2 + 1) / 2 : n2 / 2;
+    const int o3 = to_centred ? (n3 + 1) / 2 : n3 / 2;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int c1 = (int)(t % n1), c2 = (int)((t / n1) % n2), c3 = (int)(t / ((int64_t)n1 * n2));
+        const int64_t src = (int64_t)((c1 + o1) % n1) +
+                            (int64_t)n1 * (((c2 + o2) % n2) + (int64_t)n2 * ((c3 + o3) % n3));
+        out[t] = in[src];
+    }
+}
+
+__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+__global__ void k_crop(const void* __restrict__ src, int src_complex, int n1, int n2, int n3,
+                       float* __restrict__ dst, int d1, int d2, int d3)
+{
+    const int64_t total = (int64_t)d1 * d2 * d3;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(t % d1), b = (int)((t / d1) % d2), c = (int)(t / ((int64_t)d1 * d2));
+        const int s1 = wrap(a - d1 / 2, n1), s2 = wrap(b - d2 / 2, n2), s3 = wrap(c - d3 / 2, n3);
+        const int64_t si = (int64_t)s1 + (int64_t)n1 * (s2 + (int64_t)n2 * s3);
+        dst[t] = src_complex ? static_cast<const float2*>(src)[si].x : static_cast<const float*>(src)[si];
+    }
+}
+
+template <class T>
+__global__ void k_pad(const T* __restrict__ src, int d1, int d2, int d3, T* __restrict__ dst, int n1,
+                      int n2, int n3)
+{
+    const int64_t total = (int64_t)n1 * n2 * n3;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int s1 = (int)(s % n1), s2 = (int)((s / n1) % n2), s3 = (int)(s / ((int64_t)n1 * n2));
+        // the centred offset of FFT index s is its signed frequency; box index = k + floor(d/2)
+        const int a = signed_freq(s1, n1) + d1 / 2;
+        const int b = signed_freq(s2, n2) + d2 / 2;
+        const int c = signed_freq(s3, n3) + d3 / 2;
+        T v = T(0);
+        if (a >= 0 && a < d1 && b >= 0 && b < d2 && c >= 0 && c < d3)
+            v = src[(int64_t)a + (int64_t)d1 * (b + (int64_t)d2 * c)];
+        dst[s] = v;
+    }
+}
+
+int grid_blocks(int64_t n, int threads)
+{
+    int64_t b = (n + threads - 1) / threads;
+    return (int)(b < 65536 * 4 ? (b < 1 ? 1 : b) : 65536 * 4);
+}
+
+}  // namespace
+
+void grid_values_fp32(const float* d_proj, int dim1, int dim2, const float* d_un, const float* d_vn,
+                      const PointTile* d_tiles, int ntiles, float2* d_F, hipStream_t st)
+{
+    if (ntiles <= 0) return;
+    const size_t lds = (size_t)((dim1 + 15) / 16) * 16 * kCB * sizeof(float);
+    GF_CHECK(lds <= 64 * 1024, GENFIRE_ERR_UNSUPPORTED, "projection Dim1 too large for the DFT kernel");
+    k_dft_fp32<<<ntiles, kVT, lds, st>>>(d_proj, dim1, dim2, d_un, d_vn, d_tiles, d_F);
+    GF_LAUNCH_CHECK();
+}
+
+void grid_scatter(const uint32_t* d_ind, const float* d_dist, const float2* d_F, int64_t begin,
+                  int64_t count, double2* d_acc, double* d_sw, uint32_t* d_hits, float2* d_exact,
+                  hipStream_t st)
+{
+    if (count <= 0) return;
+    const int threads = 256;
+    const int blocks = (int)((count + threads - 1) / threads);
+    k_scatter<<<blocks, threads, 0, st>>>(d_ind, d_dist, d_F, begin, count, d_acc, d_sw, d_hits, d_exact);
+    GF_LAUNCH_CHECK();
+}
+
+void grid_finalize(const HalfGrid& hg, int n1, int n2, int centro, const double2* d_acc,
+                   const double* d_sw, const uint32_t* d_hits, const float2* d_exact, float2* d_M,
+                   hipStream_t st)
+{
+    const int64_t total = (int64_t)n1 * n2 * n1;
+    k_finalize<<<grid_blocks(total, 256), 256, 0, st>>>(hg, n1, n2, centro, d_acc, d_sw, d_hits,
+                                                        d_exact, d_M);
+    GF_LAUNCH_CHECK();
+}
+
+void shift_complex(const float2* d_in, float2* d_out, int n1, int n2, int n3, int to_centred,
+                   hipStream_t st)
+{
+    const int64_t total = (int64_t)n1 * n2 * n3;
+    k_shift<<<grid_blocks(total, 256), 256, 0, st>>>(d_in, d_out, n1, n2, n3, to_centred);
+    GF_LAUNCH_CHECK();
+}
+
+void crop_centre(const void* d_src, int src_complex, int n1, int n2, int n3, float* d_dst, int d1,
+                 int d2, int d3, hipStream_t st)
+{
+    const int64_t total = (int64_t)d1 * d2 * d3;
+    k_crop<<<grid_blocks(total, 256), 256, 0, st>>>(d_src, src_complex, n1, n2, n3, d_dst, d1, d2, d3);
+    GF_LAUNCH_CHECK();
+}
+
+void pad_centre(const float* d_src, int d1, int d2, int d3, float* d_dst, int n1, int n2, int n3,
+                hipStream_t st)
+{
+    const int64_t total = (int64_t)n1 * n2 * n3;
+    k_pad<float><<<grid_blocks(total, 256), 256, 0, st>>>(d_src, d1, d2, d3, d_dst, n1, n2, n3);
+    GF_LAUNCH_CHECK();
+}
+
+void pad_centre_u8(const uint8_t* d_src, int d1, int d2, int d3, uint8_t* d_dst, int n1, int n2,
+                   int n3, hipStream_t st)
+{
+    const int64_t total = (int64_t)n1 * n2 * n3;
+    k_pad<uint8_t><<<grid_blocks(total, 256), 256, 0, st>>>(d_src, d1, d2, d3, d_dst, n1, n2, n3);
+    GF_LAUNCH_CHECK();
+}
+
+}  // namespace gf

@@ -1,0 +1,183 @@
+/*
+ * genfire.h -- C ABI of libgenfire, the MI355X (gfx950) implementation of the
+ * GENFIRE v2 reconstruction hot path: DFT gridding, the GENFIRE / Douglas-Rachford
+ * iteration and the real-space R-factor.
+ *
+ * Conventions
+ *  - Plain C types only.  Arrays are caller-allocated HOST buffers in MATLAB
+ *    (column-major, dim 1 fastest) layout; complex arrays are interleaved
+ *    (re, im) float pairs, the MATLAB R2018a+ interleaved-complex layout.
+ *  - Every call returns GENFIRE_OK (0) or a negative status; the message of the
+ *    last failure on a context is returned by genfire_last_error().  The MATLAB
+ *    reference raises error('GENFIRE: ...') in the same situations.
+ *  - A context owns one GPU; a job owns the device-resident state of one
+ *    reconstruction (projections, k-grid, support, iterate) and its own HIP
+ *    stream.  Use one job per host thread; calls return when the work is done,
+ *    except genfire_job_submit, which only enqueues.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *  - @GENFIRE_Reconstructor/GENFIRE_Reconstructor.m:5-71   properties -> genfire_params
+ *  - GENFIRE_Reconstructor.m:115-171 CheckPrepareData      -> genfire_job_create
+ *  - GENFIRE_Reconstructor.m:172-186 runGridding(method 2) -> genfire_job_grid
+ *  - fillInFourierGrid_DFT.m:85-310                          -> genfire_job_grid
+ *  - CentroSymmetricity.m:2-72                               -> genfire_job_grid
+ *  - reconstruct.m:28-194                                    -> genfire_job_reconstruct
+ *  - src/calculate3Dprojection_RealSpaceinterp.m:1-57        -> genfire_project
+ *  - src/Tian_calc_Rfactor_realspace.m:1-24                  -> genfire_rfactor
+ */
+#ifndef GENFIRE_H
+#define GENFIRE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GENFIRE_OK 0
+#define GENFIRE_ERR_ARG (-1)         /* invalid argument or size                 */
+#define GENFIRE_ERR_HIP (-2)         /* HIP runtime error                         */
+#define GENFIRE_ERR_NOMEM (-3)       /* device allocation failed                  */
+#define GENFIRE_ERR_STATE (-4)       /* call out of order (e.g. reconstruct before grid) */
+#define GENFIRE_ERR_UNSUPPORTED (-5) /* size or option this build does not handle  */
+
+#define GENFIRE_ALG_GENFIRE 0        /* reconstruct.m:117-186                    */
+#define GENFIRE_ALG_DR 1             /* Douglas-Rachford on the same projections */
+
+#define GENFIRE_PREC_FP32 0          /* gridding sums in FP32                    */
+#define GENFIRE_PREC_SPLIT_BF16 1    /* gridding GEMM in split BF16 (hi+lo)      */
+#define GENFIRE_PREC_FP16 2          /* gridding GEMM in FP16, per-projection scale */
+
+typedef struct genfire_ctx genfire_ctx;
+typedef struct genfire_job genfire_job;
+
+/* Mirrors the GENFIRE_Reconstructor properties (GENFIRE_Reconstructor.m:30-70),
+ * same names and defaults; the last three fields are additions of this library. */
+typedef struct genfire_params {
+    int32_t numIterations;              /* 50    (:42) */
+    double oversamplingRatio;           /* 3     (:43) */
+    double interpolationCutoffDistance; /* 0.7   (:44) */
+    int32_t constraintEnforcementMode;  /* 1     (:58)  1, 2 or 3 */
+    int32_t constraintPositivity;       /* 1     (:59) */
+    int32_t constraintSupport;          /* 1     (:60) */
+    int32_t DFT_CentroSymmetricity;     /* 1     (:63) */
+    double vector1[3];                  /* [0 0 1] (:50) phi axis   */
+    double vector2[3];                  /* [0 1 0] (:51) theta axis */
+    double vector3[3];                  /* [1 0 0] (:52) psi axis   */
+    int32_t monitor_R;                  /* 0     (:31) */
+    int32_t monitorR_loopLength;        /* 10    (:32) */
+    int32_t algorithm;                  /* GENFIRE_ALG_GENFIRE */
+    int32_t precision;                  /* GENFIRE_PREC_FP32 */
+    int32_t compute_recIFFT;            /* 1: also form recIFFT (fillInFourierGrid_DFT.m:304) */
+} genfire_params;
+
+/* Library version, major*10000 + minor*100 + patch. */
+int genfire_version(void);
+
+/* Fills p with the defaults listed above (GENFIRE_Reconstructor.m:30-70). */
+int genfire_params_default(genfire_params* p);
+
+/* Context: one GPU.  No counterpart in the reference (MATLAB owns its device). */
+int genfire_ctx_create(int32_t device, genfire_ctx** out);
+int genfire_ctx_destroy(genfire_ctx* ctx);
+const char* genfire_last_error(const genfire_ctx* ctx);
+
+/* CheckPrepareData (GENFIRE_Reconstructor.m:115-171): validates the parameters,
+ * sets n1_oversampled = round(Dim1*ratio), n2_oversampled = round(Dim2*ratio)
+ * (:168-169) and the enforcement schedule (:156-165), and creates a job. */
+int genfire_job_create(genfire_ctx* ctx, const genfire_params* p, int32_t dim1, int32_t dim2,
+                       int32_t num_projs, genfire_job** out);
+int genfire_job_destroy(genfire_job* job);
+
+/* n1_oversampled, n2_oversampled and the half-grid extent (l1, l2, l3) of
+ * fillInFourierGrid_DFT.m:134-136. */
+int genfire_job_sizes(const genfire_job* job, int32_t* n1, int32_t* n2, int64_t half_len[3]);
+
+/* Uploads InputProjections (Dim1 x Dim2 x P float), InputAngles (P x 3 double,
+ * column-major: phi[P], theta[P], psi[P], degrees), Support (Dim1 x Dim2 x Dim1
+ * bytes, nonzero = inside; NULL = all ones, :146-148) and initialObject
+ * (Dim1 x Dim2 x Dim1 float or NULL = zeros, reconstruct.m:76-80). */
+int genfire_job_set_inputs(genfire_job* job, const float* projections, const double* angles,
+                           const uint8_t* support, const float* initial_object);
+
+/* runGridding with griddingMethod = 2 (GENFIRE_Reconstructor.m:172-186):
+ * fillInFourierGrid_DFT + CentroSymmetricity + single cast + measuredK_mask. */
+int genfire_job_grid(genfire_job* job);
+
+/* Gridded point list in projection order (the master lists of
+ * fillInFourierGrid_DFT.m:257-261): 0-based half-grid linear index, projection
+ * (0-based), distance D, u/n1, v/n2 and the DFT value (interleaved complex).
+ * Any output pointer may be NULL. */
+int genfire_job_grid_count(const genfire_job* job, int64_t* num_points);
+int genfire_job_get_points(genfire_job* job, int64_t* ind, int32_t* proj, float* dist, float* un,
+                           float* vn, float* values);
+
+/* measuredK (n1 x n2 x n1 interleaved complex, centred as in the reference) and
+ * the number of gridded points that landed on each half-grid voxel. */
+int genfire_job_get_measuredK(genfire_job* job, float* measuredK);
+int genfire_job_get_hits(genfire_job* job, int32_t* hits);
+/* recIFFT (Dim1 x Dim2 x Dim1), fillInFourierGrid_DFT.m:304. */
+int genfire_job_get_recIFFT(genfire_job* job, float* rec);
+
+/* Replaces measuredK (centred, interleaved complex) and measuredK_mask (n1*n2*n1
+ * bytes or NULL = measuredK ~= 0, GENFIRE_Reconstructor.m:183) on the device. */
+int genfire_job_set_measuredK(genfire_job* job, const float* measuredK, const uint8_t* mask);
+
+/* reconstruct (reconstruct.m:28-194) or the Douglas-Rachford variant. */
+int genfire_job_reconstruct(genfire_job* job);
+/* reconstruction (Dim1 x Dim2 x Dim1 float), reconstruct.m:192. */
+int genfire_job_get_reconstruction(genfire_job* job, float* rec);
+/* errK (numIterations floats), reconstruct.m:135. */
+int genfire_job_get_errK(genfire_job* job, float* errK);
+/* errR (floor(N/L)), Rarr_record and Rarr2_record (P x floor(N/L), column-major),
+ * reconstruct.m:168-185.  Any pointer may be NULL. */
+int genfire_job_get_errR(genfire_job* job, float* errR, double* Rarr, double* Rarr2);
+
+/* Asynchronous form for serving: grid + reconstruct enqueued on the job's
+ * worker; poll returns 1 in *done when finished; wait blocks. */
+int genfire_job_submit(genfire_job* job);
+int genfire_job_poll(genfire_job* job, int32_t* done);
+int genfire_job_wait(genfire_job* job);
+
+/* Wall time of the last grid / reconstruct calls measured with HIP events on the
+ * job's stream: ms[0] gridding, ms[1] iterations, ms[2] R-factor monitor,
+ * ms[3] whole job. */
+int genfire_job_timing(const genfire_job* job, double ms[4]);
+
+/* Per-kernel HIP-event timing on the job's stream (off by default).
+ * kernel ids: GENFIRE_KSTAT_*; returns launches and summed milliseconds. */
+#define GENFIRE_KSTAT_GRID_VALUES 0   /* DFT value kernel, per projection launch  */
+#define GENFIRE_KSTAT_FFT_PASS 1      /* one axis pass of the 3D FFT (complex)    */
+#define GENFIRE_KSTAT_FFT_FUSED 2     /* fused forward/replace/inverse pass       */
+#define GENFIRE_KSTAT_ITERATION 3     /* one whole iteration                      */
+#define GENFIRE_KSTAT_PROJECT 4       /* projector + R sums, all projections      */
+#define GENFIRE_KSTAT_COUNT 5
+int genfire_job_set_profiling(genfire_job* job, int32_t on);
+int genfire_job_kernel_stats(const genfire_job* job, int32_t kernel, int64_t* launches,
+                             double* total_ms);
+
+/* calculate3Dprojection_RealSpaceinterp(Vol, phi, theta, psi)
+ * (src/calculate3Dprojection_RealSpaceinterp.m:1-57) with trilinear interpolation
+ * and extrapolation value 0: Vol d1 x d2 x d3 float, proj d1 x d2 float. */
+int genfire_project(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3,
+                    double phi, double theta, double psi, float* proj);
+
+/* Tian_calc_Rfactor_realspace(model, Projs, Angles) (src/Tian_calc_Rfactor_realspace.m:1-24),
+ * with the projector above: model d1 x d2 x d3, projections p1 x p2 x P (p1 <= d1,
+ * p2 <= d2), angles P x 3 column-major.  Outputs Rtotal, Rarr[P] and, if not
+ * NULL, simuProjs (p1 x p2 x P). */
+int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
+                    const float* projections, int32_t p1, int32_t p2, int32_t num_projs,
+                    const double* angles, double* Rtotal, double* Rarr, float* simuProjs);
+
+/* fftn (inverse = 0) or ifftn (inverse = 1, scaled by 1/N) of an n1 x n2 x n3
+ * interleaved-complex array in place, with this library's 3D FFT (MATLAB's
+ * fftn/ifftn at reconstruct.m:133,165 and src/my_ifft.m:11). */
+int genfire_fftn(genfire_ctx* ctx, float* data, int32_t n1, int32_t n2, int32_t n3,
+                 int32_t inverse);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GENFIRE_H */

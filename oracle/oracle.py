@@ -1,0 +1,572 @@
+"""CPU oracle for the GENFIRE v2 reconstruction hot path.
+
+TEST INFRASTRUCTURE ONLY: this module is the checker that the HIP library in
+``genfire_amd/`` is compared against.  Only ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` may import it; the product never does.
+
+It restates, function by function, the MATLAB reference (paths relative to the
+reference repository):
+
+* ``@GENFIRE_Reconstructor/fillInFourierGrid_DFT.m:85-310``  -> :func:`grid_dft`
+  (index pass in the FP32 reference order via ``genfire_oracle.c``; values as an
+  FP64 direct DFT; FP64 weighted average, :267-291)
+* ``@GENFIRE_Reconstructor/CentroSymmetricity.m:2-72``       -> :func:`centro_symmetricity`
+* ``@GENFIRE_Reconstructor/GENFIRE_Reconstructor.m:115-186`` -> :func:`delay_indicators`,
+  :func:`oversampled_size`, :func:`volumn_index`
+* ``@GENFIRE_Reconstructor/reconstruct.m:28-194``            -> :func:`reconstruct`
+* ``src/calculate3Dprojection_RealSpaceinterp.m``             -> :func:`project`
+* ``src/My_paddzero.m``, ``src/My_stripzero.m``, ``src/My_volumn_index.m``,
+  ``src/My_iffshift3_ind.m``, ``src/make_Kspace_indices.m``, ``src/my_ifft.m``
+
+Parity status per function is recorded in each docstring; "parity unpinned" marks
+what no golden vector, closed form or invariant fixes.
+
+Array convention: a MATLAB array of size (n1, n2, n3) is a numpy array of the same
+shape; MATLAB linear indices are the 0-based Fortran-order ravel index.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.fft
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load oracle/liboracle.so (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.go_rotation.argtypes = [_f64p, _f64p, _f64p, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, _f32p, _f64p]
+        L.go_rotation.restype = None
+        L.go_select.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_double,
+                                ctypes.c_int64, _i64p, _f32p, _f32p, _f32p,
+                                ctypes.POINTER(ctypes.c_int64)]
+        L.go_select.restype = ctypes.c_int64
+        L.go_tie_census.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_double,
+                                    ctypes.c_int, _i64p]
+        L.go_tie_census.restype = None
+        L.go_lu_solve.argtypes = [_f32p, _f32p, _f32p]
+        L.go_lu_solve.restype = None
+        for name in ("go_dft", "go_dft_brute"):
+            fn = getattr(L, name)
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int64, _f32p, _f32p, _f64p]
+            fn.restype = None
+        L.go_zyz.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _f64p]
+        L.go_zyz.restype = None
+        L.go_project.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
+        L.go_project.restype = None
+        _LIB = L
+    return _LIB
+
+
+# ---------------------------------------------------------------------------
+# MATLAB semantics (Appendix-A style helpers)
+# ---------------------------------------------------------------------------
+
+def matlab_round(x):
+    """MATLAB ``round``: half away from zero."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+
+def matlab_linspace(d1: float, d2: float, n: int) -> np.ndarray:
+    """MATLAB ``linspace(d1,d2,n)``: ``d1 + (0:n1).*(d2-d1)./n1`` with the endpoint
+    set exactly to d2 (MathWorks linspace.m, finite case)."""
+    n = int(math.floor(n))
+    if n < 1:
+        return np.zeros(0)
+    if n == 1:
+        return np.array([float(d2)])
+    n1 = n - 1
+    k = np.arange(n1 + 1, dtype=np.float64)
+    y = d1 + (k * (d2 - d1)) / n1
+    if d1 == d2:
+        y[:] = d1
+    else:
+        y[-1] = d2
+    return y
+
+
+def matlab_colon(a: float, d: float, b: float) -> np.ndarray:
+    """MATLAB ``a:d:b`` for non-integer steps, restated from MathWorks' published
+    colon algorithm (colonop): n = round((b-a)/d), reduced by one if a+n*d overshoots
+    b by more than tol = 2*eps*max(|a|,|b|); the right end c = a+n*d snaps to b
+    within tol; elements k <= n/2 are a+k*d, the others c-(n-k)*d, and for even n
+    the middle one is (a+c)/2.  Parity unpinned (no MATLAB here); only
+    constraintEnforcementMode 1 and 2 use it."""
+    tol = 2.0 * np.finfo(float).eps * max(abs(a), abs(b))
+    sig = math.copysign(1.0, d)
+    n = int(matlab_round((b - a) / d))
+    if sig * (a + n * d - b) > tol:
+        n -= 1
+    if n < 0:
+        return np.zeros(0)
+    c = a + n * d
+    if sig * (c - b) > -tol:
+        c = b
+    v = np.zeros(n + 1)
+    for k in range(0, n // 2 + 1):
+        v[k] = a + k * d
+        v[n - k] = c - k * d
+    if n % 2 == 0:
+        v[n // 2] = (a + c) / 2
+    return v
+
+
+def delay_indicators(mode: int) -> np.ndarray:
+    """constraintEnforcementDelayIndicators, GENFIRE_Reconstructor.m:156-165."""
+    if mode == 1:
+        return np.concatenate([matlab_colon(0.95, -0.1, -0.15), [-10.0, -10.0, -10.0],
+                               matlab_colon(-0.15, 0.1, 0.95)])
+    if mode == 2:
+        return np.concatenate([matlab_colon(0.95, -0.1, -0.15), [-10.0, -10.0]])
+    if mode == 3:
+        return np.array([-999.0] * 5)
+    raise ValueError(f"GENFIRE: ERROR! constraintEnforcementMode value {mode} not understood")
+
+
+def cutoff_schedule(num_iterations: int, num_delays: int) -> np.ndarray:
+    """iterationNumsToChangeCutoff = round(linspace(1,N,numel(delays))), reconstruct.m:103."""
+    return matlab_round(matlab_linspace(1.0, float(num_iterations), num_delays)).astype(np.int64)
+
+
+def oversampled_size(dim: int, ratio: float) -> int:
+    """n_os = round(Dim*ratio), GENFIRE_Reconstructor.m:168-169."""
+    return int(matlab_round(dim * ratio))
+
+
+def find(mask: np.ndarray) -> np.ndarray:
+    """MATLAB ``find`` on an array: ascending 0-based Fortran-order linear indices."""
+    return np.flatnonzero(np.asarray(mask).ravel(order="F"))
+
+
+def iffshift3_ind(shape, inds: np.ndarray) -> np.ndarray:
+    """src/My_iffshift3_ind.m:1-7 on 0-based indices: s = (c + ceil(n/2)) mod n per dim."""
+    s1, s2, s3 = np.unravel_index(inds, shape, order="F")
+    s1 = (s1 + (shape[0] + 1) // 2) % shape[0]
+    s2 = (s2 + (shape[1] + 1) // 2) % shape[1]
+    s3 = (s3 + (shape[2] + 1) // 2) % shape[2]
+    return np.ravel_multi_index((s1, s2, s3), shape, order="F")
+
+
+def _pad_start(big: int, small: int) -> int:
+    """0-based start of the centre-preserving window (My_paddzero.m:33-49,
+    My_stripzero.m:30-46, My_volumn_index.m:12-28 all reduce to this)."""
+    if big % 2 == 0:
+        return (big - small) // 2 if small % 2 == 0 else (big - small + 1) // 2
+    return (big - small - 1) // 2 if small % 2 == 0 else (big - small) // 2
+
+
+def paddzero(vol: np.ndarray, shape) -> np.ndarray:
+    """src/My_paddzero.m:24-59 (output single, :30)."""
+    out = np.zeros(shape, dtype=np.float32)
+    sl = tuple(slice(_pad_start(B, s), _pad_start(B, s) + s) for B, s in zip(shape, vol.shape))
+    out[sl] = vol
+    return out
+
+
+def stripzero(vol: np.ndarray, shape) -> np.ndarray:
+    """src/My_stripzero.m:22-55."""
+    sl = tuple(slice(_pad_start(B, s), _pad_start(B, s) + s) for B, s in zip(vol.shape, shape))
+    return vol[sl]
+
+
+def volumn_index(big, small) -> np.ndarray:
+    """src/My_volumn_index.m:3-34, 1-based [start end] rows like the reference."""
+    return np.array([[_pad_start(B, s) + 1, _pad_start(B, s) + s] for B, s in zip(big, small)])
+
+
+def make_kspace_indices(shape) -> np.ndarray:
+    """src/make_Kspace_indices.m:16-62: Q = sqrt(Kx.^2+Ky.^2+Kz.^2) in double with
+    Kx along dim 2 (meshgrid(vec2,vec1,vec3)), evaluated left to right."""
+    vecs = []
+    for n in shape:
+        if n % 2 == 0:
+            c = n // 2
+            vecs.append(np.arange(-c, c, dtype=np.float64) / c)
+        elif n == 1:
+            vecs.append(np.zeros(1))
+        else:
+            c = (n - 1) // 2
+            vecs.append(np.arange(-c, c + 1, dtype=np.float64) / c)
+    v1 = vecs[0][:, None, None]
+    v2 = vecs[1][None, :, None]
+    v3 = vecs[2][None, None, :]
+    return np.sqrt((v2 * v2 + v1 * v1) + v3 * v3)
+
+
+def my_ifft(k: np.ndarray) -> np.ndarray:
+    """src/my_ifft.m:10-12: fftshift(ifftn(ifftshift(k)))."""
+    return np.fft.fftshift(scipy.fft.ifftn(np.fft.ifftshift(k), workers=-1))
+
+
+# ---------------------------------------------------------------------------
+# Gridding (kernel a)
+# ---------------------------------------------------------------------------
+
+def rotation(phi: float, theta: float, psi: float, v1=(0, 0, 1), v2=(0, 1, 0), v3=(1, 0, 0)):
+    """fillInFourierGrid_DFT.m:108-118.  Returns (R_single, R_double), each (3,3)
+    with R[i, j] = row i, column j."""
+    Rf = np.zeros(9, np.float32)
+    Rd = np.zeros(9, np.float64)
+    lib().go_rotation(np.asarray(v1, np.float64), np.asarray(v2, np.float64),
+                      np.asarray(v3, np.float64), float(phi), float(theta), float(psi), Rf, Rd)
+    return Rf.reshape(3, 3, order="F"), Rd.reshape(3, 3, order="F")
+
+
+def half_ranges(n1: int, n2: int, centro: bool):
+    """fillInFourierGrid_DFT.m:122-136: (lo, len) per dimension of the k grid."""
+    if centro:
+        return (-(n1 // 2), -(n2 // 2), -(n1 // 2)), (n1 // 2 + 1, 2 * (n2 // 2) + 1, 2 * (n1 // 2) + 1)
+    return (-(n1 // 2), -(n2 // 2), -(n1 // 2)), (n1, n2, n1)
+
+
+def select(n1: int, n2: int, centro: bool, R: np.ndarray, cutoff: float):
+    """fillInFourierGrid_DFT.m:175-192 for one projection, whole-grid scan.
+    Returns (Gind, D, u, v, n_wrong); Gind ascending 0-based half-grid indices."""
+    _, ln = half_ranges(n1, n2, centro)
+    cap = ln[0] * ln[1] * ln[2]
+    # the slab |D| < cutoff holds at most a few times cutoff*area points; grow on demand
+    cap_try = min(cap, max(1 << 16, int(8 * (2 * cutoff + 2) * max(ln) ** 2)))
+    Rf = np.ascontiguousarray(np.asarray(R, np.float32).ravel(order="F"))
+    while True:
+        ind = np.empty(cap_try, np.int64)
+        D = np.empty(cap_try, np.float32)
+        u = np.empty(cap_try, np.float32)
+        v = np.empty(cap_try, np.float32)
+        wrong = ctypes.c_int64(0)
+        m = lib().go_select(n1, n2, int(bool(centro)), Rf, float(cutoff), cap_try, ind, D, u, v,
+                            ctypes.byref(wrong))
+        if m >= 0:
+            return ind[:m].copy(), D[:m].copy(), u[:m].copy(), v[:m].copy(), int(wrong.value)
+        cap_try = min(cap, cap_try * 4)
+
+
+def tie_census(n1: int, n2: int, centro: bool, R: np.ndarray, cutoff: float, ulps: int = 1):
+    out = np.zeros(2, np.int64)
+    Rf = np.ascontiguousarray(np.asarray(R, np.float32).ravel(order="F"))
+    lib().go_tie_census(n1, n2, int(bool(centro)), Rf, float(cutoff), int(ulps), out)
+    return int(out[0]), int(out[1])
+
+
+def lu_solve(R: np.ndarray, b) -> np.ndarray:
+    x = np.zeros(3, np.float32)
+    lib().go_lu_solve(np.ascontiguousarray(np.asarray(R, np.float32).ravel(order="F")),
+                      np.asarray(b, np.float32), x)
+    return x
+
+
+def dft64(proj: np.ndarray, un: np.ndarray, vn: np.ndarray, brute: bool = False) -> np.ndarray:
+    """FP64 direct DFT of one projection at in-plane points (un, vn) (:236-248)."""
+    d1, d2 = proj.shape
+    P = np.ascontiguousarray(np.asarray(proj, np.float32).ravel(order="F"))
+    un = np.ascontiguousarray(un, np.float32)
+    vn = np.ascontiguousarray(vn, np.float32)
+    out = np.zeros(2 * len(un), np.float64)
+    fn = lib().go_dft_brute if brute else lib().go_dft
+    fn(d1, d2, P, len(un), un, vn, out)
+    return out[0::2] + 1j * out[1::2]
+
+
+@dataclass
+class GridResult:
+    n1: int
+    n2: int
+    half_shape: tuple
+    master_ind: np.ndarray      # 0-based half-grid linear index per point
+    master_proj: np.ndarray     # projection number per point (0-based)
+    master_dist: np.ndarray     # D per point (single)
+    master_un: np.ndarray       # u/n1 per point (single)
+    master_vn: np.ndarray       # v/n2 per point (single)
+    master_val: np.ndarray      # F per point (complex128 from the FP64 DFT)
+    hits: np.ndarray            # per half-grid voxel hit count (int32)
+    FS: np.ndarray              # half grid after averaging (complex128)
+    measuredK: np.ndarray       # centred full grid, complex64 (runGridding single cast)
+    n_wrong: int = 0
+    extra: dict = field(default_factory=dict)
+
+
+def weighted_average(master_ind, master_dist, master_val, total: int):
+    """fillInFourierGrid_DFT.m:267-291 with FP64 accumulation:
+    w = 1/D (0 where D < 1e-7); FS = sum(w F)/sum(w) where sum(w) > 0;
+    then FS(exact) = F_exact assigned in list order (the last one wins)."""
+    dist = np.asarray(master_dist, np.float32)
+    index_exact = dist.astype(np.float64) < 1e-7          # :271
+    w = np.zeros(len(dist), np.float32)
+    w[~index_exact] = np.float32(1.0) / dist[~index_exact]  # :275, single reciprocal
+    w = w.astype(np.float64)
+    vals = np.asarray(master_val, np.complex128)
+    FS = np.zeros(total, np.complex128)
+    np.add.at(FS, master_ind, vals * w)                     # :285
+    sw = np.zeros(total, np.float64)
+    np.add.at(sw, master_ind, w)                            # :286
+    interp = sw > 0                                         # :287
+    FS[interp] = FS[interp] / sw[interp]                    # :289
+    ex_ind = master_ind[index_exact]
+    FS[ex_ind] = vals[index_exact]                          # :291 (numpy: last write wins)
+    return FS
+
+
+def centro_symmetricity(FS: np.ndarray, n1: int, n_k1: int, n2: int, dtype=np.complex64):
+    """CentroSymmetricity.m:2-72, restated literally with reshapes and flips
+    (fliplr = dim 2, the permute/flipud/permute = dim 3, flipud = dim 1)."""
+    out = np.zeros((n1, n2, n1), dtype=dtype)
+
+    def flip_all(a):
+        a = a[:, ::-1, :]      # fliplr
+        a = a[:, :, ::-1]      # permute(flipud(permute(.,[3 1 2])),[2 3 1])
+        return a[::-1, :, :]   # flipud
+
+    if n2 % 2 == 1:
+        if n1 % 2 == 1:
+            half = FS.reshape(n_k1 + 1, n2, n1, order="F")
+            out[: n_k1 + 1] = half
+            out[n_k1 + 1:] = np.conj(flip_all(half[:-1]))
+        else:
+            ext = FS.reshape(n_k1 + 1, n2, n1 + 1, order="F")
+            out[: n_k1 + 1] = ext[:, :, :-1]
+            out[n_k1 + 1:] = np.conj(flip_all(ext[1:-1])[:, :, :-1])
+    else:
+        if n1 % 2 == 1:
+            ext = FS.reshape(n_k1 + 1, n2 + 1, n1, order="F")
+            out[: n_k1 + 1] = ext[:, :-1, :]
+            out[n_k1 + 1:] = np.conj(flip_all(ext[:-1])[:, :-1, :])
+        else:
+            ext = FS.reshape(n_k1 + 1, n2 + 1, n1 + 1, order="F")
+            out[: n_k1 + 1] = ext[:, :-1, :-1]
+            out[n_k1 + 1:] = np.conj(flip_all(ext[1:-1])[:, :-1, :-1])
+    return out
+
+
+def grid_dft(projections: np.ndarray, angles: np.ndarray, n1: int, n2: int, cutoff: float,
+             centro: bool = True, vectors=((0, 0, 1), (0, 1, 0), (1, 0, 0)),
+             values: bool = True, value_fn=None) -> GridResult:
+    """fillInFourierGrid_DFT.m:85-310 + runGridding's single cast (GENFIRE_Reconstructor.m:182).
+
+    projections: (Dim1, Dim2, P) array; angles: (P, 3) [phi theta psi] degrees.
+    ``value_fn(p, proj, un, vn)`` may replace the FP64 DFT (e.g. to feed the GPU's
+    point values through the same averaging); ``values=False`` skips values."""
+    P = projections.shape[2]
+    lo, ln = half_ranges(n1, n2, centro)
+    total = ln[0] * ln[1] * ln[2]
+    inds, dists, uns, vns, vals, projs = [], [], [], [], [], []
+    n_wrong = 0
+    for p in range(P):
+        Rf, _ = rotation(angles[p, 0], angles[p, 1], angles[p, 2], *vectors)
+        ind, D, u, v, wr = select(n1, n2, centro, Rf, cutoff)
+        n_wrong += wr
+        un = (u / np.float32(n1)).astype(np.float32)   # :237, single / n1
+        vn = (v / np.float32(n2)).astype(np.float32)   # :238
+        inds.append(ind); dists.append(D); uns.append(un); vns.append(vn)
+        projs.append(np.full(len(ind), p, np.int32))
+        if values:
+            proj = np.asarray(projections[:, :, p], np.float32)   # gpuArray product is single
+            if value_fn is not None:
+                vals.append(np.asarray(value_fn(p, proj, un, vn), np.complex128))
+            else:
+                vals.append(dft64(proj, un, vn))
+    master_ind = np.concatenate(inds) if inds else np.zeros(0, np.int64)
+    master_dist = np.concatenate(dists) if dists else np.zeros(0, np.float32)
+    hits = np.bincount(master_ind, minlength=total).astype(np.int32)
+    res = GridResult(n1=n1, n2=n2, half_shape=tuple(ln), master_ind=master_ind,
+                     master_proj=np.concatenate(projs) if projs else np.zeros(0, np.int32),
+                     master_dist=master_dist,
+                     master_un=np.concatenate(uns) if uns else np.zeros(0, np.float32),
+                     master_vn=np.concatenate(vns) if vns else np.zeros(0, np.float32),
+                     master_val=np.concatenate(vals) if vals else np.zeros(0, np.complex128),
+                     hits=hits, FS=None, measuredK=None, n_wrong=n_wrong)
+    if values:
+        FS = weighted_average(master_ind, master_dist, res.master_val, total)
+        res.FS = FS
+        if centro:
+            res.measuredK = centro_symmetricity(FS, n1, n1 // 2, n2, dtype=np.complex128).astype(np.complex64)
+        else:
+            res.measuredK = FS.reshape(n1, n2, n1, order="F").astype(np.complex64)
+    return res
+
+
+def rec_ifft(measuredK: np.ndarray, dim1: int, dim2: int) -> np.ndarray:
+    """recIFFT = My_stripzero(real(my_ifft(measuredK)), [Dim1 Dim2 Dim1]), :304."""
+    return stripzero(np.real(my_ifft(measuredK)), (dim1, dim2, dim1))
+
+
+# ---------------------------------------------------------------------------
+# Projector and R-factor (kernel c)
+# ---------------------------------------------------------------------------
+
+def zyz(phi: float, theta: float, psi: float) -> np.ndarray:
+    R = np.zeros(9)
+    lib().go_zyz(float(phi), float(theta), float(psi), R)
+    return R.reshape(3, 3, order="F")
+
+
+def project(vol: np.ndarray, phi: float, theta: float, psi: float) -> np.ndarray:
+    """calculate3Dprojection_RealSpaceinterp(Vol,phi,theta,psi) with trilinear
+    interpolation and extrapolation value 0 (FP64).  The reference calls
+    interp3(...,'cubic',0) (:45,51); the trilinear interpolant is the one this
+    build computes, the cubic kernel is parity unpinned."""
+    d1, d2, d3 = vol.shape
+    R = zyz(phi, theta, psi)
+    out = np.zeros(d1 * d2, np.float64)
+    lib().go_project(np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F")),
+                     d1, d2, d3, np.ascontiguousarray(R.ravel(order="F")), out)
+    return out.reshape(d1, d2, order="F")
+
+
+def rfactors(model: np.ndarray, projections: np.ndarray, angles: np.ndarray):
+    """R-factor monitor body, reconstruct.m:171-178: per projection
+    R1 = sum|back-pj|/sum|pj| and R2 = ||back-pj||_F/||pj||_F."""
+    P = projections.shape[2]
+    R1 = np.zeros(P)
+    R2 = np.zeros(P)
+    for i in range(P):
+        pj = np.asarray(projections[:, :, i], np.float64)
+        back = project(model, *angles[i])
+        R1[i] = np.sum(np.abs(back - pj)) / np.sum(np.abs(pj))
+        R2[i] = np.linalg.norm(back - pj) / np.linalg.norm(pj)
+    return R1, R2
+
+
+def rfactor_posthoc(model: np.ndarray, projections: np.ndarray, angles: np.ndarray):
+    """src/Tian_calc_Rfactor_realspace.m:1-24 with the projector defined (the
+    reference calls a missing ..._RealSpaceinterpZYX, :15): mean of
+    sum||back|-|pj|| / sum|pj| after cropping the projection to the pj size."""
+    P = projections.shape[2]
+    R = np.zeros(P)
+    sims = np.zeros(projections.shape, np.float32)
+    for i in range(P):
+        pj = np.asarray(projections[:, :, i], np.float64)
+        back = stripzero(project(model, *angles[i]), pj.shape)
+        R[i] = np.sum(np.abs(np.abs(back) - np.abs(pj))) / np.sum(np.abs(pj))
+        sims[:, :, i] = back
+    return float(np.mean(R)), R, sims
+
+
+# ---------------------------------------------------------------------------
+# Iteration (kernel b)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class ReconResult:
+    reconstruction: np.ndarray
+    errK: np.ndarray
+    errR: np.ndarray | None = None
+    Rarr_record: np.ndarray | None = None
+    Rarr2_record: np.ndarray | None = None
+    last_iterate: np.ndarray | None = None
+
+
+def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, dim1: int,
+                dim2: int, num_iterations: int, mode: int = 3, positivity: bool = True,
+                support_on: bool = True, algorithm: str = "genfire", precision: str = "fp32",
+                initial_object: np.ndarray | None = None, monitor_R: bool = False,
+                loop_length: int = 10, projections=None, angles=None) -> ReconResult:
+    """reconstruct.m:28-194 (GENFIRE, algorithm='genfire').
+
+    algorithm='dr' is this build's Douglas-Rachford variant composed of the same two
+    projections (P_R = positivity/support, :126-131; P_F = replacement + real(ifftn),
+    :163-165): y = 2 P_R(x) - x, x <- x + P_F(y) - P_R(x); errK is evaluated on
+    fftn(y), the input of P_F, exactly as GENFIRE evaluates it on fftn(P_R(x)); the
+    reconstruction is P_R(x) of the best iteration.  The reference has no DR (its
+    README claims one), so the DR path is parity unpinned against the reference and
+    pinned only by its fixed-point and convex-feasibility properties.
+
+    precision: 'fp32' (MATLAB single, as when the support constraint is on) or 'fp64'.
+    """
+    cdt = np.complex64 if precision == "fp32" else np.complex128
+    rdt = np.float32 if precision == "fp32" else np.float64
+    shape = measuredK.shape
+    M = np.asarray(measuredK, cdt)
+    padded_support = paddzero(np.asarray(support, np.float32), shape).astype(rdt)   # :34
+    Q = make_kspace_indices(shape)                                                    # :36
+    measured = (M != 0) & np.asarray(mask, bool)
+    indicators = np.zeros(shape)                                                      # :39
+    indicators[measured] = 1 - Q[measured]                                            # :42
+    delays = delay_indicators(mode)
+    if initial_object is None:
+        x = np.zeros(shape, rdt)                                                      # :77
+    else:
+        x = paddzero(np.asarray(initial_object, np.float32), shape).astype(rdt)
+    errK = np.zeros(num_iterations, np.float32)                                       # :95
+    err_ind = find(measured)                                                          # :99
+    err_sh = iffshift3_ind(shape, err_ind)                                            # :100
+    Mf = M.ravel(order="F")
+    M_err = Mf[err_ind]
+    den = np.sum(np.abs(M_err).astype(np.float64))
+    sched = cutoff_schedule(num_iterations, len(delays))                              # :103
+    cur = 0
+    S = np.fft.ifftshift(padded_support)                                              # :109
+    x = np.fft.ifftshift(x)                                                           # :110
+    nR = num_iterations // loop_length if monitor_R else 0
+    errR = np.zeros(nR)
+    Rrec = np.zeros((projections.shape[2] if monitor_R else 0, nR))
+    R2rec = np.zeros_like(Rrec)
+    vol_ind = volumn_index(shape, (dim1, dim2, dim1))
+    best_err = 1e30
+    recon = None
+    c_sh = None
+    c_val = None
+    for it in range(1, num_iterations + 1):                                          # :117
+        if cur < len(sched) and it == sched[cur]:                                     # :118
+            cur = int(np.nonzero(sched == it)[0][-1])
+            c_ind = find((indicators > delays[cur]) & measured)                       # :120
+            c_sh = iffshift3_ind(shape, c_ind)                                        # :121
+            c_val = Mf[c_ind]
+            cur += 1
+            best_err = 1e30                                                           # :123
+        if algorithm == "genfire":
+            if positivity:
+                x = np.where(x < 0, rdt(0), x)                                        # :127
+            if support_on:
+                x = x * S                                                             # :130
+            r = x
+            y = x
+        else:
+            r = x
+            if positivity:
+                r = np.where(r < 0, rdt(0), r)
+            if support_on:
+                r = r * S
+            y = (rdt(2) * r - x).astype(rdt)
+        k = scipy.fft.fftn(y.astype(cdt), workers=-1)                                 # :133
+        kf = k.reshape(-1, order="F")
+        e = np.abs(np.abs(kf[err_sh]).astype(np.float64) - np.abs(M_err).astype(np.float64))
+        errK[it - 1] = np.float32(np.sum(e) / den)                                    # :135
+        if errK[it - 1] < best_err:                                                   # :149-153
+            best_err = errK[it - 1]
+            recon = r.copy()
+        kf = kf.copy()
+        kf[c_sh] = c_val                                                              # :163
+        k = kf.reshape(shape, order="F")
+        pf = np.real(scipy.fft.ifftn(k, workers=-1)).astype(rdt)                      # :165
+        x = pf if algorithm == "genfire" else (x + pf - r).astype(rdt)
+        if monitor_R and it % loop_length == 0:                                       # :168
+            model = np.fft.fftshift(pf)
+            sl = tuple(slice(a - 1, b) for a, b in vol_ind)
+            model = model[sl]
+            R1, R2 = rfactors(model, projections, angles)
+            j = it // loop_length - 1
+            errR[j] = np.mean(R1)
+            Rrec[:, j] = R1
+            R2rec[:, j] = R2
+    reconstruction = stripzero(np.fft.fftshift(recon), (dim1, dim2, dim1))            # :192
+    return ReconResult(reconstruction=np.asarray(reconstruction, np.float32), errK=errK,
+                       errR=errR if monitor_R else None,
+                       Rarr_record=Rrec if monitor_R else None,
+                       Rarr2_record=R2rec if monitor_R else None,
+                       last_iterate=x)
