@@ -172,7 +172,9 @@ struct genfire_job {
     bool gridded = false, have_mask = false;
 
     // iteration
-    DevBuf x, K, S, flags, best, pfbuf, err_part, errK, den_tmp, den, recon;
+    DevBuf x, K, S, best, pfbuf, err_part, errK, den_tmp, den, recon;
+    DevBuf csr_cnt, csr_off, csr_meta, csr_val;
+    int64_t nmeas = 0;
     DevBuf vol, Rmat, rsums, rout;
     std::vector<float> errK_h, errR_h;
     std::vector<double> Rarr_h, Rarr2_h;
@@ -374,12 +376,12 @@ void do_reconstruct(genfire_job* j)
     j->x.alloc(sizeof(float) * nf);
     j->K.alloc(sizeof(float2) * nf);
     j->S.alloc(nf);
-    j->flags.alloc(nf);
     j->best.alloc(sizeof(float) * nf);
     if (dr && monitor) j->pfbuf.alloc(sizeof(float) * nf);
     IterArgs ia{};
     ia.n1 = n1; ia.n2 = n2; ia.n3 = n3;
-    const int nparts = iter_fused_blocks(ia);
+    GF_CHECK(n1 <= 2048 && n3 <= 2048, GENFIRE_ERR_UNSUPPORTED, "GENFIRE: grid edge above 2048");
+    const int64_t nparts = iter_fused_partials(ia);
     j->err_part.alloc(sizeof(double) * nparts);
     j->errK.alloc(sizeof(float) * std::max(N, 1));
     const int ntmp = 1024;
@@ -410,11 +412,21 @@ void do_reconstruct(genfire_job* j)
     const std::vector<int> sched = cutoff_schedule(N, (int)j->delays.size());
     GF_CHECK(N >= 1 && sched.back() == N, GENFIRE_ERR_ARG, "GENFIRE: numIterations must be >= 1");
 
-    // errK denominator: sum |M(errInd)| (reconstruct.m:135), mask fixed for the run
-    iter_flags(j->M.as<float2>(), j->have_mask ? j->mask.as<uint8_t>() : nullptr, n1, n2, n3,
-               j->delays[0], j->flags.as<uint8_t>(), st);
-    iter_den(j->M.as<float2>(), j->flags.as<uint8_t>(), nf, j->den_tmp.as<double>(), ntmp,
-             j->den.as<double>(), st);
+    // measured voxels (errInd, reconstruct.m:99) as per-z-pencil lists
+    const int64_t npen = (int64_t)n1 * n2;
+    const uint8_t* mask = j->have_mask ? j->mask.as<uint8_t>() : nullptr;
+    j->csr_cnt.alloc(sizeof(uint32_t) * npen);
+    j->csr_off.alloc(sizeof(int64_t) * (npen + 1));
+    csr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
+    grid_index_scan(j->csr_cnt.as<uint32_t>(), npen, j->csr_off.as<int64_t>(), j->csr_off.as<int64_t>() + npen, st);
+    GF_HIP(hipMemcpyAsync(&j->nmeas, j->csr_off.as<int64_t>() + npen, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    j->csr_meta.alloc(sizeof(uint32_t) * std::max<int64_t>(j->nmeas, 1));
+    j->csr_val.alloc(sizeof(float2) * std::max<int64_t>(j->nmeas, 1));
+    csr_fill(j->M.as<float2>(), mask, n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(),
+             j->csr_val.as<float2>(), st);
+    // errK denominator: sum |M(errInd)| (reconstruct.m:135)
+    csr_den(j->csr_val.as<float2>(), j->nmeas, j->den_tmp.as<double>(), ntmp, j->den.as<double>(), st);
     double den = 0.0;
     GF_HIP(hipMemcpyAsync(&den, j->den.p, sizeof(double), hipMemcpyDeviceToHost, st));
     GF_HIP(hipStreamSynchronize(st));
@@ -428,8 +440,9 @@ void do_reconstruct(genfire_job* j)
     ia.x = j->x.as<float>();
     ia.K = j->K.as<float2>();
     ia.S = j->S.as<uint8_t>();
-    ia.M = j->M.as<float2>();
-    ia.flags = j->flags.as<uint8_t>();
+    ia.csr_off = j->csr_off.as<int64_t>();
+    ia.csr_meta = j->csr_meta.as<uint32_t>();
+    ia.csr_val = j->csr_val.as<float2>();
     ia.err_part = j->err_part.as<double>();
     ia.positivity = j->prm.constraintPositivity ? 1 : 0;
     ia.support = j->prm.constraintSupport ? 1 : 0;
@@ -442,8 +455,7 @@ void do_reconstruct(genfire_job* j)
             size_t last = cur;
             for (size_t q = cur; q < sched.size(); ++q)
                 if (sched[q] == it) last = q;
-            iter_flags(j->M.as<float2>(), j->have_mask ? j->mask.as<uint8_t>() : nullptr, n1, n2, n3,
-                       j->delays[last], j->flags.as<uint8_t>(), st);
+            csr_enforce(n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(), j->delays[last], st);
             cur = last + 1;
         }
         // The schedule always ends with a change at it == N, which resets bestErr

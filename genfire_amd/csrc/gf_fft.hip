@@ -124,6 +124,161 @@ constexpr int pick_radix(int rem)
     return rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : rem % 5 == 0 ? 5 : rem;
 }
 
+// 16 = 4 x 4: X[c + 4b] = sum_a W4^(ab) W16^(ac) sum_m x[a + 4m] W4^(mc).
+template <bool INV> struct Dft<16, INV> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        float2 t[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            float2 q[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
+            Dft<4, INV>::run(q);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) t[a][c] = q[c];
+        }
+        // W16^(ac), a, c in 1..3 (forward sign -)
+        const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+        const float c2 = 0.70710678118654752440f;
+        const float2 w1 = make_float2(c1, INV ? s1 : -s1);
+        const float2 w2 = make_float2(c2, INV ? c2 : -c2);
+        const float2 w3 = make_float2(s1, INV ? c1 : -c1);
+        const float2 w6 = make_float2(-c2, INV ? c2 : -c2);
+        const float2 w9 = make_float2(-c1, INV ? -s1 : s1);
+        t[1][1] = cmul(t[1][1], w1); t[1][2] = cmul(t[1][2], w2); t[1][3] = cmul(t[1][3], w3);
+        t[2][1] = cmul(t[2][1], w2); t[2][2] = mul_mi<INV>(t[2][2]); t[2][3] = cmul(t[2][3], w6);
+        t[3][1] = cmul(t[3][1], w3); t[3][2] = cmul(t[3][2], w6); t[3][3] = cmul(t[3][3], w9);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float2 q[4] = {t[0][c], t[1][c], t[2][c], t[3][c]};
+            Dft<4, INV>::run(q);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) v[c + 4 * b] = q[b];
+        }
+    }
+};
+
+struct Tile {
+    int64_t base, ps, es;
+    int valid;
+};
+
+// Tile t of TP pencils along `axis`: axis 0 pencils are contiguous rows; axes 1
+// and 2 take TP consecutive x-positions at one (z) or (y) coordinate.
+__device__ __forceinline__ Tile tile_of(int axis, int n1, int n2, int n3, int TP, int64_t t)
+{
+    Tile g;
+    if (axis == 0) {
+        const int64_t np = (int64_t)n2 * n3;
+        const int64_t first = t * TP;
+        g.base = first * n1; g.ps = n1; g.es = 1;
+        g.valid = (int)min((int64_t)TP, np - first);
+    } else {
+        const int tpr = (n1 + TP - 1) / TP;
+        const int64_t outer = t / tpr;
+        const int i1 = (int)(t % tpr) * TP;
+        g.ps = 1;
+        if (axis == 1) { g.base = i1 + (int64_t)n1 * n2 * outer; g.es = n1; }
+        else { g.base = i1 + (int64_t)n1 * outer; g.es = (int64_t)n1 * n2; }
+        g.valid = min(TP, n1 - i1);
+    }
+    return g;
+}
+
+__host__ __device__ constexpr int64_t ntiles(int axis, int n1, int n2, int n3, int TP)
+{
+    return axis == 0 ? ((int64_t)n2 * n3 + TP - 1) / TP
+                     : (int64_t)((n1 + TP - 1) / TP) * (axis == 1 ? n3 : n2);
+}
+
+enum Mode { kPlain = 0, kFirst = 1, kFused = 2, kLast = 3 };
+
+struct PassArgs {
+    IterArgs it;
+    float2* data;        // plain pass data (in place)
+    const float2* tw;    // exp(-2 pi i m / N), m < N
+    int axis;
+};
+
+__device__ __forceinline__ float pr_apply(float v, uint8_t s, int positivity, int support)
+{
+    if (positivity && v < 0.0f) v = 0.0f;   // x(x<0) = 0   (reconstruct.m:127)
+    if (support) v = v * (float)s;          // x .* S       (:130)
+    return v;
+}
+
+// Element load of a pass (mode-specific): complex spectrum, or the iterate through
+// P_R (GENFIRE: P_R(x); DR: 2 P_R(x) - x) with the optional best-copy of P_R(x).
+template <int MODE>
+__device__ __forceinline__ float2 pass_load(const PassArgs& a, float2* K, int64_t addr)
+{
+    if (MODE == kFirst) {
+        const IterArgs& it = a.it;
+        const float xv = it.x[addr];
+        const float r = pr_apply(xv, it.S[addr], it.positivity, it.support);
+        if (it.best) it.best[addr] = r;
+        return make_float2(it.dr ? (2.0f * r - xv) : r, 0.0f);
+    }
+    return K[addr];
+}
+
+// Element store of a pass: complex (scaled), or real(ifftn) folded into the
+// iterate (GENFIRE: x = P_F; DR: x = x + P_F - P_R(x)).
+template <int MODE>
+__device__ __forceinline__ void pass_store(const PassArgs& a, float2* K, int64_t addr, float2 val, float scale)
+{
+    if (MODE == kLast) {
+        const IterArgs& it = a.it;
+        const float pf = val.x * scale;                 // real(ifftn(k)) (:165)
+        if (it.pf_out) it.pf_out[addr] = pf;
+        if (it.dr) {
+            const float xv = it.x[addr];
+            const float r = pr_apply(xv, it.S[addr], it.positivity, it.support);
+            it.x[addr] = (xv + pf) - r;
+        } else {
+            it.x[addr] = pf;
+        }
+    } else {
+        K[addr] = scale != 1.0f ? cscale(val, scale) : val;
+    }
+}
+
+// LDS column swizzle of the register kernel: element `col` of pencil `p` lives at
+// col ^ (((col >> 3) ^ p) & 15).  The 16 lanes an LDS access serves together then
+// fall on 16 distinct bank pairs for every pattern used here: 16 consecutive
+// columns of one pencil, the radix-8 Stockham writes 8b+q and 64m+k+8q, and one
+// column of 16 different pencils (rows N*8 bytes apart would otherwise share a bank).
+__device__ __forceinline__ int swz(int col, int p) { return col ^ (((col >> 3) ^ p) & 15); }
+
+// errK numerator (reconstruct.m:135) and Fourier replacement (:163) on the
+// natural-order spectrum of the tile's z-pencils [x0, x0+valid) at row y.  The
+// measured voxels come from the per-pencil list (meta = z | x<<11 | enforced<<31).
+template <bool SWZ>
+__device__ __forceinline__ double csr_replace(float2* lds, int N, const IterArgs& it, int x0, int y,
+                                              int valid, int tid, int nthreads)
+{
+    const int64_t r0 = (int64_t)x0 + (int64_t)it.n1 * y;
+    const int64_t beg = it.csr_off[r0], end = it.csr_off[r0 + valid];
+    double err = 0.0;
+    for (int64_t e = beg + tid; e < end; e += nthreads) {
+        const uint32_t m = it.csr_meta[e];
+        const int z = (int)(m & 0x7FFu), x = (int)((m >> 11) & 0x7FFu);
+        float2* slot = lds + (x - x0) * N + (SWZ ? swz(z, x - x0) : z);
+        const float2 kv = *slot, mv = it.csr_val[e];
+        err += (double)fabsf(sqrtf(kv.x * kv.x + kv.y * kv.y) - sqrtf(mv.x * mv.x + mv.y * mv.y));
+        if (m >> 31) *slot = mv;
+    }
+    return err;
+}
+
+__device__ __forceinline__ void wave_partial(double err, double* out, int64_t slot)
+{
+    for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
+    if ((threadIdx.x & 63) == 0) out[slot] = err;
+}
+
+// ============================================================================
+// Generic LDS kernel (any length with factors 2,3,5): every stage in LDS.
+// ============================================================================
 constexpr int tile_pencils(int N) { return N <= 256 ? 32 : (N <= 512 ? 16 : 8); }
 
 // One Stockham stage of radix R on every pencil of the tile; LS = product of the
@@ -181,73 +336,20 @@ template <int N, int TP, bool INV, int LS> struct Stages<N, TP, INV, LS, 1> {
     static __device__ __forceinline__ void run(float2*, const float2*) {}
 };
 
-struct Tile {
-    int64_t base, ps, es;
-    int valid;
-};
-
-__device__ __forceinline__ Tile tile_of(int axis, int n1, int n2, int n3, int TP, int64_t t)
-{
-    Tile g;
-    if (axis == 0) {
-        const int64_t np = (int64_t)n2 * n3;
-        const int64_t first = t * TP;
-        g.base = first * n1; g.ps = n1; g.es = 1;
-        g.valid = (int)min((int64_t)TP, np - first);
-    } else {
-        const int tpr = (n1 + TP - 1) / TP;
-        const int64_t outer = t / tpr;
-        const int i1 = (int)(t % tpr) * TP;
-        g.ps = 1;
-        if (axis == 1) { g.base = i1 + (int64_t)n1 * n2 * outer; g.es = n1; }
-        else { g.base = i1 + (int64_t)n1 * outer; g.es = (int64_t)n1 * n2; }
-        g.valid = min(TP, n1 - i1);
-    }
-    return g;
-}
-
-enum Mode { kPlain = 0, kFirst = 1, kFused = 2, kLast = 3 };
-
-struct PassArgs {
-    IterArgs it;
-    float2* data;        // plain pass data (in place)
-    const float2* tw;    // exp(-2 pi i m / N), m < N
-    int axis;
-};
-
-__device__ __forceinline__ float pr_apply(float v, uint8_t s, int positivity, int support)
-{
-    if (positivity && v < 0.0f) v = 0.0f;   // x(x<0) = 0   (reconstruct.m:127)
-    if (support) v = v * (float)s;          // x .* S       (:130)
-    return v;
-}
-
 template <int N, int MODE, bool INV>
 __global__ __launch_bounds__(kFT) void k_fft(PassArgs a)
 {
     constexpr int TP = tile_pencils(N);
     extern __shared__ float2 lds[];
     const IterArgs& it = a.it;
-    const int n1 = it.n1, n2 = it.n2, n3 = it.n3;
-    const Tile g = tile_of(a.axis, n1, n2, n3, TP, blockIdx.x);
+    const Tile g = tile_of(a.axis, it.n1, it.n2, it.n3, TP, blockIdx.x);
     float2* __restrict__ K = (MODE == kPlain) ? a.data : it.K;
 
-    // ---- load ----
     for (int idx = threadIdx.x; idx < TP * N; idx += kFT) {
         const int p = a.axis == 0 ? idx / N : idx % TP;
         const int e = a.axis == 0 ? idx % N : idx / TP;
         float2 val = make_float2(0.0f, 0.0f);
-        if (p < g.valid) {
-            const int64_t addr = g.base + p * g.ps + e * g.es;
-            if (MODE == kFirst) {
-                const float xv = it.x[addr];
-                const float r = pr_apply(xv, it.S[addr], it.positivity, it.support);
-                if (it.best) it.best[addr] = r;
-                val.x = it.dr ? (2.0f * r - xv) : r;
-            } else {
-                val = K[addr];
-            }
-        }
+        if (p < g.valid) val = pass_load<MODE>(a, K, g.base + p * g.ps + e * g.es);
         lds[p * N + e] = val;
     }
     __syncthreads();
@@ -255,51 +357,160 @@ __global__ __launch_bounds__(kFT) void k_fft(PassArgs a)
     Stages<N, TP, INV, 1, N>::run(lds, a.tw);
 
     if (MODE == kFused) {
-        // errK numerator (reconstruct.m:135) and Fourier replacement (:163)
-        double err = 0.0;
-        for (int idx = threadIdx.x; idx < TP * N; idx += kFT) {
-            const int p = idx % TP, e = idx / TP;
-            if (p >= g.valid) continue;
-            const int64_t addr = g.base + p * g.ps + e * g.es;
-            const uint8_t f = it.flags[addr];
-            if (f) {
-                const float2 m = it.M[addr];
-                const float2 kv = lds[p * N + e];
-                if (f & kFlagMeasured)
-                    err += (double)fabsf(sqrtf(kv.x * kv.x + kv.y * kv.y) - sqrtf(m.x * m.x + m.y * m.y));
-                if (f & kFlagEnforced) lds[p * N + e] = m;
-            }
-        }
-        // one partial per wave: no LDS beyond the tile (which fills 64 KiB at N=512)
-        for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
-        if ((threadIdx.x & 63) == 0) it.err_part[(int64_t)blockIdx.x * (kFT / 64) + (threadIdx.x >> 6)] = err;
+        const int tpr = (it.n1 + TP - 1) / TP;
+        const int x0 = (int)(blockIdx.x % tpr) * TP, y = (int)(blockIdx.x / tpr);
+        const double err = csr_replace<false>(lds, N, it, x0, y, g.valid, threadIdx.x, kFT);
+        wave_partial(err, it.err_part, (int64_t)blockIdx.x * (kFT / 64) + (threadIdx.x >> 6));
         __syncthreads();
         Stages<N, TP, true, 1, N>::run(lds, a.tw);
     }
 
-    // ---- store ----
     const float scale = (INV || MODE == kFused) ? 1.0f / (float)N : 1.0f;
     for (int idx = threadIdx.x; idx < TP * N; idx += kFT) {
         const int p = a.axis == 0 ? idx / N : idx % TP;
         const int e = a.axis == 0 ? idx % N : idx / TP;
         if (p >= g.valid) continue;
-        const int64_t addr = g.base + p * g.ps + e * g.es;
-        float2 val = lds[p * N + e];
-        if (MODE == kLast) {
-            const float pf = val.x * scale;           // real(ifftn(k)) (:165)
-            if (it.pf_out) it.pf_out[addr] = pf;
-            if (it.dr) {
-                const float xv = it.x[addr];
-                const float r = pr_apply(xv, it.S[addr], it.positivity, it.support);
-                it.x[addr] = (xv + pf) - r;
-            } else {
-                it.x[addr] = pf;
-            }
-        } else {
-            if (scale != 1.0f) val = cscale(val, scale);
-            K[addr] = val;
-        }
+        pass_store<MODE>(a, K, g.base + p * g.ps + e * g.es, lds[p * N + e], scale);
     }
+}
+
+// ============================================================================
+// Register kernel (power-of-two lengths 64..1024).  A pencil is owned by
+// T = N/E lanes holding E elements each; every radix stage is computed in
+// registers and only the two exchanges between the three stages go through
+// LDS.  The first stage's inputs and the last stage's outputs are the elements
+// j + T*i + r*N/R of the lane's pencil, so global loads and stores go straight
+// from and to registers: along axis 0 consecutive lanes take consecutive
+// elements, along axes 1 and 2 consecutive lanes take consecutive pencils
+// (TP * 8 bytes contiguous per row).
+// ============================================================================
+template <int N> struct RegPlan { static constexpr bool ok = false; };
+template <> struct RegPlan<64> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1, TP = 32; };
+template <> struct RegPlan<128> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 2, TP = 16; };
+template <> struct RegPlan<256> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 4, TP = 16; };
+template <> struct RegPlan<512> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 8, TP = 16; };
+template <> struct RegPlan<1024> { static constexpr bool ok = true; static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 4, TP = 8; };
+
+template <int N> constexpr int reg_threads() { return RegPlan<N>::TP * (N / RegPlan<N>::E); }
+template <int N> constexpr int reg_last_radix() { return RegPlan<N>::R2 > 1 ? RegPlan<N>::R2 : RegPlan<N>::R1; }
+
+template <int N, int E, int R, int LS, bool INV>
+__device__ __forceinline__ void reg_stage(float2 (&v)[E], int j, const float2* __restrict__ tw)
+{
+    constexpr int T = N / E;
+#pragma unroll
+    for (int i = 0; i < E / R; ++i) {
+        if (LS > 1) {
+            const int k = (j + T * i) & (LS - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = tw[r * k * (N / (LS * R))];
+                if (INV) w.y = -w.y;
+                v[i * R + r] = cmul(v[i * R + r], w);
+            }
+        }
+        Dft<R, INV>::run(&v[i * R]);
+    }
+}
+
+template <int N, int E, int R, int LS>
+__device__ __forceinline__ void reg_write(float2* row, const float2 (&v)[E], int j, int p)
+{
+    constexpr int T = N / E;
+#pragma unroll
+    for (int i = 0; i < E / R; ++i) {
+        const int b = j + T * i;
+        const int k = b & (LS - 1);
+        const int base = (b - k) * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) row[swz(base + q * LS, p)] = v[i * R + q];
+    }
+}
+
+template <int N, int E, int R>
+__device__ __forceinline__ void reg_read(const float2* row, float2 (&v)[E], int j, int p)
+{
+    constexpr int T = N / E;
+#pragma unroll
+    for (int i = 0; i < E / R; ++i)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[i * R + r] = row[swz(j + T * i + r * (N / R), p)];
+}
+
+template <int N, bool INV>
+__device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row, int j, int p, const float2* tw)
+{
+    using PL = RegPlan<N>;
+    constexpr int E = PL::E, R0 = PL::R0, R1 = PL::R1, R2 = PL::R2;
+    reg_stage<N, E, R0, 1, INV>(v, j, tw);
+    reg_write<N, E, R0, 1>(row, v, j, p);
+    __syncthreads();
+    reg_read<N, E, R1>(row, v, j, p);
+    __syncthreads();
+    reg_stage<N, E, R1, R0, INV>(v, j, tw);
+    if constexpr (R2 > 1) {
+        reg_write<N, E, R1, R0>(row, v, j, p);
+        __syncthreads();
+        reg_read<N, E, R2>(row, v, j, p);
+        __syncthreads();
+        reg_stage<N, E, R2, R0 * R1, INV>(v, j, tw);
+    }
+}
+
+template <int N, int MODE, bool INV>
+__global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
+{
+    using PL = RegPlan<N>;
+    constexpr int E = PL::E, T = N / E, TP = PL::TP, R0 = PL::R0, RL = reg_last_radix<N>();
+    constexpr int NT = reg_threads<N>();
+    extern __shared__ float2 lds[];
+    const IterArgs& it = a.it;
+    const int t = threadIdx.x;
+    const int j = a.axis == 0 ? t % T : t / TP;
+    const int p = a.axis == 0 ? t / T : t % TP;
+    const Tile g = tile_of(a.axis, it.n1, it.n2, it.n3, TP, blockIdx.x);
+    const bool valid = p < g.valid;
+    float2* __restrict__ K = (MODE == kPlain) ? a.data : it.K;
+    float2* row = lds + p * N;
+    const int64_t pbase = g.base + p * g.ps;
+
+    float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E / R0; ++i)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int e = j + T * i + r * (N / R0);
+            v[i * R0 + r] = valid ? pass_load<MODE>(a, K, pbase + e * g.es) : make_float2(0.0f, 0.0f);
+        }
+
+    reg_fft<N, INV>(v, row, j, p, a.tw);
+
+    if (MODE == kFused) {
+        // natural order into LDS, replace, back into the first-stage layout
+#pragma unroll
+        for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+            for (int q = 0; q < RL; ++q) row[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
+        __syncthreads();
+        const int tpr = (it.n1 + TP - 1) / TP;
+        const int x0 = (int)(blockIdx.x % tpr) * TP, y = (int)(blockIdx.x / tpr);
+        const double err = csr_replace<true>(lds, N, it, x0, y, g.valid, t, NT);
+        wave_partial(err, it.err_part, (int64_t)blockIdx.x * (NT / 64) + (t >> 6));
+        __syncthreads();
+        reg_read<N, E, R0>(row, v, j, p);
+        __syncthreads();
+        reg_fft<N, true>(v, row, j, p, a.tw);
+    }
+
+    if (!valid) return;
+    const float scale = (INV || MODE == kFused) ? 1.0f / (float)N : 1.0f;
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) {
+            const int e = j + T * i + q * (N / RL);
+            pass_store<MODE>(a, K, pbase + e * g.es, v[i * RL + q], scale);
+        }
 }
 
 // ---- twiddle tables, one per length, built once per device ----
@@ -322,7 +533,6 @@ const float2* twiddles(int n)
     if (itr != g_tw.end()) return itr->second;
     std::vector<float2> h(n);
     for (int m = 0; m < n; ++m) {
-        // exp(-2 pi i m / n) from an exactly reduced FP64 angle
         const double ang = -2.0 * M_PI * (double)m / (double)n;
         h[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
@@ -333,25 +543,47 @@ const float2* twiddles(int n)
     return d;
 }
 
-int64_t num_tiles(int axis, int n1, int n2, int n3, int TP)
+#define GF_FFT_SIZES(X) \
+    X(4) X(8) X(16) X(32) X(45) X(48) X(64) X(75) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024)
+
+// Workgroups and per-wave partial slots of a pass launch.
+struct LaunchShape {
+    int64_t blocks;
+    int threads;
+    size_t lds;
+};
+
+template <int N> LaunchShape shape_n(int axis, int n1, int n2, int n3)
 {
-    if (axis == 0) return ((int64_t)n2 * n3 + TP - 1) / TP;
-    const int64_t tpr = (n1 + TP - 1) / TP;
-    return tpr * (axis == 1 ? n3 : n2);
+    if constexpr (RegPlan<N>::ok) {
+        return {ntiles(axis, n1, n2, n3, RegPlan<N>::TP), reg_threads<N>(),
+                (size_t)RegPlan<N>::TP * N * sizeof(float2)};
+    } else {
+        return {ntiles(axis, n1, n2, n3, tile_pencils(N)), kFT, (size_t)tile_pencils(N) * N * sizeof(float2)};
+    }
+}
+
+LaunchShape shape_of(int n, int axis, int n1, int n2, int n3)
+{
+    switch (n) {
+#define GF_CASE(NN) case NN: return shape_n<NN>(axis, n1, n2, n3);
+        GF_FFT_SIZES(GF_CASE)
+#undef GF_CASE
+        default:
+            raise(GENFIRE_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not supported");
+    }
 }
 
 template <int N, int MODE, bool INV>
 void launch_n(const PassArgs& a, hipStream_t st)
 {
-    constexpr int TP = tile_pencils(N);
-    const int64_t tiles = num_tiles(a.axis, a.it.n1, a.it.n2, a.it.n3, TP);
-    const size_t lds = (size_t)TP * N * sizeof(float2);
-    k_fft<N, MODE, INV><<<(unsigned)tiles, kFT, lds, st>>>(a);
+    const LaunchShape sh = shape_n<N>(a.axis, a.it.n1, a.it.n2, a.it.n3);
+    if constexpr (RegPlan<N>::ok)
+        k_fft_reg<N, MODE, INV><<<(unsigned)sh.blocks, sh.threads, sh.lds, st>>>(a);
+    else
+        k_fft<N, MODE, INV><<<(unsigned)sh.blocks, sh.threads, sh.lds, st>>>(a);
     GF_LAUNCH_CHECK();
 }
-
-#define GF_FFT_SIZES(X) \
-    X(4) X(8) X(16) X(32) X(45) X(48) X(64) X(75) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024)
 
 template <int MODE, bool INV>
 void launch(int n, const PassArgs& a, hipStream_t st)
@@ -362,21 +594,21 @@ void launch(int n, const PassArgs& a, hipStream_t st)
 #undef GF_CASE
         default:
             raise(GENFIRE_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) +
-                                               " is not supported (supported: 16..1024 with factors 2,3,5 in the built list)");
+                                               " is not supported (lengths with factors 2, 3, 5 from the built list)");
     }
 }
 
 int axis_len(const IterArgs& it, int axis) { return axis == 0 ? it.n1 : (axis == 1 ? it.n2 : it.n3); }
 
 // errK[it] = sum(part) / den, one workgroup, fixed summation order.
-__global__ __launch_bounds__(256) void k_err_reduce(const double* part, int n, double den, float* out)
+__global__ __launch_bounds__(1024) void k_err_reduce(const double* part, int64_t n, double den, float* out)
 {
-    __shared__ double s[256];
+    __shared__ double s[1024];
     double t = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) t += part[i];
+    for (int64_t i = threadIdx.x; i < n; i += 1024) t += part[i];
     s[threadIdx.x] = t;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = 512; off > 0; off >>= 1) {
         if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
         __syncthreads();
     }
@@ -391,40 +623,71 @@ __device__ __forceinline__ double kspace_coord(int c, int n)
     return (double)(c - (n - 1) / 2) / (double)((n - 1) / 2);
 }
 
-// flags per FFT-order voxel: measured = (M ~= 0) & mask (reconstruct.m:99);
-// enforced = measured & (1 - Q) > delay (:42,:120), Q from the centred index
-// exactly as make_Kspace_indices computes it in FP64.
-__global__ void k_flags(const float2* __restrict__ M, const uint8_t* __restrict__ mask, int n1, int n2,
-                        int n3, double delay, uint8_t* __restrict__ flags)
+__device__ __forceinline__ bool measured_at(const float2* M, const uint8_t* mask, int64_t s)
 {
-    const int64_t total = (int64_t)n1 * n2 * n3;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total;
-         s += (int64_t)gridDim.x * blockDim.x) {
-        const float2 m = M[s];
-        const bool meas = (m.x != 0.0f || m.y != 0.0f) && (mask == nullptr || mask[s] != 0);
-        uint8_t f = 0;
-        if (meas) {
-            const int s1 = (int)(s % n1), s2 = (int)((s / n1) % n2), s3 = (int)(s / ((int64_t)n1 * n2));
-            const int c1 = (s1 + n1 / 2) % n1, c2 = (s2 + n2 / 2) % n2, c3 = (s3 + n3 / 2) % n3;
-            const double v1 = kspace_coord(c1, n1), v2 = kspace_coord(c2, n2), v3 = kspace_coord(c3, n3);
-            const double q = sqrt((v2 * v2 + v1 * v1) + v3 * v3);
-            f = kFlagMeasured;
-            if (1.0 - q > delay) f |= kFlagEnforced;
-        }
-        flags[s] = f;
+    const float2 m = M[s];
+    return (m.x != 0.0f || m.y != 0.0f) && (mask == nullptr || mask[s] != 0);
+}
+
+// number of measured voxels (M ~= 0 & mask, reconstruct.m:99) per z-pencil (x, y)
+__global__ void k_csr_count(const float2* __restrict__ M, const uint8_t* __restrict__ mask, int n1, int n2,
+                            int n3, uint32_t* __restrict__ cnt)
+{
+    const int64_t np = (int64_t)n1 * n2;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t c = 0;
+        for (int z = 0; z < n3; ++z) c += measured_at(M, mask, q + np * z) ? 1u : 0u;
+        cnt[q] = c;
     }
 }
 
-__global__ __launch_bounds__(256) void k_den_part(const float2* __restrict__ M,
-                                                  const uint8_t* __restrict__ flags, int64_t n,
+__global__ void k_csr_fill(const float2* __restrict__ M, const uint8_t* __restrict__ mask, int n1, int n2,
+                           int n3, const int64_t* __restrict__ off, uint32_t* __restrict__ meta,
+                           float2* __restrict__ val)
+{
+    const int64_t np = (int64_t)n1 * n2;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = off[q];
+        const uint32_t x = (uint32_t)(q % n1);
+        for (int z = 0; z < n3; ++z) {
+            const int64_t s = q + np * z;
+            if (!measured_at(M, mask, s)) continue;
+            meta[o] = (uint32_t)z | (x << 11);
+            val[o] = M[s];
+            ++o;
+        }
+    }
+}
+
+// enforced = (1 - Q) > delay (reconstruct.m:42,120), Q from the centred index
+// exactly as make_Kspace_indices computes it in FP64.
+__global__ void k_csr_enforce(int n1, int n2, int n3, const int64_t* __restrict__ off,
+                              uint32_t* __restrict__ meta, double delay)
+{
+    const int64_t np = (int64_t)n1 * n2;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+        const int s1 = (int)(q % n1), s2 = (int)(q / n1);
+        const double v1 = kspace_coord((s1 + n1 / 2) % n1, n1);
+        const double v2 = kspace_coord((s2 + n2 / 2) % n2, n2);
+        for (int64_t e = off[q]; e < off[q + 1]; ++e) {
+            uint32_t m = meta[e] & 0x7FFFFFFFu;
+            const int z = (int)(m & 0x7FFu);
+            const double v3 = kspace_coord((z + n3 / 2) % n3, n3);
+            const double Q = sqrt((v2 * v2 + v1 * v1) + v3 * v3);
+            if (1.0 - Q > delay) m |= 0x80000000u;
+            meta[e] = m;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_den_part(const float2* __restrict__ val, int64_t n,
                                                   double* __restrict__ part)
 {
     double t = 0.0;
-    for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < n; s += (int64_t)gridDim.x * 256)
-        if (flags[s] & kFlagMeasured) {
-            const float2 m = M[s];
-            t += (double)sqrtf(m.x * m.x + m.y * m.y);
-        }
+    for (int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x; s < n; s += (int64_t)gridDim.x * 256) {
+        const float2 m = val[s];
+        t += (double)sqrtf(m.x * m.x + m.y * m.y);
+    }
     __shared__ double r[256];
     r[threadIdx.x] = t;
     __syncthreads();
@@ -443,6 +706,8 @@ __global__ void k_sum_parts(const double* part, int n, double* out)
         *out = t;
     }
 }
+
+int pencil_blocks(int64_t np) { return (int)std::min<int64_t>((np + 255) / 256, 4096); }
 
 }  // namespace
 
@@ -483,16 +748,10 @@ void iter_first(const IterArgs& it, hipStream_t st)
     launch<kFirst, false>(it.n1, a, st);
 }
 
-int iter_fused_blocks(const IterArgs& it)
+int64_t iter_fused_partials(const IterArgs& it)
 {
-    int TP = 8;
-    switch (it.n3) {
-#define GF_CASE(NN) case NN: TP = tile_pencils(NN); break;
-        GF_FFT_SIZES(GF_CASE)
-#undef GF_CASE
-        default: break;
-    }
-    return (int)num_tiles(2, it.n1, it.n2, it.n3, TP) * (kFT / 64);
+    const LaunchShape sh = shape_of(it.n3, 2, it.n1, it.n2, it.n3);
+    return sh.blocks * (sh.threads / 64);
 }
 
 void iter_fused(const IterArgs& it, hipStream_t st)
@@ -513,25 +772,34 @@ void iter_last(const IterArgs& it, hipStream_t st)
     launch<kLast, true>(it.n1, a, st);
 }
 
-void iter_errk_reduce(const double* d_part, int n, double den, float* d_errK_it, hipStream_t st)
+void iter_errk_reduce(const double* d_part, int64_t n, double den, float* d_errK_it, hipStream_t st)
 {
-    k_err_reduce<<<1, 256, 0, st>>>(d_part, n, den, d_errK_it);
+    k_err_reduce<<<1, 1024, 0, st>>>(d_part, n, den, d_errK_it);
     GF_LAUNCH_CHECK();
 }
 
-void iter_flags(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, double delay,
-                uint8_t* d_flags, hipStream_t st)
+void csr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st)
 {
-    const int64_t total = (int64_t)n1 * n2 * n3;
-    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 65536);
-    k_flags<<<blocks, 256, 0, st>>>(d_M, d_mask, n1, n2, n3, delay, d_flags);
+    k_csr_count<<<pencil_blocks((int64_t)n1 * n2), 256, 0, st>>>(d_M, d_mask, n1, n2, n3, d_cnt);
     GF_LAUNCH_CHECK();
 }
 
-void iter_den(const float2* d_M, const uint8_t* d_flags, int64_t n, double* d_tmp, int ntmp,
-              double* d_out, hipStream_t st)
+void csr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_off,
+              uint32_t* d_meta, float2* d_val, hipStream_t st)
 {
-    k_den_part<<<ntmp, 256, 0, st>>>(d_M, d_flags, n, d_tmp);
+    k_csr_fill<<<pencil_blocks((int64_t)n1 * n2), 256, 0, st>>>(d_M, d_mask, n1, n2, n3, d_off, d_meta, d_val);
+    GF_LAUNCH_CHECK();
+}
+
+void csr_enforce(int n1, int n2, int n3, const int64_t* d_off, uint32_t* d_meta, double delay, hipStream_t st)
+{
+    k_csr_enforce<<<pencil_blocks((int64_t)n1 * n2), 256, 0, st>>>(n1, n2, n3, d_off, d_meta, delay);
+    GF_LAUNCH_CHECK();
+}
+
+void csr_den(const float2* d_val, int64_t n, double* d_tmp, int ntmp, double* d_out, hipStream_t st)
+{
+    k_den_part<<<ntmp, 256, 0, st>>>(d_val, n, d_tmp);
     GF_LAUNCH_CHECK();
     k_sum_parts<<<1, 64, 0, st>>>(d_tmp, ntmp, d_out);
     GF_LAUNCH_CHECK();

@@ -61,35 +61,39 @@ void fft_axis(float2* d, int n1, int n2, int n3, int axis, int inverse, hipStrea
 void fft3(float2* d, int n1, int n2, int n3, int inverse, hipStream_t st);
 
 // ---- iteration passes (gf_fft.hip) ----
+// The measured voxels (measuredK ~= 0 & measuredK_mask) are kept per z-pencil
+// (x, y) in a compressed list: csr_off[x + n1*y] .. csr_off[x + n1*y + 1] index
+// csr_meta (z | x << 11 | enforced << 31) and csr_val (measuredK at the voxel).
 struct IterArgs {
-    float* x;            // iterate, real, FFT order
-    float2* K;           // spectrum workspace
-    const uint8_t* S;    // padded support, FFT order (0/1)
-    const float2* M;     // measuredK, FFT order
-    const uint8_t* flags;  // bit0 measured (errK set), bit1 enforced (replacement set)
-    float* best;         // P_R(x) output (nullable)
-    float* pf_out;       // P_F output copy (nullable)
-    double* err_part;    // per-workgroup partial errK numerators
+    float* x;                 // iterate, real, FFT order
+    float2* K;                // spectrum workspace
+    const uint8_t* S;         // padded support, FFT order (0/1)
+    const int64_t* csr_off;   // n1*n2 + 1 offsets
+    const uint32_t* csr_meta; // per measured voxel
+    const float2* csr_val;    // per measured voxel
+    float* best;              // P_R(x) output (nullable)
+    float* pf_out;            // P_F output copy (nullable)
+    double* err_part;         // per-wave partial errK numerators of the fused pass
     int n1, n2, n3;
     int positivity, support, dr;
 };
-constexpr uint8_t kFlagMeasured = 1;
-constexpr uint8_t kFlagEnforced = 2;
 // Pass 1: P_R (or the DR reflection) fused into the axis-0 forward FFT.
 void iter_first(const IterArgs& a, hipStream_t st);
 // Axis-2 forward FFT, errK partials, replacement, axis-2 inverse FFT.
-int iter_fused_blocks(const IterArgs& a);
+int64_t iter_fused_partials(const IterArgs& a);
 void iter_fused(const IterArgs& a, hipStream_t st);
 // Axis-0 inverse FFT fused with real part and the GENFIRE / DR update.
 void iter_last(const IterArgs& a, hipStream_t st);
 // errK[it] = sum(partials) / den.
-void iter_errk_reduce(const double* d_part, int n, double den, float* d_errK_it, hipStream_t st);
-// flags from M, mask and the enforcement threshold (1-Q > delay), reconstruct.m:42,120.
-void iter_flags(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, double delay,
-                uint8_t* d_flags, hipStream_t st);
-// sum |M| over measured voxels (errK denominator), into d_out (double).
-void iter_den(const float2* d_M, const uint8_t* d_flags, int64_t n, double* d_tmp, int ntmp,
-              double* d_out, hipStream_t st);
+void iter_errk_reduce(const double* d_part, int64_t n, double den, float* d_errK_it, hipStream_t st);
+// measured-voxel lists: per-pencil counts, fill (after an exclusive scan of the
+// counts), enforcement bits for a schedule stage ((1-Q) > delay, reconstruct.m:42,120)
+// and the errK denominator sum |M(errInd)|.
+void csr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st);
+void csr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_off,
+              uint32_t* d_meta, float2* d_val, hipStream_t st);
+void csr_enforce(int n1, int n2, int n3, const int64_t* d_off, uint32_t* d_meta, double delay, hipStream_t st);
+void csr_den(const float2* d_val, int64_t n, double* d_tmp, int ntmp, double* d_out, hipStream_t st);
 
 // ---- projector (gf_project.hip) ----
 struct ProjArgs {
