@@ -173,7 +173,7 @@ struct genfire_job {
 
     // iteration
     DevBuf x, K, S, best, pfbuf, err_part, errK, den_tmp, den, recon;
-    DevBuf csr_cnt, csr_off, csr_meta, csr_val;
+    DevBuf csr_cnt, csr_off, csr_meta, csr_val, csr_mag;
     int64_t nmeas = 0;
     DevBuf vol, Rmat, rsums, rout;
     std::vector<float> errK_h, errR_h;
@@ -373,15 +373,17 @@ void do_reconstruct(genfire_job* j)
     const bool monitor = j->prm.monitor_R != 0;
     const int L = std::max(1, j->prm.monitorR_loopLength);
 
+    const bool half = half_supported(n1, n2, n3);
+    const int64_t hn = (int64_t)(n1 / 2 + 1) * n2 * n3;   // half-spectrum size
     j->x.alloc(sizeof(float) * nf);
-    j->K.alloc(sizeof(float2) * nf);
+    j->K.alloc(sizeof(float2) * (half ? std::max<int64_t>(2 * hn, nf) : nf));
     j->S.alloc(nf);
     j->best.alloc(sizeof(float) * nf);
     if (dr && monitor) j->pfbuf.alloc(sizeof(float) * nf);
     IterArgs ia{};
     ia.n1 = n1; ia.n2 = n2; ia.n3 = n3;
     GF_CHECK(n1 <= 2048 && n3 <= 2048, GENFIRE_ERR_UNSUPPORTED, "GENFIRE: grid edge above 2048");
-    const int64_t nparts = iter_fused_partials(ia);
+    const int64_t nparts = half ? half_partials(n1, n2, n3) : iter_fused_partials(ia);
     j->err_part.alloc(sizeof(double) * nparts);
     j->errK.alloc(sizeof(float) * std::max(N, 1));
     const int ntmp = 1024;
@@ -412,21 +414,31 @@ void do_reconstruct(genfire_job* j)
     const std::vector<int> sched = cutoff_schedule(N, (int)j->delays.size());
     GF_CHECK(N >= 1 && sched.back() == N, GENFIRE_ERR_ARG, "GENFIRE: numIterations must be >= 1");
 
-    // measured voxels (errInd, reconstruct.m:99) as per-z-pencil lists
-    const int64_t npen = (int64_t)n1 * n2;
+    // measured voxels (errInd, reconstruct.m:99) as per-pencil lists
     const uint8_t* mask = j->have_mask ? j->mask.as<uint8_t>() : nullptr;
+    const int64_t npen = half ? (int64_t)(n1 / 2 + 1) * n2 : (int64_t)n1 * n2;
     j->csr_cnt.alloc(sizeof(uint32_t) * npen);
     j->csr_off.alloc(sizeof(int64_t) * (npen + 1));
-    csr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
+    if (half) hcsr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
+    else csr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
     grid_index_scan(j->csr_cnt.as<uint32_t>(), npen, j->csr_off.as<int64_t>(), j->csr_off.as<int64_t>() + npen, st);
     GF_HIP(hipMemcpyAsync(&j->nmeas, j->csr_off.as<int64_t>() + npen, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     GF_HIP(hipStreamSynchronize(st));
-    j->csr_meta.alloc(sizeof(uint32_t) * std::max<int64_t>(j->nmeas, 1));
-    j->csr_val.alloc(sizeof(float2) * std::max<int64_t>(j->nmeas, 1));
-    csr_fill(j->M.as<float2>(), mask, n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(),
-             j->csr_val.as<float2>(), st);
+    const int64_t ne = std::max<int64_t>(j->nmeas, 1);
+    j->csr_meta.alloc(sizeof(uint32_t) * ne);
+    j->csr_val.alloc(sizeof(float2) * ne);
+    if (half) {
+        j->csr_mag.alloc(sizeof(float2) * ne);
+        hcsr_fill(j->M.as<float2>(), mask, n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(),
+                  j->csr_val.as<float2>(), j->csr_mag.as<float2>(), st);
+        hcsr_den(j->csr_mag.as<float2>(), j->csr_meta.as<uint32_t>(), j->nmeas, j->den_tmp.as<double>(), ntmp,
+                 j->den.as<double>(), st);
+    } else {
+        csr_fill(j->M.as<float2>(), mask, n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(),
+                 j->csr_val.as<float2>(), st);
+        csr_den(j->csr_val.as<float2>(), j->nmeas, j->den_tmp.as<double>(), ntmp, j->den.as<double>(), st);
+    }
     // errK denominator: sum |M(errInd)| (reconstruct.m:135)
-    csr_den(j->csr_val.as<float2>(), j->nmeas, j->den_tmp.as<double>(), ntmp, j->den.as<double>(), st);
     double den = 0.0;
     GF_HIP(hipMemcpyAsync(&den, j->den.p, sizeof(double), hipMemcpyDeviceToHost, st));
     GF_HIP(hipStreamSynchronize(st));
@@ -447,6 +459,20 @@ void do_reconstruct(genfire_job* j)
     ia.positivity = j->prm.constraintPositivity ? 1 : 0;
     ia.support = j->prm.constraintSupport ? 1 : 0;
     ia.dr = dr ? 1 : 0;
+    HalfIterArgs ha{};
+    ha.x = ia.x;
+    ha.A = j->K.as<float2>();
+    ha.B = j->K.as<float2>() + hn;
+    ha.S = ia.S;
+    ha.csr_off = ia.csr_off;
+    ha.csr_meta = ia.csr_meta;
+    ha.csr_val = ia.csr_val;
+    ha.csr_mag = j->csr_mag.as<float2>();
+    ha.err_part = ia.err_part;
+    ha.n1 = n1; ha.n2 = n2; ha.n3 = n3;
+    ha.positivity = ia.positivity;
+    ha.support = ia.support;
+    ha.dr = ia.dr;
 
     GF_HIP(hipEventRecord(loop.a, st));
     size_t cur = 0;
@@ -455,7 +481,8 @@ void do_reconstruct(genfire_job* j)
             size_t last = cur;
             for (size_t q = cur; q < sched.size(); ++q)
                 if (sched[q] == it) last = q;
-            csr_enforce(n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(), j->delays[last], st);
+            if (half) hcsr_enforce(n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(), j->delays[last], st);
+            else csr_enforce(n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(), j->delays[last], st);
             cur = last + 1;
         }
         // The schedule always ends with a change at it == N, which resets bestErr
@@ -466,15 +493,29 @@ void do_reconstruct(genfire_job* j)
         ia.pf_out = (mon_it && dr) ? j->pfbuf.as<float>() : nullptr;
         std::pair<hipEvent_t, hipEvent_t> e_it, e_pass, e_fused;
         kstat_begin(j, GENFIRE_KSTAT_ITERATION, &e_it);
-        iter_first(ia, st);
-        kstat_begin(j, GENFIRE_KSTAT_FFT_PASS, &e_pass);
-        fft_axis(j->K.as<float2>(), n1, n2, n3, 1, 0, st);
-        kstat_end(j, GENFIRE_KSTAT_FFT_PASS, e_pass);
-        kstat_begin(j, GENFIRE_KSTAT_FFT_FUSED, &e_fused);
-        iter_fused(ia, st);
-        kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
-        fft_axis(j->K.as<float2>(), n1, n2, n3, 1, 1, st);
-        iter_last(ia, st);
+        if (half) {
+            ha.best = ia.best;
+            ha.pf_out = ia.pf_out;
+            half_first(ha, st);
+            kstat_begin(j, GENFIRE_KSTAT_FFT_PASS, &e_pass);
+            half_y(ha, 0, st);
+            kstat_end(j, GENFIRE_KSTAT_FFT_PASS, e_pass);
+            kstat_begin(j, GENFIRE_KSTAT_FFT_FUSED, &e_fused);
+            half_z(ha, st);
+            kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
+            half_y(ha, 1, st);
+            half_last(ha, st);
+        } else {
+            iter_first(ia, st);
+            kstat_begin(j, GENFIRE_KSTAT_FFT_PASS, &e_pass);
+            fft_axis(j->K.as<float2>(), n1, n2, n3, 1, 0, st);
+            kstat_end(j, GENFIRE_KSTAT_FFT_PASS, e_pass);
+            kstat_begin(j, GENFIRE_KSTAT_FFT_FUSED, &e_fused);
+            iter_fused(ia, st);
+            kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
+            fft_axis(j->K.as<float2>(), n1, n2, n3, 1, 1, st);
+            iter_last(ia, st);
+        }
         iter_errk_reduce(j->err_part.as<double>(), nparts, den, j->errK.as<float>() + (it - 1), st);
         kstat_end(j, GENFIRE_KSTAT_ITERATION, e_it);
         if (mon_it) {

@@ -19,143 +19,14 @@
 #include <mutex>
 #include <unordered_map>
 
+#include "gf_fft_core.h"
 #include "gf_grid.h"
 
 namespace gf {
 
 namespace {
 
-constexpr int kFT = 256;
-
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b)
-{
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
-// a * (-i) for the forward transform, a * (+i) for the inverse.
-template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a)
-{
-    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
-}
-
-template <int R, bool INV> struct Dft;
-
-template <bool INV> struct Dft<2, INV> {
-    static __device__ __forceinline__ void run(float2* v)
-    {
-        const float2 a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    }
-};
-
-template <bool INV> struct Dft<4, INV> {
-    static __device__ __forceinline__ void run(float2* v)
-    {
-        const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        const float2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
-        v[0] = cadd(t0, t2);
-        v[2] = csub(t0, t2);
-        v[1] = cadd(t1, t3);
-        v[3] = csub(t1, t3);
-    }
-};
-
-template <bool INV> struct Dft<8, INV> {
-    static __device__ __forceinline__ void run(float2* v)
-    {
-        float2 e[4] = {v[0], v[2], v[4], v[6]};
-        float2 o[4] = {v[1], v[3], v[5], v[7]};
-        Dft<4, INV>::run(e);
-        Dft<4, INV>::run(o);
-        const float c = 0.70710678118654752440f;
-        // o[q] *= exp(-+ i pi q / 4)
-        o[1] = INV ? make_float2(c * (o[1].x - o[1].y), c * (o[1].x + o[1].y))
-                   : make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));
-        o[2] = mul_mi<INV>(o[2]);
-        o[3] = INV ? make_float2(-c * (o[3].x + o[3].y), c * (o[3].x - o[3].y))
-                   : make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            v[q] = cadd(e[q], o[q]);
-            v[q + 4] = csub(e[q], o[q]);
-        }
-    }
-};
-
-template <bool INV> struct Dft<3, INV> {
-    static __device__ __forceinline__ void run(float2* v)
-    {
-        const float s3 = 0.86602540378443864676f;
-        const float2 t = cadd(v[1], v[2]);
-        const float2 d = csub(v[1], v[2]);
-        const float2 a = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
-        const float2 b = cscale(mul_mi<INV>(d), s3);  // -i*s3*d forward
-        v[0] = cadd(v[0], t);
-        v[1] = cadd(a, b);
-        v[2] = csub(a, b);
-    }
-};
-
-template <bool INV> struct Dft<5, INV> {
-    static __device__ __forceinline__ void run(float2* v)
-    {
-        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
-        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
-        const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-        const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
-        const float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
-        const float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
-        const float2 b1 = make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
-        const float2 b2 = make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
-        const float2 ib1 = mul_mi<INV>(b1), ib2 = mul_mi<INV>(b2);  // -i*b forward
-        v[0] = cadd(v[0], cadd(t1, t2));
-        v[1] = cadd(a1, ib1);
-        v[4] = csub(a1, ib1);
-        v[2] = cadd(a2, ib2);
-        v[3] = csub(a2, ib2);
-    }
-};
-
-constexpr int pick_radix(int rem)
-{
-    return rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : rem % 5 == 0 ? 5 : rem;
-}
-
-// 16 = 4 x 4: X[c + 4b] = sum_a W4^(ab) W16^(ac) sum_m x[a + 4m] W4^(mc).
-template <bool INV> struct Dft<16, INV> {
-    static __device__ __forceinline__ void run(float2* v)
-    {
-        float2 t[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            float2 q[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
-            Dft<4, INV>::run(q);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) t[a][c] = q[c];
-        }
-        // W16^(ac), a, c in 1..3 (forward sign -)
-        const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
-        const float c2 = 0.70710678118654752440f;
-        const float2 w1 = make_float2(c1, INV ? s1 : -s1);
-        const float2 w2 = make_float2(c2, INV ? c2 : -c2);
-        const float2 w3 = make_float2(s1, INV ? c1 : -c1);
-        const float2 w6 = make_float2(-c2, INV ? c2 : -c2);
-        const float2 w9 = make_float2(-c1, INV ? -s1 : s1);
-        t[1][1] = cmul(t[1][1], w1); t[1][2] = cmul(t[1][2], w2); t[1][3] = cmul(t[1][3], w3);
-        t[2][1] = cmul(t[2][1], w2); t[2][2] = mul_mi<INV>(t[2][2]); t[2][3] = cmul(t[2][3], w6);
-        t[3][1] = cmul(t[3][1], w3); t[3][2] = cmul(t[3][2], w6); t[3][3] = cmul(t[3][3], w9);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            float2 q[4] = {t[0][c], t[1][c], t[2][c], t[3][c]};
-            Dft<4, INV>::run(q);
-#pragma unroll
-            for (int b = 0; b < 4; ++b) v[c + 4 * b] = q[b];
-        }
-    }
-};
+using namespace fftc;
 
 struct Tile {
     int64_t base, ps, es;
@@ -242,16 +113,6 @@ __device__ __forceinline__ void pass_store(const PassArgs& a, float2* K, int64_t
     }
 }
 
-// LDS column swizzle of the register kernel: element `col` of pencil `p` lives at
-// col ^ (((col >> 3) ^ p) & 15).  The 16 lanes an LDS access serves together then
-// fall on 16 distinct bank pairs for every pattern used here: 16 consecutive
-// columns of one pencil, the radix-8 Stockham writes 8b+q and 64m+k+8q, and one
-// column of 16 different pencils (rows N*8 bytes apart would otherwise share a bank).
-__device__ __forceinline__ int swz(int col, int p) { return col ^ (((col >> 3) ^ p) & 15); }
-
-// errK numerator (reconstruct.m:135) and Fourier replacement (:163) on the
-// natural-order spectrum of the tile's z-pencils [x0, x0+valid) at row y.  The
-// measured voxels come from the per-pencil list (meta = z | x<<11 | enforced<<31).
 template <bool SWZ>
 __device__ __forceinline__ double csr_replace(float2* lds, int N, const IterArgs& it, int x0, int y,
                                               int valid, int tid, int nthreads)
@@ -374,89 +235,6 @@ __global__ __launch_bounds__(kFT) void k_fft(PassArgs a)
     }
 }
 
-// ============================================================================
-// Register kernel (power-of-two lengths 64..1024).  A pencil is owned by
-// T = N/E lanes holding E elements each; every radix stage is computed in
-// registers and only the two exchanges between the three stages go through
-// LDS.  The first stage's inputs and the last stage's outputs are the elements
-// j + T*i + r*N/R of the lane's pencil, so global loads and stores go straight
-// from and to registers: along axis 0 consecutive lanes take consecutive
-// elements, along axes 1 and 2 consecutive lanes take consecutive pencils
-// (TP * 8 bytes contiguous per row).
-// ============================================================================
-template <int N> struct RegPlan { static constexpr bool ok = false; };
-template <> struct RegPlan<64> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1, TP = 32; };
-template <> struct RegPlan<128> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 2, TP = 16; };
-template <> struct RegPlan<256> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 4, TP = 16; };
-template <> struct RegPlan<512> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 8, TP = 16; };
-template <> struct RegPlan<1024> { static constexpr bool ok = true; static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 4, TP = 8; };
-
-template <int N> constexpr int reg_threads() { return RegPlan<N>::TP * (N / RegPlan<N>::E); }
-template <int N> constexpr int reg_last_radix() { return RegPlan<N>::R2 > 1 ? RegPlan<N>::R2 : RegPlan<N>::R1; }
-
-template <int N, int E, int R, int LS, bool INV>
-__device__ __forceinline__ void reg_stage(float2 (&v)[E], int j, const float2* __restrict__ tw)
-{
-    constexpr int T = N / E;
-#pragma unroll
-    for (int i = 0; i < E / R; ++i) {
-        if (LS > 1) {
-            const int k = (j + T * i) & (LS - 1);
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                float2 w = tw[r * k * (N / (LS * R))];
-                if (INV) w.y = -w.y;
-                v[i * R + r] = cmul(v[i * R + r], w);
-            }
-        }
-        Dft<R, INV>::run(&v[i * R]);
-    }
-}
-
-template <int N, int E, int R, int LS>
-__device__ __forceinline__ void reg_write(float2* row, const float2 (&v)[E], int j, int p)
-{
-    constexpr int T = N / E;
-#pragma unroll
-    for (int i = 0; i < E / R; ++i) {
-        const int b = j + T * i;
-        const int k = b & (LS - 1);
-        const int base = (b - k) * R + k;
-#pragma unroll
-        for (int q = 0; q < R; ++q) row[swz(base + q * LS, p)] = v[i * R + q];
-    }
-}
-
-template <int N, int E, int R>
-__device__ __forceinline__ void reg_read(const float2* row, float2 (&v)[E], int j, int p)
-{
-    constexpr int T = N / E;
-#pragma unroll
-    for (int i = 0; i < E / R; ++i)
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[i * R + r] = row[swz(j + T * i + r * (N / R), p)];
-}
-
-template <int N, bool INV>
-__device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row, int j, int p, const float2* tw)
-{
-    using PL = RegPlan<N>;
-    constexpr int E = PL::E, R0 = PL::R0, R1 = PL::R1, R2 = PL::R2;
-    reg_stage<N, E, R0, 1, INV>(v, j, tw);
-    reg_write<N, E, R0, 1>(row, v, j, p);
-    __syncthreads();
-    reg_read<N, E, R1>(row, v, j, p);
-    __syncthreads();
-    reg_stage<N, E, R1, R0, INV>(v, j, tw);
-    if constexpr (R2 > 1) {
-        reg_write<N, E, R1, R0>(row, v, j, p);
-        __syncthreads();
-        reg_read<N, E, R2>(row, v, j, p);
-        __syncthreads();
-        reg_stage<N, E, R2, R0 * R1, INV>(v, j, tw);
-    }
-}
-
 template <int N, int MODE, bool INV>
 __global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
 {
@@ -511,36 +289,6 @@ __global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
             const int e = j + T * i + q * (N / RL);
             pass_store<MODE>(a, K, pbase + e * g.es, v[i * RL + q], scale);
         }
-}
-
-// ---- twiddle tables, one per length, built once per device ----
-struct TwKey {
-    int dev, n;
-    bool operator==(const TwKey& o) const { return dev == o.dev && n == o.n; }
-};
-struct TwHash {
-    size_t operator()(const TwKey& k) const { return (size_t)k.dev * 131071u + (size_t)k.n; }
-};
-std::mutex g_tw_mu;
-std::unordered_map<TwKey, float2*, TwHash> g_tw;
-
-const float2* twiddles(int n)
-{
-    int dev = 0;
-    GF_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_tw_mu);
-    auto itr = g_tw.find({dev, n});
-    if (itr != g_tw.end()) return itr->second;
-    std::vector<float2> h(n);
-    for (int m = 0; m < n; ++m) {
-        const double ang = -2.0 * M_PI * (double)m / (double)n;
-        h[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-    }
-    float2* d = nullptr;
-    GF_HIP(hipMalloc(&d, sizeof(float2) * n));
-    GF_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-    g_tw[{dev, n}] = d;
-    return d;
 }
 
 #define GF_FFT_SIZES(X) \
@@ -711,6 +459,39 @@ int pencil_blocks(int64_t np) { return (int)std::min<int64_t>((np + 255) / 256, 
 
 }  // namespace
 
+// ---- twiddle tables, one per length, built once per device ----
+namespace {
+struct TwKey {
+    int dev, n;
+    bool operator==(const TwKey& o) const { return dev == o.dev && n == o.n; }
+};
+struct TwHash {
+    size_t operator()(const TwKey& k) const { return (size_t)k.dev * 131071u + (size_t)k.n; }
+};
+std::mutex g_tw_mu;
+std::unordered_map<TwKey, float2*, TwHash> g_tw;
+}  // namespace
+
+const float2* fft_twiddles(int n)
+{
+    int dev = 0;
+    GF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto itr = g_tw.find({dev, n});
+    if (itr != g_tw.end()) return itr->second;
+    std::vector<float2> h(n);
+    for (int m = 0; m < n; ++m) {
+        const double ang = -2.0 * M_PI * (double)m / (double)n;
+        h[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    float2* d = nullptr;
+    GF_HIP(hipMalloc(&d, sizeof(float2) * n));
+    GF_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    g_tw[{dev, n}] = d;
+    return d;
+}
+
+
 bool fft_supported(int n)
 {
     switch (n) {
@@ -729,7 +510,7 @@ void fft_axis(float2* d, int n1, int n2, int n3, int axis, int inverse, hipStrea
     a.axis = axis;
     const int n = axis_len(a.it, axis);
     if (n == 1) return;
-    a.tw = twiddles(n);
+    a.tw = fft_twiddles(n);
     if (inverse) launch<kPlain, true>(n, a, st);
     else launch<kPlain, false>(n, a, st);
 }
@@ -744,7 +525,7 @@ void iter_first(const IterArgs& it, hipStream_t st)
     PassArgs a{};
     a.it = it;
     a.axis = 0;
-    a.tw = twiddles(it.n1);
+    a.tw = fft_twiddles(it.n1);
     launch<kFirst, false>(it.n1, a, st);
 }
 
@@ -759,7 +540,7 @@ void iter_fused(const IterArgs& it, hipStream_t st)
     PassArgs a{};
     a.it = it;
     a.axis = 2;
-    a.tw = twiddles(it.n3);
+    a.tw = fft_twiddles(it.n3);
     launch<kFused, false>(it.n3, a, st);
 }
 
@@ -768,7 +549,7 @@ void iter_last(const IterArgs& it, hipStream_t st)
     PassArgs a{};
     a.it = it;
     a.axis = 0;
-    a.tw = twiddles(it.n1);
+    a.tw = fft_twiddles(it.n1);
     launch<kLast, true>(it.n1, a, st);
 }
 

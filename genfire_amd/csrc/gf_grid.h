@@ -55,6 +55,8 @@ void pad_centre_u8(const uint8_t* d_src, int d1, int d2, int d3, uint8_t* d_dst,
 
 // ---- 3D FFT (gf_fft.hip) ----
 bool fft_supported(int n);
+// exp(-2 pi i m / n), m < n, FP32 from FP64, cached per device.
+const float2* fft_twiddles(int n);
 // Complex-to-complex pass along one axis of an n1 x n2 x n3 grid, in place.
 // inverse passes scale by 1/n_axis.
 void fft_axis(float2* d, int n1, int n2, int n3, int axis, int inverse, hipStream_t st);
@@ -94,6 +96,43 @@ void csr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, 
               uint32_t* d_meta, float2* d_val, hipStream_t st);
 void csr_enforce(int n1, int n2, int n3, const int64_t* d_off, uint32_t* d_meta, double delay, hipStream_t st);
 void csr_den(const float2* d_val, int64_t n, double* d_tmp, int ntmp, double* d_out, hipStream_t st);
+
+// ---- half-spectrum iteration (gf_iter_half.hip) ----
+// Spectrum of the real iterate for kx in [0, n1/2] in two layouts:
+// A[y + n2*(kx + h*z)] and B[z + n3*(ky + n2*kx)], h = n1/2 + 1.  The measured
+// voxels of each B pencil (ky + n2*kx) carry meta = z | ky << 11 | flags, the
+// replacement value and |M| of the voxel and of its unstored mate.
+struct HalfIterArgs {
+    float* x;
+    float2* A;
+    float2* B;
+    const uint8_t* S;
+    const int64_t* csr_off;
+    const uint32_t* csr_meta;
+    const float2* csr_val;
+    const float2* csr_mag;
+    float* best;
+    float* pf_out;
+    double* err_part;
+    int n1, n2, n3;
+    int positivity, support, dr;
+};
+constexpr uint32_t kHalfTermA = 1u << 28;     // voxel measured: errK term |k| vs mag.x
+constexpr uint32_t kHalfTermB = 1u << 29;     // mate measured: errK term |k| vs mag.y
+constexpr uint32_t kHalfAverage = 1u << 30;   // only one of the pair measured: k = (V + k)/2
+constexpr uint32_t kHalfEnforced = 1u << 31;  // replaced at the current schedule stage
+bool half_supported(int n1, int n2, int n3);
+int64_t half_partials(int n1, int n2, int n3);
+void half_first(const HalfIterArgs& a, hipStream_t st);
+void half_y(const HalfIterArgs& a, int inverse, hipStream_t st);
+void half_z(const HalfIterArgs& a, hipStream_t st);
+void half_last(const HalfIterArgs& a, hipStream_t st);
+void hcsr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st);
+void hcsr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_off,
+               uint32_t* d_meta, float2* d_val, float2* d_mag, hipStream_t st);
+void hcsr_enforce(int n1, int n2, int n3, const int64_t* d_off, uint32_t* d_meta, double delay, hipStream_t st);
+void hcsr_den(const float2* d_mag, const uint32_t* d_meta, int64_t n, double* d_tmp, int ntmp, double* d_out,
+              hipStream_t st);
 
 // ---- projector (gf_project.hip) ----
 struct ProjArgs {

@@ -1,0 +1,564 @@
+// gf_iter_half.hip -- the GENFIRE / DR iteration on the half spectrum
+// (reconstruct.m:126-165 with the object real).
+//
+// The iterate is real, so its spectrum is kept for kx in [0, n1/2] only (h =
+// n1/2+1 planes).  real(ifftn(k')) of the replaced full spectrum k' equals the
+// C2R transform of the half spectrum when
+//   * for 0 < kx < n1/2 the stored value is the Hermitian part
+//     (k'(s) + conj(k'(mate)))/2 of the voxel s and its unstored mate -s, which
+//     is M(s) where both are measured with M(-s) = conj(M(s)) (all interior
+//     voxels of a CentroSymmetricity grid), the average of the two measured
+//     values where they differ (the +-n/2 lines the reference fills from its
+//     extended half grid), and (M + k)/2 where only one of the pair is measured;
+//   * the kx = 0 and kx = n1/2 planes hold both s and -s, are replaced
+//     voxel by voxel, and contribute only their real part after the y and z
+//     inverse transforms (the C2R takes Re of its first and last bins).
+// errK's numerator counts |k(-s)| = |k(s)| for the mate, so each stored voxel
+// carries up to two terms.  The rules are exact for any measuredK and mask.
+//
+// Five passes, each with one contiguous side and one side of 128-byte rows at a
+// 4-KiB stride (no 2-MiB strides):
+//   P1 x: real rows -> P_R / DR reflection -> R2C (complex FFT of n1/2 + split)
+//         -> layout A  [y][kx][z]   (A[y + n2*(kx + h*z)])
+//   P2 y: A pencils (contiguous) -> FFT -> layout B  [z][ky][kx] (B[z + n3*(ky + n2*kx)])
+//   P3 z: B pencils (contiguous) -> FFT -> errK + replacement -> IFFT -> B
+//   P4 y: B (strided) -> IFFT -> A
+//   P5 x: A (strided) -> Re of kx = 0, n1/2 -> C2R -> x (GENFIRE) or DR update
+#include <cmath>
+
+#include "gf_fft_core.h"
+#include "gf_grid.h"
+
+namespace gf {
+
+namespace {
+
+using namespace fftc;
+
+constexpr int kHP = 16;  // pencils per workgroup: 16 x 8 bytes = one 128-byte row
+
+template <int N> constexpr int half_threads() { return kHP * (N / RegPlan<N>::E); }
+
+__device__ __forceinline__ float prh(float v, uint8_t s, int positivity, int support)
+{
+    if (positivity && v < 0.0f) v = 0.0f;   // x(x<0) = 0   (reconstruct.m:127)
+    if (support) v = v * (float)s;          // x .* S       (:130)
+    return v;
+}
+
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// ---------------------------------------------------------------------------
+// P1: P_R and the real-to-complex x transform.  N = n1, complex length M = N/2.
+// Tile: 16 consecutive y rows at one z.  Lane (p, j), p = t / T, reads the
+// element pair (2e, 2e+1) of row p as z[e] = x[2e] + i x[2e+1].
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfIterArgs a, const float2* __restrict__ twM,
+                                                                      const float2* __restrict__ twN)
+{
+    constexpr int M = N / 2;
+    using PL = RegPlan<M>;
+    constexpr int E = PL::E, T = M / E, R0 = PL::R0, RL = reg_last_radix<M>();
+    extern __shared__ float2 lds[];
+    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int n2 = a.n2, h = M + 1;
+    const int tpr = n2 / kHP;
+    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
+    const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
+
+    float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E / R0; ++i)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int e = j + T * i + r * (M / R0);
+            const int64_t ad = row + 2 * e;
+            const float2 xv = *reinterpret_cast<const float2*>(a.x + ad);
+            const uint16_t s2 = *reinterpret_cast<const uint16_t*>(a.S + ad);
+            const float ra = prh(xv.x, (uint8_t)(s2 & 0xFF), a.positivity, a.support);
+            const float rb = prh(xv.y, (uint8_t)(s2 >> 8), a.positivity, a.support);
+            if (a.best) *reinterpret_cast<float2*>(a.best + ad) = make_float2(ra, rb);
+            v[i * R0 + r] = a.dr ? make_float2(2.0f * ra - xv.x, 2.0f * rb - xv.y) : make_float2(ra, rb);
+        }
+    float2* rowl = lds + p * M;
+    reg_fft<M, false>(v, rowl, j, p, twM);
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (M / RL), p)] = v[i * RL + q];
+    __syncthreads();
+    // X[k] = (Z[k] + conj Z[M-k])/2 + W_N^k (Z[k] - conj Z[M-k])/(2i), k = 0..M
+    const int pp = t % kHP, c = t / kHP;
+    const float2* rp = lds + pp * M;
+    float2* out = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
+    for (int kx = c; kx <= M; kx += T) {
+        const float2 zk = rp[swz(kx & (M - 1), pp)];
+        const float2 zm = conjf2(rp[swz((M - kx) & (M - 1), pp)]);
+        const float2 ev = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+        const float2 od = cscale(mul_mi<false>(csub(zk, zm)), 0.5f);
+        out[(int64_t)n2 * kx] = cadd(ev, cmul(twN[kx], od));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2: y forward, A -> B.  Tile: one kx, 16 consecutive z.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfIterArgs a, const float2* __restrict__ tw)
+{
+    using PL = RegPlan<N>;
+    constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
+    extern __shared__ float2 lds[];
+    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int n2 = N, n3 = a.n3, h = a.n1 / 2 + 1;
+    const int tpr = n3 / kHP;
+    const int kx = blockIdx.x / tpr, i3_0 = (blockIdx.x % tpr) * kHP;
+    const float2* src = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p));
+    float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E / R0; ++i)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[i * R0 + r] = src[j + T * i + r * (N / R0)];
+    float2* rowl = lds + p * N;
+    reg_fft<N, false>(v, rowl, j, p, tw);
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
+    __syncthreads();
+    const int pp = t % kHP, c = t / kHP;
+    const float2* rp = lds + pp * N;
+    float2* dst = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
+    for (int e = c; e < N; e += T) dst[(int64_t)n3 * e] = rp[swz(e, pp)];
+}
+
+// ---------------------------------------------------------------------------
+// P3: z forward, errK, replacement, z inverse, in place on B.  Tile: one kx,
+// 16 consecutive ky (16 contiguous pencils).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(half_threads<N>()) void k_half_z_fused(HalfIterArgs a, const float2* __restrict__ tw)
+{
+    using PL = RegPlan<N>;
+    constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
+    constexpr int NT = half_threads<N>();
+    extern __shared__ float2 lds[];
+    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int n2 = a.n2, n3 = N;
+    const int tpr = n2 / kHP;
+    const int kx = blockIdx.x / tpr, ky_0 = (blockIdx.x % tpr) * kHP;
+    float2* pen = a.B + (int64_t)n3 * ((ky_0 + p) + (int64_t)n2 * kx);
+    float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E / R0; ++i)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[i * R0 + r] = pen[j + T * i + r * (N / R0)];
+    float2* rowl = lds + p * N;
+    reg_fft<N, false>(v, rowl, j, p, tw);
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
+    __syncthreads();
+    // errK numerator (:135) and replacement (:163) from the half-spectrum lists
+    double err = 0.0;
+    {
+        const int64_t r0 = (int64_t)ky_0 + (int64_t)n2 * kx;
+        const int64_t beg = a.csr_off[r0], end = a.csr_off[r0 + kHP];
+        for (int64_t e = beg + t; e < end; e += NT) {
+            const uint32_t m = a.csr_meta[e];
+            const int z = (int)(m & 0x7FFu), ky = (int)((m >> 11) & 0x7FFu);
+            const int pl = ky - ky_0;
+            float2* slot = lds + pl * N + swz(z, pl);
+            const float2 kv = *slot;
+            const float2 mg = a.csr_mag[e];
+            const float ak = sqrtf(kv.x * kv.x + kv.y * kv.y);
+            if (m & kHalfTermA) err += (double)fabsf(ak - mg.x);
+            if (m & kHalfTermB) err += (double)fabsf(ak - mg.y);
+            if (m & kHalfEnforced) {
+                const float2 V = a.csr_val[e];
+                *slot = (m & kHalfAverage) ? make_float2(0.5f * (V.x + kv.x), 0.5f * (V.y + kv.y)) : V;
+            }
+        }
+    }
+    __syncthreads();
+    reg_read<N, E, R0>(rowl, v, j, p);
+    __syncthreads();
+    reg_fft<N, true>(v, rowl, j, p, tw);
+    const float sc = 1.0f / (float)N;
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) pen[j + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
+    // one errK partial per workgroup (the tile's LDS is free after the last stage)
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(lds);
+    for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
+    if ((t & 63) == 0) red[t >> 6] = err;
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.0;
+        for (int w = 0; w < NT / 64; ++w) s += red[w];
+        a.err_part[blockIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P4: y inverse, B -> A.  Tile: one kx, 16 consecutive z (strided read).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfIterArgs a, const float2* __restrict__ tw)
+{
+    using PL = RegPlan<N>;
+    constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
+    extern __shared__ float2 lds[];
+    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int n2 = N, n3 = a.n3, h = a.n1 / 2 + 1;
+    const int tpr = n3 / kHP;
+    const int kx = blockIdx.x / tpr, i3_0 = (blockIdx.x % tpr) * kHP;
+    {
+        const int pp = t % kHP, c = t / kHP;
+        float2* rp = lds + pp * N;
+        const float2* src = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
+        for (int e = c; e < N; e += T) rp[swz(e, pp)] = src[(int64_t)n3 * e];
+    }
+    __syncthreads();
+    float2* rowl = lds + p * N;
+    float2 v[E];
+    reg_read<N, E, R0>(rowl, v, j, p);
+    __syncthreads();
+    reg_fft<N, true>(v, rowl, j, p, tw);
+    float2* dst = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p));
+    const float sc = 1.0f / (float)N;
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) dst[j + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
+}
+
+// ---------------------------------------------------------------------------
+// P5: Re of the kx = 0 and kx = M bins, complex-to-real x transform, and the
+// iterate update.  Tile as P1.  The C2R split is
+//   Z[k] = (X[k] + conj X[M-k])/2 + i W_N^-k (X[k] - conj X[M-k])/2,
+//   z = IFFT_M(Z) / M,  x[2m] = Re z[m], x[2m+1] = Im z[m].
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfIterArgs a, const float2* __restrict__ twM,
+                                                                     const float2* __restrict__ twN)
+{
+    constexpr int M = N / 2;
+    using PL = RegPlan<M>;
+    constexpr int E = PL::E, T = M / E, R0 = PL::R0, RL = reg_last_radix<M>();
+    extern __shared__ float2 lds[];
+    float2* nyq = lds + kHP * M;   // X[M] of each pencil
+    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int n2 = a.n2, h = M + 1;
+    const int tpr = n2 / kHP;
+    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
+    {
+        const int pp = t % kHP, c = t / kHP;
+        float2* rp = lds + pp * M;
+        const float2* src = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
+        for (int kx = c; kx <= M; kx += T) {
+            float2 X = src[(int64_t)n2 * kx];
+            if (kx == 0 || kx == M) X.y = 0.0f;
+            if (kx < M) rp[swz(kx, pp)] = X;
+            else nyq[pp] = X;
+        }
+    }
+    __syncthreads();
+    float2* rowl = lds + p * M;
+    float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E / R0; ++i)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int e = j + T * i + r * (M / R0);
+            const float2 xk = rowl[swz(e, p)];
+            const float2 xm = conjf2(e == 0 ? nyq[p] : rowl[swz(M - e, p)]);
+            const float2 ev = make_float2(0.5f * (xk.x + xm.x), 0.5f * (xk.y + xm.y));
+            const float2 od = cscale(cmul(csub(xk, xm), conjf2(twN[e])), 0.5f);
+            v[i * R0 + r] = make_float2(ev.x - od.y, ev.y + od.x);   // ev + i*od
+        }
+    __syncthreads();
+    reg_fft<M, true>(v, rowl, j, p, twM);
+    const float sc = 1.0f / (float)M;
+    const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) {
+            const int m = j + T * i + q * (M / RL);
+            const int64_t ad = row + 2 * m;
+            const float2 pf = cscale(v[i * RL + q], sc);       // real(ifftn(k')) (:165)
+            if (a.pf_out) *reinterpret_cast<float2*>(a.pf_out + ad) = pf;
+            if (a.dr) {
+                const float2 xv = *reinterpret_cast<const float2*>(a.x + ad);
+                const uint16_t s2 = *reinterpret_cast<const uint16_t*>(a.S + ad);
+                const float ra = prh(xv.x, (uint8_t)(s2 & 0xFF), a.positivity, a.support);
+                const float rb = prh(xv.y, (uint8_t)(s2 >> 8), a.positivity, a.support);
+                *reinterpret_cast<float2*>(a.x + ad) = make_float2((xv.x + pf.x) - ra, (xv.y + pf.y) - rb);
+            } else {
+                *reinterpret_cast<float2*>(a.x + ad) = pf;
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Half-spectrum measured-voxel lists (pencil id ky + n2*kx, z ascending).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool hmeas(const float2* M, const uint8_t* mask, int64_t s)
+{
+    const float2 m = M[s];
+    return (m.x != 0.0f || m.y != 0.0f) && (mask == nullptr || mask[s] != 0);
+}
+
+struct HalfVox {
+    int64_t s, mate;
+    bool self_paired;
+};
+
+__device__ __forceinline__ HalfVox half_vox(int kx, int ky, int z, int n1, int n2, int n3)
+{
+    HalfVox v;
+    v.s = (int64_t)kx + (int64_t)n1 * (ky + (int64_t)n2 * z);
+    const int mx = (n1 - kx) % n1, my = (n2 - ky) % n2, mz = (n3 - z) % n3;
+    v.mate = (int64_t)mx + (int64_t)n1 * (my + (int64_t)n2 * mz);
+    v.self_paired = (kx == 0) || (2 * kx == n1);
+    return v;
+}
+
+__global__ void k_hcsr_count(const float2* __restrict__ M, const uint8_t* __restrict__ mask, int n1, int n2,
+                             int n3, uint32_t* __restrict__ cnt)
+{
+    const int h = n1 / 2 + 1;
+    const int64_t np = (int64_t)h * n2;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(q / n2), ky = (int)(q % n2);
+        uint32_t c = 0;
+        for (int z = 0; z < n3; ++z) {
+            const HalfVox hv = half_vox(kx, ky, z, n1, n2, n3);
+            const bool A = hmeas(M, mask, hv.s);
+            const bool B = !hv.self_paired && hmeas(M, mask, hv.mate);
+            c += (A || B) ? 1u : 0u;
+        }
+        cnt[q] = c;
+    }
+}
+
+__global__ void k_hcsr_fill(const float2* __restrict__ M, const uint8_t* __restrict__ mask, int n1, int n2,
+                            int n3, const int64_t* __restrict__ off, uint32_t* __restrict__ meta,
+                            float2* __restrict__ val, float2* __restrict__ mag)
+{
+    const int h = n1 / 2 + 1;
+    const int64_t np = (int64_t)h * n2;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(q / n2), ky = (int)(q % n2);
+        int64_t o = off[q];
+        for (int z = 0; z < n3; ++z) {
+            const HalfVox hv = half_vox(kx, ky, z, n1, n2, n3);
+            const bool A = hmeas(M, mask, hv.s);
+            const bool B = !hv.self_paired && hmeas(M, mask, hv.mate);
+            if (!A && !B) continue;
+            const float2 ms = M[hv.s];
+            const float2 mm = hv.self_paired ? make_float2(0.0f, 0.0f) : conjf2(M[hv.mate]);
+            uint32_t f = (uint32_t)z | ((uint32_t)ky << 11);
+            float2 V;
+            if (A && B) V = make_float2(0.5f * (ms.x + mm.x), 0.5f * (ms.y + mm.y));
+            else V = A ? ms : mm;
+            if (A) f |= kHalfTermA;
+            if (B) f |= kHalfTermB;
+            if (!hv.self_paired && A != B) f |= kHalfAverage;   // the mate is stored on the kx = 0, n1/2 planes
+            meta[o] = f;
+            val[o] = V;
+            mag[o] = make_float2(A ? sqrtf(ms.x * ms.x + ms.y * ms.y) : 0.0f,
+                                 B ? sqrtf(mm.x * mm.x + mm.y * mm.y) : 0.0f);
+            ++o;
+        }
+    }
+}
+
+__device__ __forceinline__ double kcoord(int c, int n)
+{
+    // make_Kspace_indices.m:19-31
+    if (n == 1) return 0.0;
+    if (n % 2 == 0) return (double)(c - n / 2) / (double)(n / 2);
+    return (double)(c - (n - 1) / 2) / (double)((n - 1) / 2);
+}
+
+// enforced = (1 - Q) > delay for the voxel (its mate has the same Q).
+__global__ void k_hcsr_enforce(int n1, int n2, int n3, const int64_t* __restrict__ off,
+                               uint32_t* __restrict__ meta, double delay)
+{
+    const int h = n1 / 2 + 1;
+    const int64_t np = (int64_t)h * n2;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(q / n2), ky = (int)(q % n2);
+        const double v1 = kcoord((kx + n1 / 2) % n1, n1);
+        const double v2 = kcoord((ky + n2 / 2) % n2, n2);
+        for (int64_t e = off[q]; e < off[q + 1]; ++e) {
+            uint32_t m = meta[e] & ~kHalfEnforced;
+            const int z = (int)(m & 0x7FFu);
+            const double v3 = kcoord((z + n3 / 2) % n3, n3);
+            const double Q = sqrt((v2 * v2 + v1 * v1) + v3 * v3);
+            if (1.0 - Q > delay) m |= kHalfEnforced;
+            meta[e] = m;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hcsr_den(const float2* __restrict__ mag, const uint32_t* __restrict__ meta,
+                                                  int64_t n, double* __restrict__ part)
+{
+    double t = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const uint32_t m = meta[e];
+        const float2 g = mag[e];
+        if (m & kHalfTermA) t += (double)g.x;
+        if (m & kHalfTermB) t += (double)g.y;
+    }
+    __shared__ double r[256];
+    r[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) r[threadIdx.x] += r[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = r[0];
+}
+
+__global__ void k_hsum(const double* part, int n, double* out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += part[i];
+        *out = s;
+    }
+}
+
+int pblocks(int64_t np) { return (int)std::min<int64_t>((np + 255) / 256, 4096); }
+
+// Dynamic LDS above 64 KiB (the 1024-point passes) must be requested per kernel.
+template <class K> void set_lds(K kern, size_t bytes)
+{
+    if (bytes > 65536)
+        GF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+template <int N> void launch_first(const HalfIterArgs& a, hipStream_t st)
+{
+    const size_t lds = (size_t)kHP * (N / 2) * sizeof(float2);
+    set_lds(k_half_first<N>, lds);
+    k_half_first<N><<<(a.n2 / kHP) * a.n3, half_threads<N / 2>(), lds, st>>>(a, fft_twiddles(N / 2), fft_twiddles(N));
+    GF_LAUNCH_CHECK();
+}
+template <int N> void launch_last(const HalfIterArgs& a, hipStream_t st)
+{
+    const size_t lds = (size_t)kHP * (N / 2 + 1) * sizeof(float2);
+    set_lds(k_half_last<N>, lds);
+    k_half_last<N><<<(a.n2 / kHP) * a.n3, half_threads<N / 2>(), lds, st>>>(a, fft_twiddles(N / 2), fft_twiddles(N));
+    GF_LAUNCH_CHECK();
+}
+template <int N> void launch_y(const HalfIterArgs& a, int inverse, hipStream_t st)
+{
+    const size_t lds = (size_t)kHP * N * sizeof(float2);
+    const int blocks = (a.n1 / 2 + 1) * (a.n3 / kHP);
+    if (inverse) {
+        set_lds(k_half_y_inv<N>, lds);
+        k_half_y_inv<N><<<blocks, half_threads<N>(), lds, st>>>(a, fft_twiddles(N));
+    } else {
+        set_lds(k_half_y_fwd<N>, lds);
+        k_half_y_fwd<N><<<blocks, half_threads<N>(), lds, st>>>(a, fft_twiddles(N));
+    }
+    GF_LAUNCH_CHECK();
+}
+template <int N> void launch_z(const HalfIterArgs& a, hipStream_t st)
+{
+    const size_t lds = (size_t)kHP * N * sizeof(float2);
+    set_lds(k_half_z_fused<N>, lds);
+    k_half_z_fused<N><<<(a.n1 / 2 + 1) * (a.n2 / kHP), half_threads<N>(), lds, st>>>(a, fft_twiddles(N));
+    GF_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+bool half_supported(int n1, int n2, int n3)
+{
+    auto p2 = [](int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; };
+    return p2(n1, 128, 1024) && p2(n2, 64, 1024) && p2(n3, 64, 1024) && n1 <= 2048 && n2 <= 2048;
+}
+
+int64_t half_partials(int n1, int n2, int n3) { (void)n3; return (int64_t)(n1 / 2 + 1) * (n2 / kHP); }
+
+#define GF_HALF_SWITCH(n, CALL)                                               \
+    switch (n) {                                                              \
+        case 64: CALL(64); break;                                             \
+        case 128: CALL(128); break;                                           \
+        case 256: CALL(256); break;                                           \
+        case 512: CALL(512); break;                                           \
+        case 1024: CALL(1024); break;                                         \
+        default: raise(GENFIRE_ERR_UNSUPPORTED, "half-spectrum length not supported"); \
+    }
+
+#define GF_HALF_SWITCH_X(n, CALL)                                             \
+    switch (n) {                                                              \
+        case 128: CALL(128); break;                                           \
+        case 256: CALL(256); break;                                           \
+        case 512: CALL(512); break;                                           \
+        case 1024: CALL(1024); break;                                         \
+        default: raise(GENFIRE_ERR_UNSUPPORTED, "half-spectrum x length not supported"); \
+    }
+
+void half_first(const HalfIterArgs& a, hipStream_t st)
+{
+#define C(NN) launch_first<NN>(a, st)
+    GF_HALF_SWITCH_X(a.n1, C)
+#undef C
+}
+void half_y(const HalfIterArgs& a, int inverse, hipStream_t st)
+{
+#define C(NN) launch_y<NN>(a, inverse, st)
+    GF_HALF_SWITCH(a.n2, C)
+#undef C
+}
+void half_z(const HalfIterArgs& a, hipStream_t st)
+{
+#define C(NN) launch_z<NN>(a, st)
+    GF_HALF_SWITCH(a.n3, C)
+#undef C
+}
+void half_last(const HalfIterArgs& a, hipStream_t st)
+{
+#define C(NN) launch_last<NN>(a, st)
+    GF_HALF_SWITCH_X(a.n1, C)
+#undef C
+}
+
+void hcsr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st)
+{
+    k_hcsr_count<<<pblocks((int64_t)(n1 / 2 + 1) * n2), 256, 0, st>>>(d_M, d_mask, n1, n2, n3, d_cnt);
+    GF_LAUNCH_CHECK();
+}
+void hcsr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_off,
+               uint32_t* d_meta, float2* d_val, float2* d_mag, hipStream_t st)
+{
+    k_hcsr_fill<<<pblocks((int64_t)(n1 / 2 + 1) * n2), 256, 0, st>>>(d_M, d_mask, n1, n2, n3, d_off, d_meta, d_val,
+                                                                       d_mag);
+    GF_LAUNCH_CHECK();
+}
+void hcsr_enforce(int n1, int n2, int n3, const int64_t* d_off, uint32_t* d_meta, double delay, hipStream_t st)
+{
+    k_hcsr_enforce<<<pblocks((int64_t)(n1 / 2 + 1) * n2), 256, 0, st>>>(n1, n2, n3, d_off, d_meta, delay);
+    GF_LAUNCH_CHECK();
+}
+void hcsr_den(const float2* d_mag, const uint32_t* d_meta, int64_t n, double* d_tmp, int ntmp, double* d_out,
+              hipStream_t st)
+{
+    k_hcsr_den<<<ntmp, 256, 0, st>>>(d_mag, d_meta, n, d_tmp);
+    GF_LAUNCH_CHECK();
+    k_hsum<<<1, 64, 0, st>>>(d_tmp, ntmp, d_out);
+    GF_LAUNCH_CHECK();
+}
+
+}  // namespace gf

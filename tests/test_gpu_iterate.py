@@ -8,23 +8,25 @@ from genfire_amd import _lib
 pytestmark = pytest.mark.gpu
 
 
-def _setup(O, dim=32, ratio=2.0, P=15, seed=0, v3=(1, 0, 0)):
+def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None):
+    """Blob phantom, trilinear tilt series and the oracle's measuredK.  dim2 != dim
+    makes the projections (and the grid) non-square: vol is dim x dim2 x dim."""
+    dim2 = dim if dim2 is None else dim2
     rng = np.random.default_rng(seed)
-    vol = np.zeros((dim, dim, dim), np.float32)
-    c = dim // 2
-    k = np.arange(dim) - c
+    vol = np.zeros((dim, dim2, dim), np.float32)
+    k1 = np.arange(dim) - dim // 2
+    k2 = np.arange(dim2) - dim2 // 2
     for _ in range(12):
-        x0 = rng.uniform(-dim / 5, dim / 5, 3)
+        x0 = rng.uniform(-min(dim, dim2) / 5, min(dim, dim2) / 5, 3)
         s = rng.uniform(1.0, 2.0)
-        g = [np.exp(-((k - q) ** 2) / (2 * s * s)) for q in x0]
+        g = [np.exp(-((kk - q) ** 2) / (2 * s * s)) for kk, q in zip((k1, k2, k1), x0)]
         vol += (rng.uniform(0.5, 1) * g[0][:, None, None] * g[1][None, :, None] * g[2][None, None, :]).astype(np.float32)
     ang = np.stack([np.zeros(P), np.linspace(-60, 60, P), np.zeros(P)], 1)
     projs = np.stack([O.project(vol, *a) for a in ang], 2).astype(np.float32)
-    n = int(round(dim * ratio))
-    g = O.grid_dft(projs, ang, n, n, 0.3)
-    sup = O.paddzero(np.ones((dim, dim, dim), np.float32), (dim, dim, dim)) > 0
-    kk = np.arange(dim) - c
-    sup = (kk[:, None, None] ** 2 + kk[None, :, None] ** 2 + kk[None, None, :] ** 2) <= (dim * 0.45) ** 2
+    n1, n2 = int(round(dim * ratio)), int(round(dim2 * ratio))
+    g = O.grid_dft(projs, ang, n1, n2, 0.3)
+    r = min(dim, dim2) * 0.45
+    sup = (k1[:, None, None] ** 2 + k2[None, :, None] ** 2 + k1[None, None, :] ** 2) <= r * r
     return vol, projs, ang, g.measuredK, sup
 
 
@@ -62,6 +64,25 @@ def test_iteration_matches_oracle(ctx, O, alg, mode):
     np.testing.assert_allclose(errR, ref.errR, atol=1e-4, rtol=0)
     np.testing.assert_allclose(R1, ref.Rarr_record, atol=1e-4, rtol=0)
     np.testing.assert_allclose(R2, ref.Rarr2_record, atol=1e-4, rtol=0)
+
+
+@pytest.mark.parametrize("alg", ["genfire", "dr"])
+@pytest.mark.parametrize("shape", [(64, 64), (64, 32)])
+def test_half_spectrum_iteration_matches_oracle(ctx, O, alg, shape):
+    """128^3 and 128 x 64 x 128 grids run the half-spectrum (R2C/C2R) passes."""
+    d1, d2 = shape
+    print(f"[iter] half spectrum alg={alg} dims={shape}", flush=True)
+    vol, projs, ang, mk, sup = _setup(O, dim=d1, dim2=d2, P=13, seed=3)
+    iters = 20
+    job = _job(ctx, mk, projs, ang, sup, iters, mode=1, alg=alg, monitor=1, L=10)
+    ref = O.reconstruct(mk, mk != 0, sup, d1, d2, iters, mode=1, algorithm=alg, precision="fp64",
+                        monitor_R=True, loop_length=10, projections=projs, angles=ang)
+    rec = job.reconstruction()
+    err = np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction)
+    assert err < 1e-4, err
+    np.testing.assert_allclose(job.errK(), ref.errK, atol=2e-5, rtol=0)
+    errR, R1, R2 = job.errR()
+    np.testing.assert_allclose(errR, ref.errR, atol=1e-4, rtol=0)
 
 
 def test_fixed_point(ctx, O):
