@@ -175,7 +175,8 @@ struct genfire_job {
     DevBuf x, K, S, best, pfbuf, err_part, errK, den_tmp, den, recon;
     DevBuf csr_cnt, csr_off, csr_meta, csr_val, csr_mag;
     int64_t nmeas = 0;
-    DevBuf vol, Rmat, rsums, rout;
+    DevBuf vol, vol_y, Rmat, rsums, rout;
+    bool rmat_any_y = false;
     std::vector<float> errK_h, errR_h;
     std::vector<double> Rarr_h, Rarr2_h;
     bool reconstructed = false;
@@ -242,6 +243,22 @@ void kstat_end(genfire_job* j, int k, std::pair<hipEvent_t, hipEvent_t>& ev)
     if (!j->prof) return;
     GF_HIP(hipEventRecord(ev.second, j->st));
     j->ks.pending[k].push_back(ev);
+}
+
+// ZYZ matrices of the projector (calculate3Dprojection_RealSpaceinterp.m:13-15),
+// kRStride floats per projection with the y-fastest flag at [9].
+std::vector<float> proj_matrices(const double* angles, int P, bool* any_y)
+{
+    std::vector<float> Rm((size_t)kRStride * P, 0.0f);
+    *any_y = false;
+    for (int p = 0; p < P; ++p) {
+        double R[9];
+        zyz_matrix(angles[p], angles[P + p], angles[2 * P + p], R);
+        float* r = Rm.data() + (size_t)kRStride * p;
+        for (int q = 0; q < 9; ++q) r[q] = (float)R[q];
+        if (project_wants_yfast(r)) { r[9] = 1.0f; *any_y = true; }
+    }
+    return Rm;
 }
 
 // ---------------------------------------------------------------------------
@@ -553,20 +570,21 @@ void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
     j->vol.alloc(sizeof(float) * box_size(j));
     crop_centre(d_model_fft, 0, j->n1, j->n2, j->n1, j->vol.as<float>(), j->dim1, j->dim2, j->dim1, st);
     if (!j->Rmat.p) {
-        std::vector<float> Rm((size_t)9 * P);
-        for (int p = 0; p < P; ++p) {
-            double R[9];
-            zyz_matrix(j->angles[p], j->angles[P + p], j->angles[2 * P + p], R);
-            for (int q = 0; q < 9; ++q) Rm[(size_t)9 * p + q] = (float)R[q];
-        }
+        std::vector<float> Rm = proj_matrices(j->angles.data(), P, &j->rmat_any_y);
         j->Rmat.alloc(sizeof(float) * Rm.size());
         GF_HIP(hipMemcpyAsync(j->Rmat.p, Rm.data(), sizeof(float) * Rm.size(), hipMemcpyHostToDevice, st));
+        GF_HIP(hipStreamSynchronize(st));
+    }
+    if (j->rmat_any_y) {
+        j->vol_y.alloc(sizeof(float) * box_size(j));
+        volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
     }
     const int nblk = project_blocks(j->dim1, j->dim2);
     j->rsums.alloc(sizeof(double) * (size_t)P * 4 * nblk);
     j->rout.alloc(sizeof(double) * (size_t)P * 4);
     ProjArgs pa{};
     pa.vol = j->vol.as<float>();
+    pa.vol_y = j->rmat_any_y ? j->vol_y.as<float>() : nullptr;
     pa.d1 = j->dim1; pa.d2 = j->dim2; pa.d3 = j->dim1;
     pa.R = j->Rmat.as<float>();
     pa.P = P;
@@ -960,19 +978,23 @@ int genfire_project(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, 
         GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: volume dimensions must be >= 2");
         hipStream_t st;
         GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        DevBuf v, r, o;
+        DevBuf v, vy, r, o;
         const size_t nv = (size_t)d1 * d2 * d3;
         v.alloc(sizeof(float) * nv);
-        r.alloc(sizeof(float) * 9);
         o.alloc(sizeof(float) * (size_t)d1 * d2);
-        double R[9];
-        float Rf[9];
-        zyz_matrix(phi, theta, psi, R);
-        for (int q = 0; q < 9; ++q) Rf[q] = (float)R[q];
+        const double ang[3] = {phi, theta, psi};
+        bool any_y = false;
+        std::vector<float> Rf = proj_matrices(ang, 1, &any_y);
+        r.alloc(sizeof(float) * Rf.size());
         GF_HIP(hipMemcpyAsync(v.p, vol, sizeof(float) * nv, hipMemcpyHostToDevice, st));
-        GF_HIP(hipMemcpyAsync(r.p, Rf, sizeof(Rf), hipMemcpyHostToDevice, st));
+        GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
+        if (any_y) {
+            vy.alloc(sizeof(float) * nv);
+            volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
+        }
         ProjArgs pa{};
         pa.vol = v.as<float>();
+        pa.vol_y = vy.as<float>();
         pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
         pa.R = r.as<float>();
         pa.P = 1;
@@ -995,26 +1017,27 @@ int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2
         GF_CHECK(p1 <= d1 && p2 <= d2, GENFIRE_ERR_ARG, "GENFIRE: projection larger than the model");
         hipStream_t st;
         GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        DevBuf v, r, o, m, s, s4;
+        DevBuf v, vy, r, o, m, s, s4;
         const size_t nv = (size_t)d1 * d2 * d3, np = (size_t)p1 * p2 * P;
         v.alloc(sizeof(float) * nv);
         m.alloc(sizeof(float) * np);
         o.alloc(sizeof(float) * np);
-        std::vector<float> Rf((size_t)9 * P);
-        for (int p = 0; p < P; ++p) {
-            double R[9];
-            zyz_matrix(angles[p], angles[P + p], angles[2 * P + p], R);
-            for (int q = 0; q < 9; ++q) Rf[(size_t)9 * p + q] = (float)R[q];
-        }
+        bool any_y = false;
+        std::vector<float> Rf = proj_matrices(angles, P, &any_y);
         r.alloc(sizeof(float) * Rf.size());
         GF_HIP(hipMemcpyAsync(v.p, model, sizeof(float) * nv, hipMemcpyHostToDevice, st));
         GF_HIP(hipMemcpyAsync(m.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, st));
         GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
+        if (any_y) {
+            vy.alloc(sizeof(float) * nv);
+            volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
+        }
         const int nblk = project_blocks(p1, p2);
         s.alloc(sizeof(double) * (size_t)P * 4 * nblk);
         s4.alloc(sizeof(double) * (size_t)P * 4);
         ProjArgs pa{};
         pa.vol = v.as<float>();
+        pa.vol_y = vy.as<float>();
         pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
         pa.R = r.as<float>();
         pa.P = P;

@@ -135,10 +135,14 @@ void hcsr_den(const float2* d_mag, const uint32_t* d_meta, int64_t n, double* d_
               hipStream_t st);
 
 // ---- projector (gf_project.hip) ----
+// Per projection the rotation is stored as kRStride floats: the column-major
+// 3x3 ZYZ matrix and, at [9], 1 when the y-fastest volume copy is to be used.
+constexpr int kRStride = 12;
 struct ProjArgs {
-    const float* vol;  // d1 x d2 x d3 column-major
+    const float* vol;  // d1 x d2 x d3 column-major (x fastest)
+    const float* vol_y; // the same volume y-fastest (index y + d2*(x + d1*z)), nullable when unused
     int d1, d2, d3;
-    const float* R;    // P x 9 (column-major 3x3 each), single
+    const float* R;    // P x kRStride, single
     int P;
     float* out;        // d1 x d2 x P back projections (nullable)
     const float* meas; // measured projections m1 x m2 x P (nullable)
@@ -147,6 +151,8 @@ struct ProjArgs {
     int post_hoc;      // 1: sum ||b|-|m|| instead of |b-m| (Tian_calc_Rfactor_realspace)
 };
 int project_blocks(int m1, int m2);
+bool project_wants_yfast(const float* R);
+void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipStream_t st);
 void project(const ProjArgs& a, hipStream_t st);
 void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st);
 

@@ -15,7 +15,6 @@ namespace gf {
 
 namespace {
 
-constexpr int kTile = 16;
 
 __device__ __forceinline__ bool lin_axis(float f, int n, int* i0, float* t)
 {
@@ -27,15 +26,42 @@ __device__ __forceinline__ bool lin_axis(float f, int n, int* i0, float* t)
     return true;
 }
 
-__global__ __launch_bounds__(kTile* kTile) void k_project(ProjArgs a)
+// Unaligned pair load: two adjacent floats of the volume in one 8-byte access.
+typedef float pair_t __attribute__((ext_vector_type(2), aligned(4)));
+
+// The workgroups of one projection run on one XCD: the hardware deals workgroups
+// to the 8 XCDs round-robin, so physical block b goes to XCD b % 8 and is mapped
+// to projection xcd + 8 * (slot / tiles) -- every XCD's L2 then streams whole
+// projections, neighbouring tiles sharing the voxels of their boundary rays.
+constexpr int kXcd = 8;
+constexpr int kLanes = 64;   // pixels of one wave along the fast pixel axis
+constexpr int kRows = 4;     // waves per workgroup along the other pixel axis
+
+// A wave's 64 lanes are 64 neighbouring pixels along the pixel axis whose
+// direction in the volume is closest to the volume's fastest axis: pixel i with
+// the x-fastest copy when |R00| >= |R11|, else pixel j with a y-fastest copy
+// (R[9] of the projection selects it).  For tilts about y the pixel-y direction
+// is the volume's y axis, so each gather of a wave reads a few contiguous runs
+// instead of one cache line per lane.
+__global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles)
 {
-    const int p = blockIdx.y;
-    const int tiles1 = (a.m1 + kTile - 1) / kTile;
-    const int ti = blockIdx.x % tiles1, tj = blockIdx.x / tiles1;
-    const int i = ti * kTile + (threadIdx.x % kTile);
-    const int j = tj * kTile + (threadIdx.x / kTile);
+    const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
+    const int p = xcd + kXcd * (slot / tiles);
+    const int tile = slot % tiles;
+    if (p >= a.P) return;
+    const float* R = a.R + kRStride * p;
+    const bool yfast = R[9] != 0.0f;
+    int i, j;
+    if (yfast) {
+        const int tiles2 = (a.m2 + kLanes - 1) / kLanes;
+        i = (tile / tiles2) * kRows + (int)(threadIdx.x / kLanes);
+        j = (tile % tiles2) * kLanes + (int)(threadIdx.x % kLanes);
+    } else {
+        const int tiles1 = (a.m1 + kLanes - 1) / kLanes;
+        i = (tile % tiles1) * kLanes + (int)(threadIdx.x % kLanes);
+        j = (tile / tiles1) * kRows + (int)(threadIdx.x / kLanes);
+    }
     const bool valid = i < a.m1 && j < a.m2;
-    const float* R = a.R + 9 * p;
     const int d1 = a.d1, d2 = a.d2, d3 = a.d3;
     // pixel (i,j) of the measured window sits at (i + s1, j + s2) of the d1 x d2
     // back projection (My_stripzero of the full projection, Tian...:16)
@@ -45,23 +71,67 @@ __global__ __launch_bounds__(kTile* kTile) void k_project(ProjArgs a)
     const float bx = R[0] * X + R[3] * Y + c1;
     const float by = R[1] * X + R[4] * Y + c2;
     const float bz = R[2] * X + R[5] * Y + c3;
+    const float rx = R[6], ry = R[7], rz = R[8];
     float acc = 0.0f;
     if (valid) {
-        for (int k = 0; k < d3; ++k) {
-            const float Z = (float)(k - d3 / 2);
-            const float fx = bx + R[6] * Z, fy = by + R[7] * Z, fz = bz + R[8] * Z;
-            int a0, b0, e0;
-            float ta, tb, te;
-            if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
-                continue;
-            const float* v = a.vol + (int64_t)a0 + (int64_t)d1 * (b0 + (int64_t)d2 * e0);
-            const int64_t sy = d1, sz = (int64_t)d1 * d2;
-            const float v000 = v[0], v100 = v[1], v010 = v[sy], v110 = v[sy + 1];
-            const float v001 = v[sz], v101 = v[sz + 1], v011 = v[sz + sy], v111 = v[sz + sy + 1];
-            const float x00 = v000 + ta * (v100 - v000), x10 = v010 + ta * (v110 - v010);
-            const float x01 = v001 + ta * (v101 - v001), x11 = v011 + ta * (v111 - v011);
-            const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
-            acc += y0 + te * (y1 - y0);
+        // steps whose sample can lie inside the box, from the three slab
+        // intervals widened by one step; the exact test stays per step
+        float klo = 0.0f, khi = (float)(d3 - 1);
+        const float bb[3] = {bx, by, bz}, rr[3] = {rx, ry, rz};
+        const float hi[3] = {(float)(d1 - 1), (float)(d2 - 1), (float)(d3 - 1)};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (rr[q] != 0.0f) {
+                const float t0 = (0.0f - bb[q]) / rr[q] + c3, t1 = (hi[q] - bb[q]) / rr[q] + c3;
+                klo = fmaxf(klo, fminf(t0, t1) - 1.0f);
+                khi = fminf(khi, fmaxf(t0, t1) + 1.0f);
+            } else if (!(bb[q] >= 0.0f && bb[q] <= hi[q])) {
+                khi = -1.0f;
+            }
+        }
+        const int k0 = (int)fminf((float)d3, fmaxf(0.0f, ceilf(klo)));
+        const int k1 = (int)fmaxf(-1.0f, fminf((float)(d3 - 1), floorf(khi)));
+        if (yfast) {
+            const float* vol = a.vol_y;           // index y + d2*(x + d1*z)
+            const int sx = d2, sz = d1 * d2;
+            for (int k = k0; k <= k1; ++k) {
+                const float Z = (float)(k - d3 / 2);
+                const float fx = bx + rx * Z, fy = by + ry * Z, fz = bz + rz * Z;
+                int a0, b0, e0;
+                float ta, tb, te;
+                if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
+                    continue;
+                const float* v = vol + (b0 + d2 * (a0 + d1 * e0));
+                const pair_t q00 = *reinterpret_cast<const pair_t*>(v);            // (a0,   b0..b0+1, e0)
+                const pair_t q10 = *reinterpret_cast<const pair_t*>(v + sx);       // (a0+1, ., e0)
+                const pair_t q01 = *reinterpret_cast<const pair_t*>(v + sz);       // (a0,   ., e0+1)
+                const pair_t q11 = *reinterpret_cast<const pair_t*>(v + sz + sx);  // (a0+1, ., e0+1)
+                // same interpolation order as the x-fastest branch: along x, y, then z
+                const float x00 = q00.x + ta * (q10.x - q00.x), x10 = q00.y + ta * (q10.y - q00.y);
+                const float x01 = q01.x + ta * (q11.x - q01.x), x11 = q01.y + ta * (q11.y - q01.y);
+                const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
+                acc += y0 + te * (y1 - y0);
+            }
+        } else {
+            const float* vol = a.vol;             // index x + d1*(y + d2*z)
+            const int sy = d1, sz = d1 * d2;
+            for (int k = k0; k <= k1; ++k) {
+                const float Z = (float)(k - d3 / 2);
+                const float fx = bx + rx * Z, fy = by + ry * Z, fz = bz + rz * Z;
+                int a0, b0, e0;
+                float ta, tb, te;
+                if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
+                    continue;
+                const float* v = vol + (a0 + d1 * (b0 + d2 * e0));
+                const pair_t q00 = *reinterpret_cast<const pair_t*>(v);
+                const pair_t q10 = *reinterpret_cast<const pair_t*>(v + sy);
+                const pair_t q01 = *reinterpret_cast<const pair_t*>(v + sz);
+                const pair_t q11 = *reinterpret_cast<const pair_t*>(v + sz + sy);
+                const float x00 = q00.x + ta * (q00.y - q00.x), x10 = q10.x + ta * (q10.y - q10.x);
+                const float x01 = q01.x + ta * (q01.y - q01.x), x11 = q11.x + ta * (q11.y - q11.x);
+                const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
+                acc += y0 + te * (y1 - y0);
+            }
         }
     }
     double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
@@ -79,7 +149,7 @@ __global__ __launch_bounds__(kTile* kTile) void k_project(ProjArgs a)
         }
     }
     if (!a.sums) return;
-    __shared__ double red[4][kTile * kTile / 64];
+    __shared__ double red[4][kRows];
     for (int off = 32; off > 0; off >>= 1) {
         q0 += __shfl_down(q0, off, 64);
         q1 += __shfl_down(q1, off, 64);
@@ -91,8 +161,28 @@ __global__ __launch_bounds__(kTile* kTile) void k_project(ProjArgs a)
     __syncthreads();
     if (threadIdx.x < 4) {
         double t = 0.0;
-        for (int k = 0; k < kTile * kTile / 64; ++k) t += red[threadIdx.x][k];
-        a.sums[((int64_t)p * 4 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+        for (int k = 0; k < kRows; ++k) t += red[threadIdx.x][k];
+        a.sums[((int64_t)p * 4 + threadIdx.x) * tiles + tile] = t;
+    }
+}
+
+// y-fastest copy of an x-fastest volume (dst[y + d2*(x + d1*z)] = src[x + d1*(y + d2*z)])
+// through a 32 x 32 LDS tile per z-slice.
+__global__ __launch_bounds__(256) void k_yfast(const float* __restrict__ src, float* __restrict__ dst, int d1, int d2,
+                                               int d3)
+{
+    __shared__ float t[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32, z = blockIdx.z;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    const int64_t zoff = (int64_t)d1 * d2 * z;
+    for (int r = ty; r < 32; r += 8) {
+        const int x = bx + tx, y = by + r;
+        if (x < d1 && y < d2) t[r][tx] = src[zoff + x + (int64_t)d1 * y];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int y = by + tx, x = bx + r;
+        if (x < d1 && y < d2) dst[zoff + y + (int64_t)d2 * x] = t[tx][r];
     }
 }
 
@@ -109,15 +199,29 @@ __global__ void k_project_reduce(const double* sums, int P, int nblk, double* ou
 
 int project_blocks(int m1, int m2)
 {
-    return ((m1 + kTile - 1) / kTile) * ((m2 + kTile - 1) / kTile);
+    // tiles are kLanes x kRows pixels, oriented either way: the larger count
+    const int a = ((m1 + kLanes - 1) / kLanes) * ((m2 + kRows - 1) / kRows);
+    const int b = ((m2 + kLanes - 1) / kLanes) * ((m1 + kRows - 1) / kRows);
+    return a > b ? a : b;
+}
+
+bool project_wants_yfast(const float* R) { return fabsf(R[4]) > fabsf(R[0]); }
+
+void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipStream_t st)
+{
+    dim3 grid((d1 + 31) / 32, (d2 + 31) / 32, d3);
+    k_yfast<<<grid, 256, 0, st>>>(d_src, d_dst, d1, d2, d3);
+    GF_LAUNCH_CHECK();
 }
 
 void project(const ProjArgs& a, hipStream_t st)
 {
     GF_CHECK(a.d1 >= 2 && a.d2 >= 2 && a.d3 >= 2, GENFIRE_ERR_ARG, "projector needs every volume dimension >= 2");
     GF_CHECK(a.m1 <= a.d1 && a.m2 <= a.d2, GENFIRE_ERR_ARG, "measured projection larger than the volume");
-    dim3 grid(project_blocks(a.m1, a.m2), a.P);
-    k_project<<<grid, kTile * kTile, 0, st>>>(a);
+    GF_CHECK((int64_t)a.d1 * a.d2 * a.d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "projector volume above 2^31 voxels");
+    const int tiles = project_blocks(a.m1, a.m2);
+    const int groups = (a.P + kXcd - 1) / kXcd;
+    k_project<<<(unsigned)(kXcd * groups * tiles), kLanes * kRows, 0, st>>>(a, tiles);
     GF_LAUNCH_CHECK();
 }
 
