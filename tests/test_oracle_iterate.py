@@ -1,0 +1,121 @@
+"""Pins the oracle's iteration (reconstruct.m) and projector without MATLAB:
+MATLAB round/linspace/colon semantics, a fixed point, idempotence of P_R,
+the last-iterate property of the schedule, DR fixed point and projector
+closed forms."""
+import numpy as np
+import pytest
+
+
+def test_schedule_matlab_round_half_away(O):
+    # round(linspace(1,200,5)) = [1 51 101 150 200]; numpy's half-to-even gives 100
+    np.testing.assert_array_equal(O.cutoff_schedule(200, 5), [1, 51, 101, 150, 200])
+    np.testing.assert_array_equal(O.cutoff_schedule(100, 5), [1, 26, 51, 75, 100])
+    assert O.cutoff_schedule(50, 27)[-1] == 50 and O.cutoff_schedule(50, 27)[0] == 1
+
+
+def test_delay_indicators(O):
+    d1 = O.delay_indicators(1)
+    assert len(d1) == 27 and len(O.delay_indicators(2)) == 14
+    np.testing.assert_allclose(d1[:12], np.arange(0.95, -0.16, -0.1), atol=1e-12)
+    assert d1[0] == 0.95 and d1[11] == -0.15 and d1[15] == -0.15 and d1[-1] == 0.95
+    np.testing.assert_array_equal(O.delay_indicators(3), [-999] * 5)
+    with pytest.raises(ValueError):
+        O.delay_indicators(4)
+
+
+def test_colon_symmetric(O):
+    v = O.matlab_colon(0.95, -0.1, -0.15)
+    assert len(v) == 12 and v[0] == 0.95 and v[-1] == -0.15
+    np.testing.assert_allclose(v, 0.95 - 0.1 * np.arange(12), atol=1e-15)
+
+
+def test_kspace_indices_closed_form(O):
+    Q = O.make_kspace_indices((8, 6, 5))
+    assert Q[4, 3, 2] == 0.0                    # centre (n/2+1 even, (n+1)/2 odd, 1-based)
+    assert Q[0, 3, 2] == 1.0 and Q[4, 0, 2] == 1.0 and Q[4, 3, 0] == 1.0
+
+
+def test_paddzero_stripzero_roundtrip(O):
+    rng = np.random.default_rng(0)
+    for small, big in [((4, 5, 6), (9, 10, 11)), ((5, 5, 5), (8, 8, 8)), ((6, 6, 6), (7, 9, 8))]:
+        v = rng.standard_normal(small).astype(np.float32)
+        np.testing.assert_array_equal(O.stripzero(O.paddzero(v, big), small), v)
+
+
+def test_iffshift_ind_matches_ifftshift(O):
+    shape = (6, 5, 4)
+    a = np.arange(np.prod(shape)).reshape(shape, order="F")
+    inds = np.arange(a.size)
+    sh = O.iffshift3_ind(shape, inds)
+    b = np.fft.ifftshift(a)
+    np.testing.assert_array_equal(b.ravel(order="F")[sh], a.ravel(order="F")[inds])
+
+
+def _obj(n=16, d=8, seed=0):
+    rng = np.random.default_rng(seed)
+    obj = np.zeros((d, d, d), np.float32)
+    obj[2:6, 2:6, 2:6] = rng.uniform(0.1, 1.0, (4, 4, 4))
+    M = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(O_pad(obj, n))))
+    return obj, M
+
+
+def O_pad(obj, n):
+    from oracle import oracle as O
+    return O.paddzero(obj, (n, n, n))
+
+
+@pytest.mark.parametrize("alg", ["genfire", "dr"])
+def test_fixed_point(O, alg):
+    """Every voxel measured and enforced with M = F(x*), x* positive in the
+    support: the iteration returns x* and errK reaches ~0."""
+    obj, M = _obj()
+    r = O.reconstruct(M, np.ones(M.shape, bool), obj > 0, 8, 8, 30, mode=3, algorithm=alg, precision="fp64")
+    np.testing.assert_allclose(r.reconstruction, obj, atol=1e-5)
+    assert r.errK[-1] < 1e-6
+
+
+def test_reconstruction_is_last_iterate(O):
+    """The schedule always changes at it = N (reconstruct.m:103,123), so the
+    returned object is P_R of the last iterate even when errK rose."""
+    obj, M = _obj(seed=2)
+    meas = np.random.default_rng(3).random(M.shape) < 0.5
+    Mm = np.where(meas, M, 0)
+    r10 = O.reconstruct(Mm, meas, obj > 0, 8, 8, 10, mode=3, precision="fp64")
+    r9 = O.reconstruct(Mm, meas, obj > 0, 8, 8, 9, mode=3, precision="fp64")
+    x9 = np.fft.fftshift(r9.last_iterate)
+    sup = O.paddzero((obj > 0).astype(np.float32), M.shape)
+    expect = O.stripzero(np.where(x9 < 0, 0, x9) * sup, (8, 8, 8))
+    np.testing.assert_allclose(r10.reconstruction, expect, atol=1e-6)
+
+
+def test_projector_axis_aligned_sums(O):
+    """phi = theta = psi = 0 samples voxel centres exactly: the projection is the
+    plain sum over z; theta = 90 sums along x (ZYZ about y)."""
+    rng = np.random.default_rng(1)
+    v = rng.uniform(0, 1, (6, 7, 8)).astype(np.float32)
+    np.testing.assert_allclose(O.project(v, 0, 0, 0), v.astype(np.float64).sum(axis=2), rtol=1e-12)
+    c = rng.uniform(0, 1, (7, 7, 7)).astype(np.float32)
+    c64 = c.astype(np.float64)
+    p90 = O.project(c, 0, 90, 0)
+    # R*[X;Y;Z] at theta = 90 is (Z, Y, -X): pixel i sums the line z = -X_i over x
+    ref = np.array([[c64[:, j, (7 // 2) - (i - 7 // 2)].sum() for j in range(7)] for i in range(7)])
+    np.testing.assert_allclose(p90, ref, rtol=1e-12)
+
+
+def test_projector_gaussian_closed_form(O):
+    """A 3D Gaussian projects to a 2D Gaussian of amplitude A*sigma*sqrt(2 pi)."""
+    n, s = 48, 3.0
+    k = np.arange(n) - n // 2
+    g = np.exp(-(k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2) / (2 * s * s))
+    proj = O.project(g.astype(np.float32), 20.0, 35.0, -10.0)
+    expect = s * np.sqrt(2 * np.pi) * np.exp(-(k[:, None] ** 2 + k[None, :] ** 2) / (2 * s * s))
+    assert np.abs(proj - expect).max() < 0.02 * expect.max()
+
+
+def test_rfactor_zero_for_self_consistent(O):
+    rng = np.random.default_rng(5)
+    v = rng.uniform(0, 1, (12, 12, 12)).astype(np.float32)
+    ang = rng.uniform(-30, 30, (3, 3))
+    projs = np.stack([O.project(v, *a) for a in ang], 2)
+    R1, R2 = O.rfactors(v, projs, ang)
+    assert np.all(R1 < 1e-12) and np.all(R2 < 1e-12)
