@@ -36,8 +36,10 @@ namespace {
 using namespace fftc;
 
 constexpr int kHP = 16;  // pencils per workgroup: 16 x 8 bytes = one 128-byte row
+constexpr int kZP = 8;   // pencils per workgroup of the fused z pass (contiguous pencils)
 
 template <int N> constexpr int half_threads() { return kHP * (N / RegPlan<N>::E); }
+template <int N> constexpr int zfused_threads() { return kZP * (N / RegPlan<N>::E); }
 
 __device__ __forceinline__ float prh(float v, uint8_t s, int positivity, int support)
 {
@@ -92,13 +94,19 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfIterAr
     const int pp = t % kHP, c = t / kHP;
     const float2* rp = lds + pp * M;
     float2* out = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
-    for (int kx = c; kx <= M; kx += T) {
+    auto split = [&](int kx) {
         const float2 zk = rp[swz(kx & (M - 1), pp)];
         const float2 zm = conjf2(rp[swz((M - kx) & (M - 1), pp)]);
         const float2 ev = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
         const float2 od = cscale(mul_mi<false>(csub(zk, zm)), 0.5f);
-        out[(int64_t)n2 * kx] = cadd(ev, cmul(twN[kx], od));
-    }
+        return cadd(ev, cmul(twN[kx], od));
+    };
+    float2 X[M / T];
+#pragma unroll
+    for (int q = 0; q < M / T; ++q) X[q] = split(c + q * T);
+#pragma unroll
+    for (int q = 0; q < M / T; ++q) out[(int64_t)n2 * (c + q * T)] = X[q];
+    if (c == 0) out[(int64_t)n2 * M] = split(M);   // Nyquist bin
 }
 
 // ---------------------------------------------------------------------------
@@ -130,24 +138,29 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfIterArgs a
     const int pp = t % kHP, c = t / kHP;
     const float2* rp = lds + pp * N;
     float2* dst = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
-    for (int e = c; e < N; e += T) dst[(int64_t)n3 * e] = rp[swz(e, pp)];
+    float2 w[N / T];
+#pragma unroll
+    for (int q = 0; q < N / T; ++q) w[q] = rp[swz(c + q * T, pp)];
+#pragma unroll
+    for (int q = 0; q < N / T; ++q) dst[(int64_t)n3 * (c + q * T)] = w[q];
 }
 
 // ---------------------------------------------------------------------------
 // P3: z forward, errK, replacement, z inverse, in place on B.  Tile: one kx,
-// 16 consecutive ky (16 contiguous pencils).
+// 8 consecutive ky (8 contiguous pencils; the kernel holds two transforms' worth
+// of registers, so smaller workgroups keep three of them resident per CU).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(half_threads<N>()) void k_half_z_fused(HalfIterArgs a, const float2* __restrict__ tw)
+__global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfIterArgs a, const float2* __restrict__ tw)
 {
     using PL = RegPlan<N>;
     constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
-    constexpr int NT = half_threads<N>();
+    constexpr int NT = zfused_threads<N>();
     extern __shared__ float2 lds[];
     const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = a.n2, n3 = N;
-    const int tpr = n2 / kHP;
-    const int kx = blockIdx.x / tpr, ky_0 = (blockIdx.x % tpr) * kHP;
+    const int tpr = n2 / kZP;
+    const int kx = blockIdx.x / tpr, ky_0 = (blockIdx.x % tpr) * kZP;
     float2* pen = a.B + (int64_t)n3 * ((ky_0 + p) + (int64_t)n2 * kx);
     float2 v[E];
 #pragma unroll
@@ -161,28 +174,38 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_z_fused(HalfIterArgs
 #pragma unroll
         for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
     __syncthreads();
-    // errK numerator (:135) and replacement (:163) from the half-spectrum lists
-    double err = 0.0;
+    // errK numerator (:135) and replacement (:163) from the half-spectrum lists;
+    // the numerator is reduced here (LDS past the tile) so that it is not live
+    // through the inverse transform
     {
+        float err = 0.0f;   // at most a few entries per lane
         const int64_t r0 = (int64_t)ky_0 + (int64_t)n2 * kx;
-        const int64_t beg = a.csr_off[r0], end = a.csr_off[r0 + kHP];
-        for (int64_t e = beg + t; e < end; e += NT) {
+        const int beg = (int)a.csr_off[r0], end = (int)a.csr_off[r0 + kZP];   // < 2^31 entries
+        for (int e = beg + t; e < end; e += NT) {
             const uint32_t m = a.csr_meta[e];
-            const int z = (int)(m & 0x7FFu), ky = (int)((m >> 11) & 0x7FFu);
-            const int pl = ky - ky_0;
+            const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
             float2* slot = lds + pl * N + swz(z, pl);
             const float2 kv = *slot;
             const float2 mg = a.csr_mag[e];
             const float ak = sqrtf(kv.x * kv.x + kv.y * kv.y);
-            if (m & kHalfTermA) err += (double)fabsf(ak - mg.x);
-            if (m & kHalfTermB) err += (double)fabsf(ak - mg.y);
+            if (m & kHalfTermA) err += fabsf(ak - mg.x);
+            if (m & kHalfTermB) err += fabsf(ak - mg.y);
             if (m & kHalfEnforced) {
                 const float2 V = a.csr_val[e];
                 *slot = (m & kHalfAverage) ? make_float2(0.5f * (V.x + kv.x), 0.5f * (V.y + kv.y)) : V;
             }
         }
+        double de = (double)err;
+        for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off, 64);
+        double* red = reinterpret_cast<double*>(lds + kZP * N);
+        if ((t & 63) == 0) red[t >> 6] = de;
+        __syncthreads();
+        if (t == 0) {
+            double sum = 0.0;
+            for (int w = 0; w < NT / 64; ++w) sum += red[w];
+            a.err_part[blockIdx.x] = sum;
+        }
     }
-    __syncthreads();
     reg_read<N, E, R0>(rowl, v, j, p);
     __syncthreads();
     reg_fft<N, true>(v, rowl, j, p, tw);
@@ -191,17 +214,6 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_z_fused(HalfIterArgs
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
         for (int q = 0; q < RL; ++q) pen[j + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
-    // one errK partial per workgroup (the tile's LDS is free after the last stage)
-    __syncthreads();
-    double* red = reinterpret_cast<double*>(lds);
-    for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
-    if ((t & 63) == 0) red[t >> 6] = err;
-    __syncthreads();
-    if (t == 0) {
-        double s = 0.0;
-        for (int w = 0; w < NT / 64; ++w) s += red[w];
-        a.err_part[blockIdx.x] = s;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -221,7 +233,11 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfIterArgs a
         const int pp = t % kHP, c = t / kHP;
         float2* rp = lds + pp * N;
         const float2* src = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
-        for (int e = c; e < N; e += T) rp[swz(e, pp)] = src[(int64_t)n3 * e];
+        float2 w[N / T];
+#pragma unroll
+        for (int q = 0; q < N / T; ++q) w[q] = src[(int64_t)n3 * (c + q * T)];
+#pragma unroll
+        for (int q = 0; q < N / T; ++q) rp[swz(c + q * T, pp)] = w[q];
     }
     __syncthreads();
     float2* rowl = lds + p * N;
@@ -260,12 +276,13 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfIterArg
         const int pp = t % kHP, c = t / kHP;
         float2* rp = lds + pp * M;
         const float2* src = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
-        for (int kx = c; kx <= M; kx += T) {
-            float2 X = src[(int64_t)n2 * kx];
-            if (kx == 0 || kx == M) X.y = 0.0f;
-            if (kx < M) rp[swz(kx, pp)] = X;
-            else nyq[pp] = X;
-        }
+        float2 w[M / T];
+#pragma unroll
+        for (int q = 0; q < M / T; ++q) w[q] = src[(int64_t)n2 * (c + q * T)];
+        if (c == 0) w[0].y = 0.0f;                 // Re of the kx = 0 bin
+#pragma unroll
+        for (int q = 0; q < M / T; ++q) rp[swz(c + q * T, pp)] = w[q];
+        if (c == 0) nyq[pp] = make_float2(src[(int64_t)n2 * M].x, 0.0f);   // Re of kx = M
     }
     __syncthreads();
     float2* rowl = lds + p * M;
@@ -475,9 +492,10 @@ template <int N> void launch_y(const HalfIterArgs& a, int inverse, hipStream_t s
 }
 template <int N> void launch_z(const HalfIterArgs& a, hipStream_t st)
 {
-    const size_t lds = (size_t)kHP * N * sizeof(float2);
+    // tile + one double per wave for the errK partial
+    const size_t lds = (size_t)kZP * N * sizeof(float2) + sizeof(double) * (zfused_threads<N>() / 64);
     set_lds(k_half_z_fused<N>, lds);
-    k_half_z_fused<N><<<(a.n1 / 2 + 1) * (a.n2 / kHP), half_threads<N>(), lds, st>>>(a, fft_twiddles(N));
+    k_half_z_fused<N><<<(a.n1 / 2 + 1) * (a.n2 / kZP), zfused_threads<N>(), lds, st>>>(a, fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
 
@@ -489,7 +507,7 @@ bool half_supported(int n1, int n2, int n3)
     return p2(n1, 128, 1024) && p2(n2, 64, 1024) && p2(n3, 64, 1024) && n1 <= 2048 && n2 <= 2048;
 }
 
-int64_t half_partials(int n1, int n2, int n3) { (void)n3; return (int64_t)(n1 / 2 + 1) * (n2 / kHP); }
+int64_t half_partials(int n1, int n2, int n3) { (void)n3; return (int64_t)(n1 / 2 + 1) * (n2 / kZP); }
 
 #define GF_HALF_SWITCH(n, CALL)                                               \
     switch (n) {                                                              \
