@@ -27,6 +27,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "recon/hour + gridded k-pts/s at 256³×91 proj; p50/p99 latency; R-factor Δ vs FP64"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the y-forward pass, keyed by grid edge; None where not
+# measured. rocprofv3 --pmc FETCH_SIZE (x2.00: the counter reads half of a known
+# 1 GiB read on MI355X) + WRITE_SIZE (x1.00, exact on a known 1 GiB fill), in
+# profiles/tilt256_hbm_traffic.md: 263218 KiB fetched, 526336 KiB written.
+TRAFFIC = {512: 2 * 263218 * 1024 + 526336 * 1024}
 
 
 def log(msg):
@@ -111,6 +116,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-kernels", type=int, default=1,
                     help="HIP-event timing of the FFT pass kernels for the roofline line")
+    ap.add_argument("--iterations", type=int, default=None,
+                    help="override the workload's iteration count (short profiling runs only)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a 1 GiB device-to-device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -118,7 +127,20 @@ def main():
 
     from genfire_amd import _lib, phantom
 
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 1 GiB read + 1 GiB write")
+        del src, dst
+
     w = phantom.WORKLOADS[args.workload]
+    if args.iterations:
+        import dataclasses
+        w = dataclasses.replace(w, iterations=args.iterations)
     ctx = _lib.Context(0)
     t0 = time.perf_counter()
     vol, projs, ang, sup = phantom.make_inputs(w, phantom.gpu_project_many(ctx))
@@ -165,17 +187,19 @@ def main():
     lat_ms = np.array(lat) * 1e3
     ms_step = float(np.mean(lat_ms))
     n = job.n1
-    grid_bytes = 16.0 * n ** 3                 # read + write of the complex64 grid per axis pass
+    h = n // 2 + 1
+    # P2 (y-forward pass of the half-spectrum iteration): one read and one write
+    # of the complex64 half spectrum
+    pass_bytes = 16.0 * h * n * n
     nfp, msfp = stats[_lib.KSTAT_FFT_PASS]
-    nfu, msfu = stats[_lib.KSTAT_FFT_FUSED]
     nit, msit = stats[_lib.KSTAT_ITERATION]
     pass_ms = msfp / max(nfp, 1)
-    achieved = grid_bytes / (pass_ms * 1e-3) / 1e9
+    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
     it_ms = msit / max(nit, 1)
-    # algorithmic bytes per iteration: 32 n^3 (two complex FFTs, one read+write each)
-    # + 12 per measured voxel + n^3/8 support bits
+    # algorithmic bytes per iteration with half-spectrum storage:
+    # 16 n^3 + 12 N_meas,half + n^3/8, N_meas,half ~ N_meas / 2
     nmeas = int(np.count_nonzero(job.measuredK()))
-    b_it = 32.0 * n ** 3 + 12.0 * nmeas + n ** 3 / 8.0
+    b_it = 16.0 * n ** 3 + 12.0 * (nmeas / 2) + n ** 3 / 8.0
 
     out = {
         "metric": METRIC,
@@ -202,9 +226,9 @@ def main():
         "rfactor_final": float(errR[-1]) if len(errR) else None,
         "iteration_roofline_frac": (b_it / (it_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_fft<{n}, plain> axis-1 pass, {grid_bytes / 1e9:.3f} GB per launch, "
-                               f"{pass_ms:.3f} ms avg over {nfp} launches"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(n),
+                     "kernel": f"k_half_y_fwd<{n}> (y-forward pass), {pass_bytes / 1e9:.3f} GB algorithmic "
+                               f"per launch, {pass_ms:.3f} ms mean over {nfp} launches"},
     }
     if not args.no_cpu_baseline:
         ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -4,6 +4,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -164,7 +165,7 @@ struct genfire_job {
     bool have_inputs = false, have_support = false, have_init = false;
 
     // gridding
-    DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles;
+    DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles, Pt;
     DevBuf acc, sw, hits, exact;
     DevBuf M, mask, recifft;
     std::vector<int64_t> proj_off;
@@ -317,18 +318,30 @@ void do_grid(genfire_job* j)
     grid_index_emit(j->planes.as<PlaneConst>(), P, j->hg, cutoff, n1, n2, nblk, j->blk_offsets.as<int64_t>(),
                     j->ind.as<uint32_t>(), j->dist.as<float>(), j->un.as<float>(), j->vn.as<float>(), st);
 
-    // DFT values (:236-248), tiles of 256 points inside one projection
+    // DFT values (:236-248).  Default: the MFMA GEMM form on 64-point tiles;
+    // GENFIRE_GRID_KERNEL=valu selects the FP32 VALU kernel (256-point tiles).
+    const char* gk = std::getenv("GENFIRE_GRID_KERNEL");
+    const bool use_valu = gk && std::string(gk) == "valu";
+    const int tile_pts = use_valu ? 256 : mfma_tile_points();
     std::vector<PointTile> tl;
     for (int p = 0; p < P; ++p)
-        for (int64_t b = j->proj_off[p]; b < j->proj_off[p + 1]; b += 256)
-            tl.push_back({b, (int32_t)std::min<int64_t>(256, j->proj_off[p + 1] - b), p});
+        for (int64_t b = j->proj_off[p]; b < j->proj_off[p + 1]; b += tile_pts)
+            tl.push_back({b, (int32_t)std::min<int64_t>(tile_pts, j->proj_off[p + 1] - b), p});
     if (!tl.empty()) {
         j->tiles.alloc(sizeof(PointTile) * tl.size());
         GF_HIP(hipMemcpyAsync(j->tiles.p, tl.data(), sizeof(PointTile) * tl.size(), hipMemcpyHostToDevice, st));
+        if (!use_valu) {
+            j->Pt.alloc(sizeof(float) * grid_prep_floats(j->dim1, j->dim2, P));
+            grid_prep_proj(j->proj.as<float>(), j->dim1, j->dim2, P, j->Pt.as<float>(), st);
+        }
         std::pair<hipEvent_t, hipEvent_t> ev;
         kstat_begin(j, GENFIRE_KSTAT_GRID_VALUES, &ev);
-        grid_values_fp32(j->proj.as<float>(), j->dim1, j->dim2, j->un.as<float>(), j->vn.as<float>(),
-                         j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
+        if (use_valu)
+            grid_values_fp32(j->proj.as<float>(), j->dim1, j->dim2, j->un.as<float>(), j->vn.as<float>(),
+                             j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
+        else
+            grid_values_mfma_fp32(j->Pt.as<float>(), j->dim1, j->dim2, j->un.as<float>(), j->vn.as<float>(),
+                                  j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
         kstat_end(j, GENFIRE_KSTAT_GRID_VALUES, ev);
     }
 

@@ -26,6 +26,13 @@ struct PointTile {
 // FP32 direct DFT of every point (fillInFourierGrid_DFT.m:236-248).
 void grid_values_fp32(const float* d_proj, int dim1, int dim2, const float* d_un, const float* d_vn,
                       const PointTile* d_tiles, int ntiles, float2* d_F, hipStream_t st);
+// MFMA form: the projections pre-transposed to row-major [a][b] (zero-padded),
+// then 64-point tiles (mfma_tile_points) through v_mfma_f32_16x16x4f32.
+int mfma_tile_points();
+size_t grid_prep_floats(int dim1, int dim2, int P);
+void grid_prep_proj(const float* d_proj, int dim1, int dim2, int P, float* d_Pt, hipStream_t st);
+void grid_values_mfma_fp32(const float* d_Pt, int dim1, int dim2, const float* d_un, const float* d_vn,
+                           const PointTile* d_tiles, int ntiles, float2* d_F, hipStream_t st);
 // Weighted-average accumulation of one projection's points (:267-291), in order.
 void grid_scatter(const uint32_t* d_ind, const float* d_dist, const float2* d_F, int64_t begin,
                   int64_t count, double2* d_acc, double* d_sw, uint32_t* d_hits, float2* d_exact,

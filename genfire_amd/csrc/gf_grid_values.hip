@@ -283,4 +283,161 @@ void pad_centre_u8(const uint8_t* d_src, int d1, int d2, int d3, uint8_t* d_dst,
     GF_LAUNCH_CHECK();
 }
 
+
+// ===========================================================================
+// MFMA form of the DFT values (fillInFourierGrid_DFT.m:243 as a complex GEMM).
+// G[j, b] = sum_a E1[j, a] P[a, b] is a real GEMM with the E1 real and imaginary
+// parts as two A operands sharing the B operand P; F_j = sum_b E2[j, b] G[j, b].
+// A wave owns 16 points (the 16 rows of v_mfma_f32_16x16x4f32); lane l supplies
+// A[l%16][l/16] = E1(point l%16, a) built from two LDS tables of FP64-reduced
+// phases (E1 = E1hi(a/16) * E1lo(a%16)), B[l/16][l%16] = P[a][b] from LDS, and
+// holds G[4(l/16)+r][16nt + l%16].  The E2 contraction runs on the accumulators
+// and ends in a 16-lane reduction.  P is pre-transposed to row-major
+// [a][b] (zero-padded) once per job.
+// ===========================================================================
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kMP = 64;    // points per workgroup (4 waves x 16)
+constexpr int kKC = 32;    // projection rows staged per step
+
+__global__ void k_transpose_proj(const float* __restrict__ proj, int dim1, int dim2, int P, int Kp, int Np,
+                                 float* __restrict__ Pt)
+{
+    const int64_t total = (int64_t)P * Kp * Np;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int b = (int)(t % Np), a = (int)((t / Np) % Kp), p = (int)(t / ((int64_t)Np * Kp));
+        Pt[t] = (a < dim1 && b < dim2) ? proj[a + (int64_t)dim1 * (b + (int64_t)dim2 * p)] : 0.0f;
+    }
+}
+
+template <int NCH>
+__global__ __launch_bounds__(256) void k_dft_mfma(const float* __restrict__ Pt, int Kp, int Np, int dim1, int dim2,
+                                                  const float* __restrict__ un, const float* __restrict__ vn,
+                                                  const PointTile* __restrict__ tiles, float2* __restrict__ F)
+{
+    constexpr int STR = NCH + 16;           // LDS row stride: 4 rows of 16 lanes on distinct banks
+    constexpr int NTT = NCH / 16;
+    extern __shared__ float smem[];
+    const int H1 = Kp / 16, H2 = Np / 16;
+    float* Pl = smem;                                         // kKC x STR
+    float2* t1hi = reinterpret_cast<float2*>(Pl + kKC * STR); // [H1][64]
+    float2* t1lo = t1hi + H1 * kMP;                           // [16][64]
+    float2* t2hi = t1lo + 16 * kMP;                           // [H2][64]
+    float2* t2lo = t2hi + H2 * kMP;                           // [16][64]
+    const PointTile tl = tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int c1 = dim1 / 2, c2 = dim2 / 2;
+    for (int idx = tid; idx < kMP * (H1 + H2 + 32); idx += 256) {
+        const int pt = idx % kMP, row = idx / kMP;
+        const bool ok = pt < tl.count;
+        const float t1 = ok ? un[tl.begin + pt] : 0.0f, t2 = ok ? vn[tl.begin + pt] : 0.0f;
+        if (row < H1) t1hi[row * kMP + pt] = cis_turns((double)(16 * row - c1), t1);
+        else if (row < H1 + 16) t1lo[(row - H1) * kMP + pt] = cis_turns((double)(row - H1), t1);
+        else if (row < H1 + 16 + H2) t2hi[(row - H1 - 16) * kMP + pt] = cis_turns((double)(16 * (row - H1 - 16) - c2), t2);
+        else t2lo[(row - H1 - 16 - H2) * kMP + pt] = cis_turns((double)(row - H1 - 16 - H2), t2);
+    }
+    const float* Pp = Pt + (int64_t)tl.proj * Kp * Np;
+    const int w = tid / 64, l = tid % 64, li = l % 16, lk = l / 16;
+    const int pa = 16 * w + li;
+    float Fr[4] = {0.0f, 0.0f, 0.0f, 0.0f}, Fi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int nc = 0; nc < Np; nc += NCH) {
+        v4f acr[NTT], aci[NTT];
+#pragma unroll
+        for (int nt = 0; nt < NTT; ++nt) { acr[nt] = v4f{0, 0, 0, 0}; aci[nt] = v4f{0, 0, 0, 0}; }
+        for (int kc = 0; kc < Kp; kc += kKC) {
+            __syncthreads();
+            for (int idx = tid; idx < kKC * NCH; idx += 256) {
+                const int k = idx / NCH, c = idx % NCH;
+                Pl[k * STR + c] = (kc + k < Kp && nc + c < Np) ? Pp[(int64_t)(kc + k) * Np + nc + c] : 0.0f;
+            }
+            __syncthreads();
+            for (int kk = 0; kk < kKC / 4; ++kk) {
+                const int a = kc + 4 * kk + lk;
+                const float2 e1 = a < Kp ? cmul(t1hi[(a >> 4) * kMP + pa], t1lo[(a & 15) * kMP + pa])
+                                         : make_float2(0.0f, 0.0f);
+                const float* prow = Pl + (4 * kk + lk) * STR + li;
+#pragma unroll
+                for (int nt = 0; nt < NTT; ++nt) {
+                    const float b = prow[16 * nt];
+                    acr[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.x, b, acr[nt], 0, 0, 0);
+                    aci[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(e1.y, b, aci[nt], 0, 0, 0);
+                }
+            }
+        }
+        // F += sum_b E2(pt, b) G(pt, b): lane holds columns nc + 16nt + li of points 16w + 4lk + r
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int pt = 16 * w + 4 * lk + r;
+            float hr = 0.0f, hi = 0.0f;
+#pragma unroll
+            for (int nt = 0; nt < NTT; ++nt) {
+                const int m = nc / 16 + nt;
+                if (m < H2) {
+                    const float2 e = t2hi[m * kMP + pt];
+                    const float gr = acr[nt][r], gi = aci[nt][r];
+                    hr += gr * e.x - gi * e.y;
+                    hi += gr * e.y + gi * e.x;
+                }
+            }
+            const float2 el = t2lo[li * kMP + pt];
+            float fr = hr * el.x - hi * el.y, fi = hr * el.y + hi * el.x;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                fr += __shfl_xor(fr, off, 64);
+                fi += __shfl_xor(fi, off, 64);
+            }
+            Fr[r] += fr;
+            Fi[r] += fi;
+        }
+    }
+    if (li == 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int pt = 16 * w + 4 * lk + r;
+            if (pt < tl.count) F[tl.begin + pt] = make_float2(Fr[r], Fi[r]);
+        }
+}
+
+}  // namespace
+
+int mfma_tile_points() { return kMP; }
+
+void grid_prep_proj(const float* d_proj, int dim1, int dim2, int P, float* d_Pt, hipStream_t st)
+{
+    const int Kp = (dim1 + kKC - 1) / kKC * kKC, Np = (dim2 + 15) / 16 * 16;
+    const int64_t total = (int64_t)P * Kp * Np;
+    k_transpose_proj<<<grid_blocks(total, 256), 256, 0, st>>>(d_proj, dim1, dim2, P, Kp, Np, d_Pt);
+    GF_LAUNCH_CHECK();
+}
+
+size_t grid_prep_floats(int dim1, int dim2, int P)
+{
+    return (size_t)P * ((dim1 + kKC - 1) / kKC * kKC) * ((dim2 + 15) / 16 * 16);
+}
+
+void grid_values_mfma_fp32(const float* d_Pt, int dim1, int dim2, const float* d_un, const float* d_vn,
+                           const PointTile* d_tiles, int ntiles, float2* d_F, hipStream_t st)
+{
+    if (ntiles <= 0) return;
+    const int Kp = (dim1 + kKC - 1) / kKC * kKC, Np = (dim2 + 15) / 16 * 16;
+    const int H1 = Kp / 16, H2 = Np / 16;
+    auto lds = [&](int nch) {
+        return sizeof(float) * kKC * (nch + 16) + sizeof(float2) * kMP * (H1 + H2 + 32);
+    };
+    if (Np >= 128) {
+        const size_t b = lds(128);
+        GF_CHECK(b <= 160 * 1024, GENFIRE_ERR_UNSUPPORTED, "projection too large for the MFMA DFT kernel");
+        if (b > 65536)
+            GF_HIP(hipFuncSetAttribute((const void*)k_dft_mfma<128>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b));
+        k_dft_mfma<128><<<ntiles, 256, b, st>>>(d_Pt, Kp, Np, dim1, dim2, d_un, d_vn, d_tiles, d_F);
+    } else if (Np >= 64) {
+        k_dft_mfma<64><<<ntiles, 256, lds(64), st>>>(d_Pt, Kp, Np, dim1, dim2, d_un, d_vn, d_tiles, d_F);
+    } else {
+        k_dft_mfma<16><<<ntiles, 256, lds(16), st>>>(d_Pt, Kp, Np, dim1, dim2, d_un, d_vn, d_tiles, d_F);
+    }
+    GF_LAUNCH_CHECK();
+}
+
 }  // namespace gf
