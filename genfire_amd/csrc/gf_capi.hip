@@ -409,7 +409,7 @@ void do_reconstruct(genfire_job* j)
     j->K.alloc(sizeof(float2) * (half ? std::max<int64_t>(2 * hn, nf) : nf));
     j->S.alloc(nf);
     j->best.alloc(sizeof(float) * nf);
-    if (dr && monitor) j->pfbuf.alloc(sizeof(float) * nf);
+    if (monitor && (dr || half)) j->pfbuf.alloc(sizeof(float) * nf);
     IterArgs ia{};
     ia.n1 = n1; ia.n2 = n2; ia.n3 = n3;
     GF_CHECK(n1 <= 2048 && n3 <= 2048, GENFIRE_ERR_UNSUPPORTED, "GENFIRE: grid edge above 2048");
@@ -524,9 +524,13 @@ void do_reconstruct(genfire_job* j)
         std::pair<hipEvent_t, hipEvent_t> e_it, e_pass, e_fused;
         kstat_begin(j, GENFIRE_KSTAT_ITERATION, &e_it);
         if (half) {
-            ha.best = ia.best;
-            ha.pf_out = ia.pf_out;
-            half_first(ha, st);
+            // P1 runs on its own only in the first iteration; later ones get it
+            // from the fused P5 + P1 pass that ends the iteration before.
+            if (it == 1) {
+                ha.best = ia.best;
+                ha.pf_out = nullptr;
+                half_first(ha, st);
+            }
             kstat_begin(j, GENFIRE_KSTAT_FFT_PASS, &e_pass);
             half_y(ha, 0, st);
             kstat_end(j, GENFIRE_KSTAT_FFT_PASS, e_pass);
@@ -534,7 +538,14 @@ void do_reconstruct(genfire_job* j)
             half_z(ha, st);
             kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
             half_y(ha, 1, st);
-            half_last(ha, st);
+            ha.pf_out = mon_it ? j->pfbuf.as<float>() : nullptr;
+            if (it < N) {
+                ha.best = (it + 1 == N) ? j->best.as<float>() : nullptr;
+                half_last_first(ha, st);
+            } else {
+                ha.best = nullptr;
+                half_last(ha, st);
+            }
         } else {
             iter_first(ia, st);
             kstat_begin(j, GENFIRE_KSTAT_FFT_PASS, &e_pass);
@@ -549,7 +560,7 @@ void do_reconstruct(genfire_job* j)
         iter_errk_reduce(j->err_part.as<double>(), nparts, den, j->errK.as<float>() + (it - 1), st);
         kstat_end(j, GENFIRE_KSTAT_ITERATION, e_it);
         if (mon_it) {
-            const float* model = dr ? j->pfbuf.as<float>() : j->x.as<float>();
+            const float* model = (dr || half) ? j->pfbuf.as<float>() : j->x.as<float>();
             run_rfactor(j, model, it / L - 1);
         }
     }

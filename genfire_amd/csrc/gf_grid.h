@@ -134,6 +134,7 @@ void half_first(const HalfIterArgs& a, hipStream_t st);
 void half_y(const HalfIterArgs& a, int inverse, hipStream_t st);
 void half_z(const HalfIterArgs& a, hipStream_t st);
 void half_last(const HalfIterArgs& a, hipStream_t st);
+void half_last_first(const HalfIterArgs& a, hipStream_t st);   // P5 of one iteration + P1 of the next
 void hcsr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st);
 void hcsr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_off,
                uint32_t* d_meta, float2* d_val, float2* d_mag, hipStream_t st);

@@ -24,6 +24,7 @@
 //   P3 z: B pencils (contiguous) -> FFT -> errK + replacement -> IFFT -> B
 //   P4 y: B (strided) -> IFFT -> A
 //   P5 x: A (strided) -> Re of kx = 0, n1/2 -> C2R -> x (GENFIRE) or DR update
+// Between two iterations P5 and the next P1 run as one pass (k_half_last_first).
 #include <cmath>
 
 #include "gf_fft_core.h"
@@ -52,37 +53,45 @@ __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.
 
 // ---------------------------------------------------------------------------
 // P1: P_R and the real-to-complex x transform.  N = n1, complex length M = N/2.
-// Tile: 16 consecutive y rows at one z.  Lane (p, j), p = t / T, reads the
-// element pair (2e, 2e+1) of row p as z[e] = x[2e] + i x[2e+1].
+// Tile: 16 consecutive y rows at one z.  Lane (p, j), p = t / T, owns the
+// element pairs (2m, 2m+1), m = j + T*s (s = 0..E-1), of row p as
+// z[m] = x[2m] + i x[2m+1].  The forward FFT takes element s in register slot
+// i*R0 + r with s = i + r*E/R0; the inverse FFT leaves element s in slot
+// i*RL + q with s = i + q*E/RL (RL its last radix).
 // ---------------------------------------------------------------------------
+template <int M> __device__ __forceinline__ constexpr int in_slot(int s)
+{
+    return (s % (RegPlan<M>::E / RegPlan<M>::R0)) * RegPlan<M>::R0 + s / (RegPlan<M>::E / RegPlan<M>::R0);
+}
+template <int M> __device__ __forceinline__ constexpr int out_slot(int s)
+{
+    constexpr int RL = reg_last_radix<M>();
+    return (s % (RegPlan<M>::E / RL)) * RL + s / (RegPlan<M>::E / RL);
+}
+
+// threadIdx.x behind an empty asm statement.  Each phase of a fused pass takes
+// its lane indices from this, so the compiler recomputes the phase's addresses
+// instead of keeping those of an earlier phase live across the whole kernel.
+__device__ __forceinline__ int phase_tid()
+{
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+// Forward M-point FFT of the tile's rows (v in input slots), the R2C split
+//   X[k] = (Z[k] + conj Z[M-k])/2 + W_N^k (Z[k] - conj Z[M-k])/(2i), k = 0..M,
+// and the transposed store to layout A.
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfIterArgs a, const float2* __restrict__ twM,
-                                                                      const float2* __restrict__ twN)
+__device__ __forceinline__ void r2c_store(const HalfIterArgs& a, float2* lds, float2 (&v)[RegPlan<N / 2>::E],
+                                          int i2_0, int i3, const float2* __restrict__ twM,
+                                          const float2* __restrict__ twN)
 {
     constexpr int M = N / 2;
     using PL = RegPlan<M>;
-    constexpr int E = PL::E, T = M / E, R0 = PL::R0, RL = reg_last_radix<M>();
-    extern __shared__ float2 lds[];
-    const int t = threadIdx.x, j = t % T, p = t / T;
+    constexpr int E = PL::E, T = M / E, RL = reg_last_radix<M>();
+    const int t = phase_tid(), j = t % T, p = t / T;
     const int n2 = a.n2, h = M + 1;
-    const int tpr = n2 / kHP;
-    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
-    const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
-
-    float2 v[E];
-#pragma unroll
-    for (int i = 0; i < E / R0; ++i)
-#pragma unroll
-        for (int r = 0; r < R0; ++r) {
-            const int e = j + T * i + r * (M / R0);
-            const int64_t ad = row + 2 * e;
-            const float2 xv = *reinterpret_cast<const float2*>(a.x + ad);
-            const uint16_t s2 = *reinterpret_cast<const uint16_t*>(a.S + ad);
-            const float ra = prh(xv.x, (uint8_t)(s2 & 0xFF), a.positivity, a.support);
-            const float rb = prh(xv.y, (uint8_t)(s2 >> 8), a.positivity, a.support);
-            if (a.best) *reinterpret_cast<float2*>(a.best + ad) = make_float2(ra, rb);
-            v[i * R0 + r] = a.dr ? make_float2(2.0f * ra - xv.x, 2.0f * rb - xv.y) : make_float2(ra, rb);
-        }
     float2* rowl = lds + p * M;
     reg_fft<M, false>(v, rowl, j, p, twM);
 #pragma unroll
@@ -90,7 +99,6 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfIterAr
 #pragma unroll
         for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (M / RL), p)] = v[i * RL + q];
     __syncthreads();
-    // X[k] = (Z[k] + conj Z[M-k])/2 + W_N^k (Z[k] - conj Z[M-k])/(2i), k = 0..M
     const int pp = t % kHP, c = t / kHP;
     const float2* rp = lds + pp * M;
     float2* out = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
@@ -107,6 +115,46 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfIterAr
 #pragma unroll
     for (int q = 0; q < M / T; ++q) out[(int64_t)n2 * (c + q * T)] = X[q];
     if (c == 0) out[(int64_t)n2 * M] = split(M);   // Nyquist bin
+}
+
+// The value P1 transforms for the real pair xv: P_R(x) (GENFIRE) or the
+// reflection 2 P_R(x) - x (DR); P_R(x) also goes to best when that is set.
+__device__ __forceinline__ float2 p1_value(const HalfIterArgs& a, float2 xv, uint16_t s2, int64_t ad)
+{
+    const float ra = prh(xv.x, (uint8_t)(s2 & 0xFF), a.positivity, a.support);
+    const float rb = prh(xv.y, (uint8_t)(s2 >> 8), a.positivity, a.support);
+    if (a.best) *reinterpret_cast<float2*>(a.best + ad) = make_float2(ra, rb);
+    return a.dr ? make_float2(2.0f * ra - xv.x, 2.0f * rb - xv.y) : make_float2(ra, rb);
+}
+
+template <int N>
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfIterArgs a, const float2* __restrict__ twM,
+                                                                      const float2* __restrict__ twN)
+{
+    constexpr int M = N / 2;
+    using PL = RegPlan<M>;
+    constexpr int E = PL::E, T = M / E;
+    extern __shared__ float2 lds[];
+    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int n2 = a.n2;
+    const int tpr = n2 / kHP;
+    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
+    const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
+
+    // All loads are issued before any store (best may be written), so they are
+    // in flight together.
+    float2 xv[E];
+    uint16_t s2[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int64_t ad = row + 2 * (j + T * s);
+        xv[s] = *reinterpret_cast<const float2*>(a.x + ad);
+        s2[s] = *reinterpret_cast<const uint16_t*>(a.S + ad);
+    }
+    float2 v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[in_slot<M>(s)] = p1_value(a, xv[s], s2[s], row + 2 * (j + T * s));
+    r2c_store<N>(a, lds, v, i2_0, i3, twM, twN);
 }
 
 // ---------------------------------------------------------------------------
@@ -151,7 +199,8 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfIterArgs a
 // of registers, so smaller workgroups keep three of them resident per CU).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfIterArgs a, const float2* __restrict__ tw)
+__global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfIterArgs a, const float2* __restrict__ tw,
+                                                                     const float2* __restrict__ tw2)
 {
     using PL = RegPlan<N>;
     constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
@@ -167,6 +216,19 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfIterAr
     for (int i = 0; i < E / R0; ++i)
 #pragma unroll
         for (int r = 0; r < R0; ++r) v[i * R0 + r] = pen[j + T * i + r * (N / R0)];
+    // The tile's list range and each lane's first entry are loaded before the
+    // forward transform, which hides their latency (a tile has fewer entries
+    // than lanes in most of the grid).
+    const int64_t r0 = (int64_t)ky_0 + (int64_t)n2 * kx;
+    const int beg = (int)a.csr_off[r0], end = (int)a.csr_off[r0 + kZP];   // < 2^31 entries
+    const int e0 = beg + t;
+    uint32_t m0 = 0;
+    float2 mg0 = make_float2(0.0f, 0.0f), V0 = mg0;
+    if (e0 < end) {
+        m0 = a.csr_meta[e0];
+        mg0 = a.csr_mag[e0];
+        V0 = a.csr_val[e0];
+    }
     float2* rowl = lds + p * N;
     reg_fft<N, false>(v, rowl, j, p, tw);
 #pragma unroll
@@ -179,21 +241,22 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfIterAr
     // through the inverse transform
     {
         float err = 0.0f;   // at most a few entries per lane
-        const int64_t r0 = (int64_t)ky_0 + (int64_t)n2 * kx;
-        const int beg = (int)a.csr_off[r0], end = (int)a.csr_off[r0 + kZP];   // < 2^31 entries
-        for (int e = beg + t; e < end; e += NT) {
-            const uint32_t m = a.csr_meta[e];
+        for (int e = e0; e < end; e += NT) {
+            uint32_t m = m0;
+            float2 mg = mg0, V = V0;
+            if (e != e0) {
+                m = a.csr_meta[e];
+                mg = a.csr_mag[e];
+                V = a.csr_val[e];
+            }
             const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
             float2* slot = lds + pl * N + swz(z, pl);
             const float2 kv = *slot;
-            const float2 mg = a.csr_mag[e];
             const float ak = sqrtf(kv.x * kv.x + kv.y * kv.y);
             if (m & kHalfTermA) err += fabsf(ak - mg.x);
             if (m & kHalfTermB) err += fabsf(ak - mg.y);
-            if (m & kHalfEnforced) {
-                const float2 V = a.csr_val[e];
+            if (m & kHalfEnforced)
                 *slot = (m & kHalfAverage) ? make_float2(0.5f * (V.x + kv.x), 0.5f * (V.y + kv.y)) : V;
-            }
         }
         double de = (double)err;
         for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off, 64);
@@ -206,14 +269,19 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfIterAr
             a.err_part[blockIdx.x] = sum;
         }
     }
-    reg_read<N, E, R0>(rowl, v, j, p);
+    // the inverse transform takes fresh lane indices and its own twiddle
+    // pointer (the same table) so that nothing of the forward one stays live
+    const int t2 = phase_tid(), j2 = t2 % T, p2 = t2 / T;
+    float2* rowl2 = lds + p2 * N;
+    reg_read<N, E, R0>(rowl2, v, j2, p2);
     __syncthreads();
-    reg_fft<N, true>(v, rowl, j, p, tw);
+    reg_fft<N, true>(v, rowl2, j2, p2, tw2);
     const float sc = 1.0f / (float)N;
+    float2* pen2 = a.B + (int64_t)n3 * ((ky_0 + p2) + (int64_t)n2 * kx);
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
-        for (int q = 0; q < RL; ++q) pen[j + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
+        for (int q = 0; q < RL; ++q) pen2[j2 + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -240,17 +308,18 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfIterArgs a
         for (int q = 0; q < N / T; ++q) rp[swz(c + q * T, pp)] = w[q];
     }
     __syncthreads();
-    float2* rowl = lds + p * N;
+    const int t2 = phase_tid(), j2 = t2 % T, p2 = t2 / T;
+    float2* rowl = lds + p2 * N;
     float2 v[E];
-    reg_read<N, E, R0>(rowl, v, j, p);
+    reg_read<N, E, R0>(rowl, v, j2, p2);
     __syncthreads();
-    reg_fft<N, true>(v, rowl, j, p, tw);
-    float2* dst = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p));
+    reg_fft<N, true>(v, rowl, j2, p2, tw);
+    float2* dst = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p2));
     const float sc = 1.0f / (float)N;
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
-        for (int q = 0; q < RL; ++q) dst[j + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
+        for (int q = 0; q < RL; ++q) dst[j2 + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -258,20 +327,18 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfIterArgs a
 // iterate update.  Tile as P1.  The C2R split is
 //   Z[k] = (X[k] + conj X[M-k])/2 + i W_N^-k (X[k] - conj X[M-k])/2,
 //   z = IFFT_M(Z) / M,  x[2m] = Re z[m], x[2m+1] = Im z[m].
+// Leaves the unscaled z in output slots of v.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfIterArgs a, const float2* __restrict__ twM,
-                                                                     const float2* __restrict__ twN)
+__device__ __forceinline__ void load_c2r(const HalfIterArgs& a, float2* lds, float2 (&v)[RegPlan<N / 2>::E], int i2_0,
+                                         int i3, const float2* __restrict__ twM, const float2* __restrict__ twN)
 {
     constexpr int M = N / 2;
     using PL = RegPlan<M>;
-    constexpr int E = PL::E, T = M / E, R0 = PL::R0, RL = reg_last_radix<M>();
-    extern __shared__ float2 lds[];
+    constexpr int E = PL::E, T = M / E, R0 = PL::R0;
     float2* nyq = lds + kHP * M;   // X[M] of each pencil
-    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int t = phase_tid(), j = t % T, p = t / T;
     const int n2 = a.n2, h = M + 1;
-    const int tpr = n2 / kHP;
-    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
     {
         const int pp = t % kHP, c = t / kHP;
         float2* rp = lds + pp * M;
@@ -286,7 +353,6 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfIterArg
     }
     __syncthreads();
     float2* rowl = lds + p * M;
-    float2 v[E];
 #pragma unroll
     for (int i = 0; i < E / R0; ++i)
 #pragma unroll
@@ -300,26 +366,101 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfIterArg
         }
     __syncthreads();
     reg_fft<M, true>(v, rowl, j, p, twM);
+}
+
+// The iterate update of the real pair at ad from pf = real(ifftn(k')) (:165):
+// x = pf (GENFIRE) or x = x + pf - P_R(x) (DR).  Returns the new pair; stores
+// it when store_x is set.
+__device__ __forceinline__ float2 p5_update(const HalfIterArgs& a, float2 pf, float2 xv, uint16_t s2, int64_t ad,
+                                           bool store_x)
+{
+    if (a.pf_out) *reinterpret_cast<float2*>(a.pf_out + ad) = pf;
+    float2 xn = pf;
+    if (a.dr) {
+        const float ra = prh(xv.x, (uint8_t)(s2 & 0xFF), a.positivity, a.support);
+        const float rb = prh(xv.y, (uint8_t)(s2 >> 8), a.positivity, a.support);
+        xn = make_float2((xv.x + pf.x) - ra, (xv.y + pf.y) - rb);
+    }
+    if (store_x) *reinterpret_cast<float2*>(a.x + ad) = xn;
+    return xn;
+}
+
+template <int N>
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfIterArgs a, const float2* __restrict__ twM,
+                                                                     const float2* __restrict__ twN)
+{
+    constexpr int M = N / 2;
+    using PL = RegPlan<M>;
+    constexpr int E = PL::E, T = M / E;
+    extern __shared__ float2 lds[];
+    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int n2 = a.n2;
+    const int tpr = n2 / kHP;
+    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
+    float2 v[E];
+    load_c2r<N>(a, lds, v, i2_0, i3, twM, twN);
     const float sc = 1.0f / (float)M;
     const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
+    float2 xv[E];
+    uint16_t s2[E];
 #pragma unroll
-    for (int i = 0; i < E / RL; ++i)
+    for (int s = 0; s < E; ++s) {
+        const int64_t ad = row + 2 * (j + T * s);
+        xv[s] = a.dr ? *reinterpret_cast<const float2*>(a.x + ad) : make_float2(0.0f, 0.0f);
+        s2[s] = a.dr ? *reinterpret_cast<const uint16_t*>(a.S + ad) : (uint16_t)0;
+    }
 #pragma unroll
-        for (int q = 0; q < RL; ++q) {
-            const int m = j + T * i + q * (M / RL);
-            const int64_t ad = row + 2 * m;
-            const float2 pf = cscale(v[i * RL + q], sc);       // real(ifftn(k')) (:165)
-            if (a.pf_out) *reinterpret_cast<float2*>(a.pf_out + ad) = pf;
-            if (a.dr) {
-                const float2 xv = *reinterpret_cast<const float2*>(a.x + ad);
-                const uint16_t s2 = *reinterpret_cast<const uint16_t*>(a.S + ad);
-                const float ra = prh(xv.x, (uint8_t)(s2 & 0xFF), a.positivity, a.support);
-                const float rb = prh(xv.y, (uint8_t)(s2 >> 8), a.positivity, a.support);
-                *reinterpret_cast<float2*>(a.x + ad) = make_float2((xv.x + pf.x) - ra, (xv.y + pf.y) - rb);
-            } else {
-                *reinterpret_cast<float2*>(a.x + ad) = pf;
-            }
-        }
+    for (int s = 0; s < E; ++s)
+        p5_update(a, cscale(v[out_slot<M>(s)], sc), xv[s], s2[s], row + 2 * (j + T * s), true);
+}
+
+// P5 of one iteration fused with P1 of the next: the C2R output updates the
+// iterate in registers, which is transformed again without a round trip of x
+// through memory.  GENFIRE's x is the C2R output itself and is not stored
+// (pf_out carries it where the R-factor needs it); DR stores x.
+//
+// The inverse and the forward transform take their twiddles through separate
+// arguments (the same tables): with one pointer the compiler keeps the inverse
+// transform's twiddle loads live for reuse by the forward one, which doubles
+// the register count and halves occupancy.
+template <int N>
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfIterArgs a,
+                                                                           const float2* __restrict__ twM,
+                                                                           const float2* __restrict__ twN,
+                                                                           const float2* __restrict__ twM2,
+                                                                           const float2* __restrict__ twN2)
+{
+    constexpr int M = N / 2;
+    using PL = RegPlan<M>;
+    constexpr int E = PL::E, T = M / E;
+    extern __shared__ float2 lds[];
+    const int t = threadIdx.x, j = t % T, p = t / T;
+    const int n2 = a.n2;
+    const int tpr = n2 / kHP;
+    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
+    float2 v[E];
+    load_c2r<N>(a, lds, v, i2_0, i3, twM, twN);
+    const float sc = 1.0f / (float)M;
+    const int tu = phase_tid(), ju = tu % T, pu = tu / T;
+    const int64_t row = (int64_t)N * ((i2_0 + pu) + (int64_t)n2 * i3) + 2 * ju;
+    float2 xv[E];
+    uint16_t s2[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int64_t ad = row + 2 * T * s;
+        xv[s] = a.dr ? *reinterpret_cast<const float2*>(a.x + ad) : make_float2(0.0f, 0.0f);
+        s2[s] = *reinterpret_cast<const uint16_t*>(a.S + ad);
+    }
+    float2 w[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int64_t ad = row + 2 * T * s;
+        const float2 xn = p5_update(a, cscale(v[out_slot<M>(s)], sc), xv[s], s2[s], ad, a.dr != 0);
+        w[in_slot<M>(s)] = p1_value(a, xn, s2[s], ad);
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = w[s];
+    r2c_store<N>(a, lds, v, i2_0, i3, twM2, twN2);
 }
 
 // ---------------------------------------------------------------------------
@@ -477,6 +618,15 @@ template <int N> void launch_last(const HalfIterArgs& a, hipStream_t st)
     k_half_last<N><<<(a.n2 / kHP) * a.n3, half_threads<N / 2>(), lds, st>>>(a, fft_twiddles(N / 2), fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
+template <int N> void launch_last_first(const HalfIterArgs& a, hipStream_t st)
+{
+    const size_t lds = (size_t)kHP * (N / 2 + 1) * sizeof(float2);
+    set_lds(k_half_last_first<N>, lds);
+    const float2* twM = fft_twiddles(N / 2);
+    const float2* twN = fft_twiddles(N);
+    k_half_last_first<N><<<(a.n2 / kHP) * a.n3, half_threads<N / 2>(), lds, st>>>(a, twM, twN, twM, twN);
+    GF_LAUNCH_CHECK();
+}
 template <int N> void launch_y(const HalfIterArgs& a, int inverse, hipStream_t st)
 {
     const size_t lds = (size_t)kHP * N * sizeof(float2);
@@ -495,7 +645,8 @@ template <int N> void launch_z(const HalfIterArgs& a, hipStream_t st)
     // tile + one double per wave for the errK partial
     const size_t lds = (size_t)kZP * N * sizeof(float2) + sizeof(double) * (zfused_threads<N>() / 64);
     set_lds(k_half_z_fused<N>, lds);
-    k_half_z_fused<N><<<(a.n1 / 2 + 1) * (a.n2 / kZP), zfused_threads<N>(), lds, st>>>(a, fft_twiddles(N));
+    k_half_z_fused<N><<<(a.n1 / 2 + 1) * (a.n2 / kZP), zfused_threads<N>(), lds, st>>>(a, fft_twiddles(N),
+                                                                                      fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
 
@@ -549,6 +700,13 @@ void half_z(const HalfIterArgs& a, hipStream_t st)
 void half_last(const HalfIterArgs& a, hipStream_t st)
 {
 #define C(NN) launch_last<NN>(a, st)
+    GF_HALF_SWITCH_X(a.n1, C)
+#undef C
+}
+
+void half_last_first(const HalfIterArgs& a, hipStream_t st)
+{
+#define C(NN) launch_last_first<NN>(a, st)
     GF_HALF_SWITCH_X(a.n1, C)
 #undef C
 }
