@@ -107,6 +107,22 @@ def rfactor_fp64_delta(ctx, model, projs, ang, idx):
     return abs(r_gpu - r_64), r_gpu
 
 
+def grid_value_error(job, projs, idx, npts=2000):
+    """Norm-wise relative error of the job's gridded values against the oracle's
+    FP64 direct DFT (fillInFourierGrid_DFT.m:243) at the same in-plane
+    coordinates, on npts seeded points of each listed projection."""
+    from oracle import oracle as O
+    pts = job.points()
+    num = den = 0.0
+    for p in idx:
+        g = np.flatnonzero(pts["proj"] == p)
+        g = np.random.default_rng(p).choice(g, size=min(npts, len(g)), replace=False)
+        F64 = O.dft64(projs[:, :, p], pts["un"][g], pts["vn"][g])
+        num += float(np.sum(np.abs(pts["val"][g].astype(np.complex128) - F64) ** 2))
+        den += float(np.sum(np.abs(F64) ** 2))
+    return (num / den) ** 0.5
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +134,8 @@ def main():
                     help="HIP-event timing of the FFT pass kernels for the roofline line")
     ap.add_argument("--iterations", type=int, default=None,
                     help="override the workload's iteration count (short profiling runs only)")
+    ap.add_argument("--precision", choices=["fp32", "split_bf16", "fp16"], default="fp32",
+                    help="gridding GEMM precision (FFT and constraint steps stay FP32)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
@@ -157,6 +175,8 @@ def main():
     for i, v in enumerate(w.vectors):
         for j in range(3):
             getattr(p, f"vector{i + 1}")[j] = v[j]
+    prec = {"fp32": _lib.PREC_FP32, "split_bf16": _lib.PREC_SPLIT_BF16, "fp16": _lib.PREC_FP16}[args.precision]
+    p.precision = prec
     job = _lib.Job(ctx, p, w.dim, w.dim, w.num_projs)
     job.set_inputs(projs, ang, sup)          # host -> HBM, outside the timed region
 
@@ -216,7 +236,7 @@ def main():
         "data": "synthetic (seeded Gaussian-blob phantom, trilinear projections, Poisson 1e4 counts)",
         "config": {"workload": "tilt256", "volume": 256, "projections": 91, "projection_size": 256,
                    "grid": n, "iterations": w.iterations, "algorithm": w.algorithm,
-                   "rfactor_every": w.monitor_every, "gridding_precision": "fp32",
+                   "rfactor_every": w.monitor_every, "gridding_precision": args.precision,
                    "cutoff": w.cutoff, "oversampling": w.oversampling},
         "gridded_kpts_per_s": npts / (tm["grid_ms"] * 1e-3),
         "gridded_points": npts,
@@ -230,6 +250,25 @@ def main():
                      "kernel": f"k_half_y_fwd<{n}> (y-forward pass), {pass_bytes / 1e9:.3f} GB algorithmic "
                                f"per launch, {pass_ms:.3f} ms mean over {nfp} launches"},
     }
+    if prec != _lib.PREC_FP32:
+        # the same job with FP32 gridding, untimed: volume and R-factor deltas
+        p.precision = _lib.PREC_FP32
+        ref = _lib.Job(ctx, p, w.dim, w.dim, w.num_projs)
+        ref.set_inputs(projs, ang, sup)
+        ref.grid()
+        ref.reconstruct()
+        rec32 = ref.reconstruction()
+        errR32 = ref.errR()[0]
+        mk32 = ref.measuredK()
+        mk = job.measuredK()
+        out["precision_check"] = {
+            "reference": "same job with FP32 gridding on the GPU",
+            "measuredK_rel_err": float(np.linalg.norm((mk - mk32).astype(np.complex128))
+                                       / np.linalg.norm(mk32.astype(np.complex128))),
+            "volume_rel_err": float(np.linalg.norm(rec - rec32) / np.linalg.norm(rec32)),
+            "rfactor_delta": float(abs(errR[-1] - errR32[-1])) if len(errR) else None,
+        }
+        del ref
     if not args.no_cpu_baseline:
         ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log("cpu baseline (oracle sample)")
@@ -238,6 +277,7 @@ def main():
             d, r = rfactor_fp64_delta(ctx, rec, projs, ang, [0, w.num_projs // 2, w.num_projs - 1])
             out["rfactor_delta_vs_fp64"] = d
             out["rfactor_posthoc_sample"] = r
+            out["grid_value_rel_err_vs_fp64"] = grid_value_error(job, projs, [0, w.num_projs // 2, w.num_projs - 1])
         except Exception as e:  # the baseline is reported, never the target
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))

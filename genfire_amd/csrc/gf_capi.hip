@@ -165,7 +165,7 @@ struct genfire_job {
     bool have_inputs = false, have_support = false, have_init = false;
 
     // gridding
-    DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles, Pt;
+    DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles, Pt, Phi, Plo, Pscale;
     DevBuf acc, sw, hits, exact;
     DevBuf M, mask, recifft;
     std::vector<int64_t> proj_off;
@@ -318,10 +318,12 @@ void do_grid(genfire_job* j)
     grid_index_emit(j->planes.as<PlaneConst>(), P, j->hg, cutoff, n1, n2, nblk, j->blk_offsets.as<int64_t>(),
                     j->ind.as<uint32_t>(), j->dist.as<float>(), j->un.as<float>(), j->vn.as<float>(), st);
 
-    // DFT values (:236-248).  Default: the MFMA GEMM form on 64-point tiles;
-    // GENFIRE_GRID_KERNEL=valu selects the FP32 VALU kernel (256-point tiles).
+    // DFT values (:236-248).  Default: the MFMA GEMM form on 64-point tiles in
+    // the job's precision; with FP32, GENFIRE_GRID_KERNEL=valu selects the VALU
+    // kernel (256-point tiles).
+    const int prec = j->prm.precision;
     const char* gk = std::getenv("GENFIRE_GRID_KERNEL");
-    const bool use_valu = gk && std::string(gk) == "valu";
+    const bool use_valu = prec == GENFIRE_PREC_FP32 && gk && std::string(gk) == "valu";
     const int tile_pts = use_valu ? 256 : mfma_tile_points();
     std::vector<PointTile> tl;
     for (int p = 0; p < P; ++p)
@@ -330,13 +332,24 @@ void do_grid(genfire_job* j)
     if (!tl.empty()) {
         j->tiles.alloc(sizeof(PointTile) * tl.size());
         GF_HIP(hipMemcpyAsync(j->tiles.p, tl.data(), sizeof(PointTile) * tl.size(), hipMemcpyHostToDevice, st));
-        if (!use_valu) {
+        if (prec != GENFIRE_PREC_FP32) {
+            const size_t ne = grid_prep_lp_elems(j->dim1, j->dim2, P);
+            j->Phi.alloc(sizeof(uint16_t) * ne);
+            if (prec == GENFIRE_PREC_SPLIT_BF16) j->Plo.alloc(sizeof(uint16_t) * ne);
+            j->Pscale.alloc(sizeof(float) * P);
+            grid_prep_lp(prec, j->proj.as<float>(), j->dim1, j->dim2, P, j->Phi.as<uint16_t>(),
+                         j->Plo.as<uint16_t>(), j->Pscale.as<float>(), st);
+        } else if (!use_valu) {
             j->Pt.alloc(sizeof(float) * grid_prep_floats(j->dim1, j->dim2, P));
             grid_prep_proj(j->proj.as<float>(), j->dim1, j->dim2, P, j->Pt.as<float>(), st);
         }
         std::pair<hipEvent_t, hipEvent_t> ev;
         kstat_begin(j, GENFIRE_KSTAT_GRID_VALUES, &ev);
-        if (use_valu)
+        if (prec != GENFIRE_PREC_FP32)
+            grid_values_mfma_lp(prec, j->Phi.as<uint16_t>(), j->Plo.as<uint16_t>(), j->Pscale.as<float>(), j->dim1,
+                                j->dim2, j->un.as<float>(), j->vn.as<float>(), j->tiles.as<PointTile>(),
+                                (int)tl.size(), j->F.as<float2>(), st);
+        else if (use_valu)
             grid_values_fp32(j->proj.as<float>(), j->dim1, j->dim2, j->un.as<float>(), j->vn.as<float>(),
                              j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
         else
@@ -379,10 +392,13 @@ void do_grid(genfire_job* j)
     float ms = 0.0f;
     GF_HIP(hipEventElapsedTime(&ms, tm.a, tm.b));
     j->t_grid = ms;
-    // the FP64 accumulators are only needed for the values above
+    // the FP64 accumulators and the GEMM operands are only needed for the values above
     j->acc.release();
     j->sw.release();
     j->exact.release();
+    j->Pt.release();
+    j->Phi.release();
+    j->Plo.release();
     j->gridded = true;
     j->reconstructed = false;
 }
@@ -709,8 +725,9 @@ int genfire_job_create(genfire_ctx* ctx, const genfire_params* p, int32_t dim1, 
         GF_CHECK(p->oversamplingRatio > 0, GENFIRE_ERR_ARG, "GENFIRE: oversamplingRatio must be positive");
         GF_CHECK(p->algorithm == GENFIRE_ALG_GENFIRE || p->algorithm == GENFIRE_ALG_DR, GENFIRE_ERR_ARG,
                  "GENFIRE: unknown algorithm");
-        GF_CHECK(p->precision == GENFIRE_PREC_FP32, GENFIRE_ERR_UNSUPPORTED,
-                 "GENFIRE: only the FP32 gridding precision is built");
+        GF_CHECK(p->precision == GENFIRE_PREC_FP32 || p->precision == GENFIRE_PREC_SPLIT_BF16 ||
+                     p->precision == GENFIRE_PREC_FP16,
+                 GENFIRE_ERR_ARG, "GENFIRE: unknown gridding precision");
         use_device(ctx);
         auto* j = new genfire_job();
         std::unique_ptr<genfire_job> guard(j);

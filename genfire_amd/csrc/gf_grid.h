@@ -33,6 +33,16 @@ size_t grid_prep_floats(int dim1, int dim2, int P);
 void grid_prep_proj(const float* d_proj, int dim1, int dim2, int P, float* d_Pt, hipStream_t st);
 void grid_values_mfma_fp32(const float* d_Pt, int dim1, int dim2, const float* d_un, const float* d_vn,
                            const PointTile* d_tiles, int ntiles, float2* d_F, hipStream_t st);
+// Reduced-precision forms (GENFIRE_PREC_SPLIT_BF16, GENFIRE_PREC_FP16) on the same
+// 64-point tiles: projections split once per job into 16-bit operand arrays
+// (hi, and lo for split-BF16) of grid_prep_lp_elems elements each, plus P
+// per-projection scales (FP16: max|P|).
+size_t grid_prep_lp_elems(int dim1, int dim2, int P);
+void grid_prep_lp(int mode, const float* d_proj, int dim1, int dim2, int P, unsigned short* d_hi,
+                  unsigned short* d_lo, float* d_scale, hipStream_t st);
+void grid_values_mfma_lp(int mode, const unsigned short* d_hi, const unsigned short* d_lo, const float* d_scale,
+                         int dim1, int dim2, const float* d_un, const float* d_vn, const PointTile* d_tiles,
+                         int ntiles, float2* d_F, hipStream_t st);
 // Weighted-average accumulation of one projection's points (:267-291), in order.
 void grid_scatter(const uint32_t* d_ind, const float* d_dist, const float2* d_F, int64_t begin,
                   int64_t count, double2* d_acc, double* d_sw, uint32_t* d_hits, float2* d_exact,

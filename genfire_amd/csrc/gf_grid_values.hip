@@ -440,4 +440,270 @@ void grid_values_mfma_fp32(const float* d_Pt, int dim1, int dim2, const float* d
     GF_LAUNCH_CHECK();
 }
 
+// ===========================================================================
+// Reduced-precision MFMA forms of the same GEMM (v_mfma_f32_16x16x32_{bf16,f16},
+// FP32 accumulation).  Lane l supplies A[l%16][8(l/16)+q] = E1(point l%16,
+// a0+q) for q = 0..7 and B[8(l/16)+q][l%16] = P[a0+q][b]: eight consecutive
+// projection rows, which the split kernel stores row-contiguous per column
+// ([p][b][a], a padded to a multiple of 64) so a lane reads them as one
+// 16-byte LDS access.
+//   kSplitBf16: x = hi + lo with hi = x rounded to bf16 and lo = the exact
+//     residual rounded to bf16 (|x - hi - lo| <= 2^-17 |x|);
+//     A.B = Ah.Bh + Ah.Bl + Al.Bh (the dropped Al.Bl is below 2^-16 |A||B|).
+//   kFp16: P / s_p in FP16 with s_p = max|P| of projection p, E1 in FP16,
+//     one product; the sum is multiplied by s_p.
+// E1 = E1hi(a >> 4) * E1lo(a & 15): a lane's eight rows share one E1hi entry
+// and always use the same eight E1lo entries, which it keeps in registers.
+// ===========================================================================
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef unsigned short v8u __attribute__((ext_vector_type(8)));
+
+constexpr int kLpKC = 64;            // projection rows per LDS stage (two k-steps of 32)
+constexpr int kLpSK = kLpKC + 8;     // LDS column stride in 16-bit elements (144 bytes)
+
+// hi = x rounded to nearest bf16, lo = the exact residual x - hi rounded to
+// nearest bf16: |lo| <= 2^-8 |x| and |x - hi - lo| <= 2^-17 |x|
+__device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned short& lo)
+{
+    const __bf16 h = (__bf16)x;
+    hi = __builtin_bit_cast(unsigned short, h);
+    lo = __builtin_bit_cast(unsigned short, (__bf16)(x - (float)h));
+}
+
+__global__ void k_proj_maxabs(const float* __restrict__ proj, int64_t n, float* __restrict__ scale)
+{
+    const float* src = proj + n * blockIdx.x;
+    float m = 0.0f;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, fabsf(src[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+    __shared__ float red[16];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, red[w]);
+        scale[blockIdx.x] = m > 0.0f ? m : 1.0f;
+    }
+}
+
+template <int MODE>
+__global__ void k_split_proj(const float* __restrict__ proj, int dim1, int dim2, int P, int Kp, int Np,
+                             const float* __restrict__ scale, unsigned short* __restrict__ hi,
+                             unsigned short* __restrict__ lo)
+{
+    const int64_t total = (int64_t)P * Np * Kp;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(t % Kp), b = (int)((t / Kp) % Np), p = (int)(t / ((int64_t)Kp * Np));
+        const float x = (a < dim1 && b < dim2) ? proj[a + (int64_t)dim1 * (b + (int64_t)dim2 * p)] : 0.0f;
+        if (MODE == GENFIRE_PREC_SPLIT_BF16) {
+            unsigned short h, l;
+            split_bf16(x, h, l);
+            hi[t] = h;
+            lo[t] = l;
+        } else {
+            hi[t] = __builtin_bit_cast(unsigned short, (_Float16)(x / scale[p]));
+        }
+    }
+}
+
+template <int MODE, int NCH>
+__global__ __launch_bounds__(256) void k_dft_mfma_lp(const unsigned short* __restrict__ Phi,
+                                                     const unsigned short* __restrict__ Plo,
+                                                     const float* __restrict__ scale, int Kp, int Np, int dim1,
+                                                     int dim2, const float* __restrict__ un,
+                                                     const float* __restrict__ vn, const PointTile* __restrict__ tiles,
+                                                     float2* __restrict__ F)
+{
+    constexpr bool SPLIT = MODE == GENFIRE_PREC_SPLIT_BF16;
+    constexpr int NTT = NCH / 16;
+    constexpr int NB = SPLIT ? 2 : 1;       // staged operand arrays (hi, lo)
+    extern __shared__ float4 smem4[];
+    unsigned short* Bl = reinterpret_cast<unsigned short*>(smem4);   // [NB][NCH][kLpSK]
+    const int H1 = Kp / 16, H2 = Np / 16;
+    float2* t1hi = reinterpret_cast<float2*>(Bl + NB * NCH * kLpSK);  // [H1][64]
+    float2* t1lo = t1hi + H1 * kMP;                                   // [16][64]
+    float2* t2hi = t1lo + 16 * kMP;                                   // [H2][64]
+    float2* t2lo = t2hi + H2 * kMP;                                   // [16][64]
+    const PointTile tl = tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int c1 = dim1 / 2, c2 = dim2 / 2;
+    for (int idx = tid; idx < kMP * (H1 + H2 + 32); idx += 256) {
+        const int pt = idx % kMP, row = idx / kMP;
+        const bool ok = pt < tl.count;
+        const float t1 = ok ? un[tl.begin + pt] : 0.0f, t2 = ok ? vn[tl.begin + pt] : 0.0f;
+        if (row < H1) t1hi[row * kMP + pt] = cis_turns((double)(16 * row - c1), t1);
+        else if (row < H1 + 16) t1lo[(row - H1) * kMP + pt] = cis_turns((double)(row - H1), t1);
+        else if (row < H1 + 16 + H2) t2hi[(row - H1 - 16) * kMP + pt] = cis_turns((double)(16 * (row - H1 - 16) - c2), t2);
+        else t2lo[(row - H1 - 16 - H2) * kMP + pt] = cis_turns((double)(row - H1 - 16 - H2), t2);
+    }
+    __syncthreads();
+    const int w = tid / 64, l = tid % 64, li = l % 16, lk = l / 16;
+    const int pa = 16 * w + li;
+    float2 elo[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) elo[q] = t1lo[(8 * (lk & 1) + q) * kMP + pa];
+    const int64_t pbase = (int64_t)tl.proj * Np * Kp;
+    float Fr[4] = {0.0f, 0.0f, 0.0f, 0.0f}, Fi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int nc = 0; nc < Np; nc += NCH) {
+        v4f acr[NTT], aci[NTT];
+#pragma unroll
+        for (int nt = 0; nt < NTT; ++nt) { acr[nt] = v4f{0, 0, 0, 0}; aci[nt] = v4f{0, 0, 0, 0}; }
+        for (int kc = 0; kc < Kp; kc += kLpKC) {
+            __syncthreads();
+            // stage columns nc..nc+NCH-1, rows kc..kc+63 (16-byte units)
+            for (int idx = tid; idx < NB * NCH * (kLpKC / 8); idx += 256) {
+                const int u = idx % (kLpKC / 8), c = (idx / (kLpKC / 8)) % NCH, h = idx / (NCH * (kLpKC / 8));
+                const unsigned short* src = h ? Plo : Phi;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (nc + c < Np) v = *reinterpret_cast<const uint4*>(src + pbase + (int64_t)(nc + c) * Kp + kc + 8 * u);
+                *reinterpret_cast<uint4*>(Bl + (h * NCH + c) * kLpSK + 8 * u) = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < kLpKC / 32; ++ks) {
+                const int a0 = kc + 32 * ks + 8 * lk;          // this lane's rows a0..a0+7
+                const float2 ehi = t1hi[(a0 >> 4) * kMP + pa];
+                v8u are, aim, are2, aim2;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float2 e = cmul(ehi, elo[q]);
+                    if (SPLIT) {
+                        unsigned short h, lo;
+                        split_bf16(e.x, h, lo);
+                        are[q] = h;
+                        are2[q] = lo;
+                        split_bf16(e.y, h, lo);
+                        aim[q] = h;
+                        aim2[q] = lo;
+                    } else {
+                        are[q] = __builtin_bit_cast(unsigned short, (_Float16)e.x);
+                        aim[q] = __builtin_bit_cast(unsigned short, (_Float16)e.y);
+                    }
+                }
+#pragma unroll
+                for (int nt = 0; nt < NTT; ++nt) {
+                    const int col = 16 * nt + li, koff = 32 * ks + 8 * lk;
+                    const v8u bh = *reinterpret_cast<const v8u*>(Bl + col * kLpSK + koff);
+                    if (SPLIT) {
+                        const v8u bl = *reinterpret_cast<const v8u*>(Bl + (NCH + col) * kLpSK + koff);
+                        const v8bf Ah = __builtin_bit_cast(v8bf, are), Al = __builtin_bit_cast(v8bf, are2);
+                        const v8bf Ih = __builtin_bit_cast(v8bf, aim), Il = __builtin_bit_cast(v8bf, aim2);
+                        const v8bf Bh = __builtin_bit_cast(v8bf, bh), Bo = __builtin_bit_cast(v8bf, bl);
+                        acr[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, acr[nt], 0, 0, 0);
+                        acr[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bo, acr[nt], 0, 0, 0);
+                        acr[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh, acr[nt], 0, 0, 0);
+                        aci[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Il, Bh, aci[nt], 0, 0, 0);
+                        aci[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ih, Bo, aci[nt], 0, 0, 0);
+                        aci[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ih, Bh, aci[nt], 0, 0, 0);
+                    } else {
+                        const v8h B = __builtin_bit_cast(v8h, bh);
+                        acr[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, are), B, acr[nt], 0, 0, 0);
+                        aci[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, aim), B, aci[nt], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        // F += sum_b E2(pt, b) G(pt, b), as in the FP32 form
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int pt = 16 * w + 4 * lk + r;
+            float hr = 0.0f, hi = 0.0f;
+#pragma unroll
+            for (int nt = 0; nt < NTT; ++nt) {
+                const int m = nc / 16 + nt;
+                if (m < H2) {
+                    const float2 e = t2hi[m * kMP + pt];
+                    const float gr = acr[nt][r], gi = aci[nt][r];
+                    hr += gr * e.x - gi * e.y;
+                    hi += gr * e.y + gi * e.x;
+                }
+            }
+            const float2 el = t2lo[li * kMP + pt];
+            float fr = hr * el.x - hi * el.y, fi = hr * el.y + hi * el.x;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                fr += __shfl_xor(fr, off, 64);
+                fi += __shfl_xor(fi, off, 64);
+            }
+            Fr[r] += fr;
+            Fi[r] += fi;
+        }
+    }
+    const float sc = SPLIT ? 1.0f : scale[tl.proj];
+    if (li == 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int pt = 16 * w + 4 * lk + r;
+            if (pt < tl.count) F[tl.begin + pt] = make_float2(Fr[r] * sc, Fi[r] * sc);
+        }
+}
+
+int lp_rows(int dim1) { return (dim1 + kLpKC - 1) / kLpKC * kLpKC; }
+int lp_cols(int dim2) { return (dim2 + 15) / 16 * 16; }
+
+template <int MODE, int NCH> size_t lp_lds(int Kp, int Np)
+{
+    constexpr int NB = MODE == GENFIRE_PREC_SPLIT_BF16 ? 2 : 1;
+    return sizeof(unsigned short) * NB * NCH * kLpSK + sizeof(float2) * kMP * (Kp / 16 + Np / 16 + 32);
+}
+
+template <int MODE, int NCH>
+void launch_lp(const unsigned short* hi, const unsigned short* lo, const float* scale, int dim1, int dim2,
+               const float* un, const float* vn, const PointTile* tiles, int ntiles, float2* F, hipStream_t st)
+{
+    const int Kp = lp_rows(dim1), Np = lp_cols(dim2);
+    const size_t b = lp_lds<MODE, NCH>(Kp, Np);
+    GF_CHECK(b <= 160 * 1024, GENFIRE_ERR_UNSUPPORTED, "projection too large for the MFMA DFT kernel");
+    if (b > 65536)
+        GF_HIP(hipFuncSetAttribute((const void*)k_dft_mfma_lp<MODE, NCH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)b));
+    k_dft_mfma_lp<MODE, NCH><<<ntiles, 256, b, st>>>(hi, lo, scale, Kp, Np, dim1, dim2, un, vn, tiles, F);
+    GF_LAUNCH_CHECK();
+}
+
+template <int MODE>
+void launch_lp_n(const unsigned short* hi, const unsigned short* lo, const float* scale, int dim1, int dim2,
+                 const float* un, const float* vn, const PointTile* tiles, int ntiles, float2* F, hipStream_t st)
+{
+    const int Np = lp_cols(dim2);
+    if (Np >= 128) launch_lp<MODE, 128>(hi, lo, scale, dim1, dim2, un, vn, tiles, ntiles, F, st);
+    else if (Np >= 64) launch_lp<MODE, 64>(hi, lo, scale, dim1, dim2, un, vn, tiles, ntiles, F, st);
+    else launch_lp<MODE, 16>(hi, lo, scale, dim1, dim2, un, vn, tiles, ntiles, F, st);
+}
+
+}  // namespace
+
+size_t grid_prep_lp_elems(int dim1, int dim2, int P) { return (size_t)P * lp_rows(dim1) * lp_cols(dim2); }
+
+void grid_prep_lp(int mode, const float* d_proj, int dim1, int dim2, int P, unsigned short* d_hi,
+                  unsigned short* d_lo, float* d_scale, hipStream_t st)
+{
+    const int Kp = lp_rows(dim1), Np = lp_cols(dim2);
+    const int64_t total = (int64_t)P * Kp * Np;
+    if (mode == GENFIRE_PREC_SPLIT_BF16) {
+        k_split_proj<GENFIRE_PREC_SPLIT_BF16><<<grid_blocks(total, 256), 256, 0, st>>>(d_proj, dim1, dim2, P, Kp, Np,
+                                                                                     d_scale, d_hi, d_lo);
+    } else {
+        GF_CHECK(mode == GENFIRE_PREC_FP16, GENFIRE_ERR_ARG, "unknown gridding precision");
+        k_proj_maxabs<<<P, 1024, 0, st>>>(d_proj, (int64_t)dim1 * dim2, d_scale);
+        GF_LAUNCH_CHECK();
+        k_split_proj<GENFIRE_PREC_FP16><<<grid_blocks(total, 256), 256, 0, st>>>(d_proj, dim1, dim2, P, Kp, Np,
+                                                                               d_scale, d_hi, d_lo);
+    }
+    GF_LAUNCH_CHECK();
+}
+
+void grid_values_mfma_lp(int mode, const unsigned short* d_hi, const unsigned short* d_lo, const float* d_scale,
+                         int dim1, int dim2, const float* d_un, const float* d_vn, const PointTile* d_tiles,
+                         int ntiles, float2* d_F, hipStream_t st)
+{
+    if (ntiles <= 0) return;
+    if (mode == GENFIRE_PREC_SPLIT_BF16)
+        launch_lp_n<GENFIRE_PREC_SPLIT_BF16>(d_hi, d_lo, d_scale, dim1, dim2, d_un, d_vn, d_tiles, ntiles, d_F, st);
+    else
+        launch_lp_n<GENFIRE_PREC_FP16>(d_hi, d_lo, d_scale, dim1, dim2, d_un, d_vn, d_tiles, ntiles, d_F, st);
+}
+
 }  // namespace gf

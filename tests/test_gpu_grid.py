@@ -117,3 +117,51 @@ def test_recIFFT(ctx, O):
     got = job.recIFFT()
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert err < 1e-5, err
+
+
+# Reduced-precision gridding GEMMs (split-BF16 hi+lo, FP16 with per-projection
+# max-abs scaling), FP32 accumulation.  The index pass is shared, so indices and
+# hits stay bit-exact; the values are held to the stated bounds against the FP64
+# DFT (norm-wise relative error over every point of each projection checked).
+LP_TOL = {_lib.PREC_SPLIT_BF16: 1.5e-5, _lib.PREC_FP16: 1.5e-3}
+LP_CASES = {
+    # name: (dim1, dim2, angles, points checked per projection or None for all)
+    "tilt64": (64, 64, np.stack([np.zeros(5), np.linspace(-60, 60, 5), np.zeros(5)], 1), None),
+    "euler_48x24": (48, 24, np.random.default_rng(21).uniform([-10, -70, -10], [10, 70, 10], (4, 3)), None),
+    "euler_256": (256, 256, np.random.default_rng(22).uniform([-5, -70, -5], [5, 70, 5], (2, 3)), 3000),
+}
+
+
+@pytest.mark.parametrize("prec", [_lib.PREC_SPLIT_BF16, _lib.PREC_FP16])
+@pytest.mark.parametrize("name", list(LP_CASES))
+def test_reduced_precision_values(ctx, O, name, prec):
+    print(f"[grid] precision {prec} {name}", flush=True)
+    d1, d2, ang, nchk = LP_CASES[name]
+    P = len(ang)
+    rng = np.random.default_rng(12)
+    projs = rng.uniform(0.0, 1.0, (d1, d2, P)).astype(np.float32)
+    jobs = {}
+    for pr in (_lib.PREC_FP32, prec):
+        p = _params(ratio=2.0, cutoff=0.3, v3=(0, 0, 1))
+        p.precision = pr
+        jobs[pr] = _lib.Job(ctx, p, d1, d2, P)
+        jobs[pr].set_inputs(projs, ang)
+        jobs[pr].grid()
+    a, b = jobs[_lib.PREC_FP32], jobs[prec]
+    np.testing.assert_array_equal(a.hits(), b.hits())
+    pa, pb = a.points(), b.points()
+    np.testing.assert_array_equal(pa["ind"], pb["ind"])
+    worst = 0.0
+    for p in range(P):
+        g = np.flatnonzero(pb["proj"] == p)
+        if nchk is not None:
+            g = np.random.default_rng(p).choice(g, size=min(nchk, len(g)), replace=False)
+        F64 = O.dft64(projs[:, :, p], pb["un"][g], pb["vn"][g])
+        err = np.linalg.norm(pb["val"][g].astype(np.complex128) - F64) / np.linalg.norm(F64)
+        worst = max(worst, err)
+    print(f"[grid] precision {prec} {name}: value error {worst:.2e}", flush=True)
+    assert worst < LP_TOL[prec], worst
+    mka, mkb = a.measuredK(), b.measuredK()
+    np.testing.assert_array_equal(mka != 0, mkb != 0)
+    err = np.linalg.norm((mkb - mka).astype(np.complex128)) / np.linalg.norm(mka.astype(np.complex128))
+    assert err < LP_TOL[prec], err
