@@ -35,7 +35,11 @@ EXPORTS = [
     "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_submit", "genfire_job_poll",
     "genfire_job_wait", "genfire_job_timing", "genfire_job_set_profiling",
     "genfire_job_kernel_stats", "genfire_project", "genfire_rfactor", "genfire_fftn",
+    "genfire_server_config_default", "genfire_server_create", "genfire_server_destroy",
+    "genfire_server_submit", "genfire_server_status", "genfire_server_wait", "genfire_server_drain",
+    "genfire_server_result", "genfire_server_release", "genfire_server_get_stats",
 ]
+REQ_QUEUED, REQ_PREPARING, REQ_ITERATING, REQ_FINISHING, REQ_DONE, REQ_FAILED = range(6)
 
 
 class GenfireParams(ctypes.Structure):
@@ -55,6 +59,36 @@ class GenfireParams(ctypes.Structure):
         ("algorithm", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("compute_recIFFT", ctypes.c_int32),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_int64),
+        ("max_active", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32),
+        ("lookahead", ctypes.c_int32),
+        ("prep_threads", ctypes.c_int32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_int64),
+        ("completed", ctypes.c_int64),
+        ("failed", ctypes.c_int64),
+        ("queued", ctypes.c_int64),
+        ("active", ctypes.c_int64),
+        ("batched_launches", ctypes.c_int64),
+        ("batched_jobs", ctypes.c_int64),
+        ("peak_footprint_bytes", ctypes.c_int64),
+        ("hbm_budget_bytes", ctypes.c_int64),
+        ("max_active_seen", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("jobs_per_hour", ctypes.c_double),
     ]
 
 
@@ -113,6 +147,16 @@ def load():
         "genfire_project": ([vp, pf, i32, i32, i32, f64, f64, f64, pf], ctypes.c_int),
         "genfire_rfactor": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pf], ctypes.c_int),
         "genfire_fftn": ([vp, pf, i32, i32, i32, i32], ctypes.c_int),
+        "genfire_server_config_default": ([P(ServerConfig)], ctypes.c_int),
+        "genfire_server_create": ([vp, P(ServerConfig), P(vp)], ctypes.c_int),
+        "genfire_server_destroy": ([vp], ctypes.c_int),
+        "genfire_server_submit": ([vp, P(GenfireParams), i32, i32, i32, pf, pd, pu8, pi64], ctypes.c_int),
+        "genfire_server_status": ([vp, i64, pi32], ctypes.c_int),
+        "genfire_server_wait": ([vp, i64], ctypes.c_int),
+        "genfire_server_drain": ([vp], ctypes.c_int),
+        "genfire_server_result": ([vp, i64, pf, pf, pf, pd], ctypes.c_int),
+        "genfire_server_release": ([vp, i64], ctypes.c_int),
+        "genfire_server_get_stats": ([vp, P(ServerStats)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -329,3 +373,81 @@ class Job:
         self._c(self.L.genfire_job_kernel_stats(self.h, int(k), ctypes.byref(n), ctypes.byref(ms)),
                 "genfire_job_kernel_stats")
         return n.value, ms.value
+
+
+def default_server_config() -> ServerConfig:
+    c = ServerConfig()
+    load().genfire_server_config_default(ctypes.byref(c))
+    return c
+
+
+class Server:
+    """Request server (genfire_server): continuous batching of reconstruction jobs."""
+
+    def __init__(self, ctx: Context, config: ServerConfig | None = None):
+        self.ctx = ctx
+        self.L = ctx.L
+        h = ctypes.c_void_p()
+        ctx.check(self.L.genfire_server_create(ctx.h, ctypes.byref(config) if config is not None else None,
+                                               ctypes.byref(h)), "genfire_server_create")
+        self.h = h
+        self._shape = {}
+
+    def close(self):
+        if self.h:
+            self.L.genfire_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, params: GenfireParams, projections, angles, support=None) -> int:
+        d1, d2, P = projections.shape
+        pj = np.ascontiguousarray(np.asarray(projections, np.float32).ravel(order="F"))
+        ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(P, 3).ravel(order="F"))
+        sup = None if support is None else np.ascontiguousarray(
+            (np.asarray(support) != 0).astype(np.uint8).ravel(order="F"))
+        rid = ctypes.c_int64()
+        self.ctx.check(self.L.genfire_server_submit(self.h, ctypes.byref(params), d1, d2, P,
+                                                    _ptr(pj, ctypes.c_float), _ptr(ang, ctypes.c_double),
+                                                    _ptr(sup, ctypes.c_uint8), ctypes.byref(rid)),
+                       "genfire_server_submit")
+        nR = params.numIterations // max(1, params.monitorR_loopLength) if params.monitor_R else 0
+        self._shape[rid.value] = (d1, d2, params.numIterations, nR)
+        return rid.value
+
+    def status(self, rid: int) -> int:
+        st = ctypes.c_int32()
+        self.ctx.check(self.L.genfire_server_status(self.h, int(rid), ctypes.byref(st)), "genfire_server_status")
+        return st.value
+
+    def wait(self, rid: int):
+        self.ctx.check(self.L.genfire_server_wait(self.h, int(rid)), "genfire_server_wait")
+
+    def drain(self):
+        self.ctx.check(self.L.genfire_server_drain(self.h), "genfire_server_drain")
+
+    def result(self, rid: int):
+        """(reconstruction, errK, errR, latency_ms {queued, running, total})"""
+        d1, d2, N, nR = self._shape[rid]
+        rec = np.zeros(d1 * d2 * d1, np.float32)
+        ek = np.zeros(N, np.float32)
+        er = np.zeros(nR, np.float32)
+        lat = np.zeros(3)
+        self.ctx.check(self.L.genfire_server_result(self.h, int(rid), _ptr(rec, ctypes.c_float),
+                                                    _ptr(ek, ctypes.c_float), _ptr(er, ctypes.c_float),
+                                                    _ptr(lat, ctypes.c_double)), "genfire_server_result")
+        return (rec.reshape(d1, d2, d1, order="F"), ek, er,
+                dict(queued=lat[0], running=lat[1], total=lat[2]))
+
+    def release(self, rid: int):
+        self.ctx.check(self.L.genfire_server_release(self.h, int(rid)), "genfire_server_release")
+        self._shape.pop(rid, None)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self.ctx.check(self.L.genfire_server_get_stats(self.h, ctypes.byref(st)), "genfire_server_get_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_ if k != "pad_"}

@@ -140,6 +140,18 @@ constexpr uint32_t kHalfAverage = 1u << 30;   // only one of the pair measured: 
 constexpr uint32_t kHalfEnforced = 1u << 31;  // replaced at the current schedule stage
 bool half_supported(int n1, int n2, int n3);
 int64_t half_partials(int n1, int n2, int n3);
+// Up to kHalfBatchMax jobs of one grid shape in one launch (launch row
+// blockIdx.y = job): the same-shape steps of a request server's jobs.
+constexpr int kHalfBatchMax = 8;
+struct HalfBatch {
+    HalfIterArgs job[kHalfBatchMax];
+    int count;
+};
+void half_first(const HalfBatch& b, hipStream_t st);
+void half_y(const HalfBatch& b, int inverse, hipStream_t st);
+void half_z(const HalfBatch& b, hipStream_t st);
+void half_last(const HalfBatch& b, hipStream_t st);
+void half_last_first(const HalfBatch& b, hipStream_t st);
 void half_first(const HalfIterArgs& a, hipStream_t st);
 void half_y(const HalfIterArgs& a, int inverse, hipStream_t st);
 void half_z(const HalfIterArgs& a, hipStream_t st);

@@ -128,9 +128,10 @@ __device__ __forceinline__ float2 p1_value(const HalfIterArgs& a, float2 xv, uin
 }
 
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfIterArgs a, const float2* __restrict__ twM,
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch hb, const float2* __restrict__ twM,
                                                                       const float2* __restrict__ twN)
 {
+    const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     constexpr int M = N / 2;
     using PL = RegPlan<M>;
     constexpr int E = PL::E, T = M / E;
@@ -161,8 +162,9 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfIterAr
 // P2: y forward, A -> B.  Tile: one kx, 16 consecutive z.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfIterArgs a, const float2* __restrict__ tw)
+__global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, const float2* __restrict__ tw)
 {
+    const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     using PL = RegPlan<N>;
     constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
     extern __shared__ float2 lds[];
@@ -199,9 +201,10 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfIterArgs a
 // of registers, so smaller workgroups keep three of them resident per CU).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfIterArgs a, const float2* __restrict__ tw,
+__global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch hb, const float2* __restrict__ tw,
                                                                      const float2* __restrict__ tw2)
 {
+    const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     using PL = RegPlan<N>;
     constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
     constexpr int NT = zfused_threads<N>();
@@ -288,8 +291,9 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfIterAr
 // P4: y inverse, B -> A.  Tile: one kx, 16 consecutive z (strided read).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfIterArgs a, const float2* __restrict__ tw)
+__global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, const float2* __restrict__ tw)
 {
+    const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     using PL = RegPlan<N>;
     constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
     extern __shared__ float2 lds[];
@@ -386,9 +390,10 @@ __device__ __forceinline__ float2 p5_update(const HalfIterArgs& a, float2 pf, fl
 }
 
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfIterArgs a, const float2* __restrict__ twM,
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch hb, const float2* __restrict__ twM,
                                                                      const float2* __restrict__ twN)
 {
+    const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     constexpr int M = N / 2;
     using PL = RegPlan<M>;
     constexpr int E = PL::E, T = M / E;
@@ -424,12 +429,13 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfIterArg
 // transform's twiddle loads live for reuse by the forward one, which doubles
 // the register count and halves occupancy.
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfIterArgs a,
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfBatch hb,
                                                                            const float2* __restrict__ twM,
                                                                            const float2* __restrict__ twN,
                                                                            const float2* __restrict__ twM2,
                                                                            const float2* __restrict__ twN2)
 {
+    const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     constexpr int M = N / 2;
     using PL = RegPlan<M>;
     constexpr int E = PL::E, T = M / E;
@@ -604,49 +610,57 @@ template <class K> void set_lds(K kern, size_t bytes)
         GF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-template <int N> void launch_first(const HalfIterArgs& a, hipStream_t st)
+template <int N> void launch_first(const HalfBatch& b, hipStream_t st)
 {
+    const HalfIterArgs& a = b.job[0];
     const size_t lds = (size_t)kHP * (N / 2) * sizeof(float2);
     set_lds(k_half_first<N>, lds);
-    k_half_first<N><<<(a.n2 / kHP) * a.n3, half_threads<N / 2>(), lds, st>>>(a, fft_twiddles(N / 2), fft_twiddles(N));
+    k_half_first<N><<<dim3((a.n2 / kHP) * a.n3, b.count), half_threads<N / 2>(), lds, st>>>(b, fft_twiddles(N / 2),
+                                                                                          fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
-template <int N> void launch_last(const HalfIterArgs& a, hipStream_t st)
+template <int N> void launch_last(const HalfBatch& b, hipStream_t st)
 {
+    const HalfIterArgs& a = b.job[0];
     const size_t lds = (size_t)kHP * (N / 2 + 1) * sizeof(float2);
     set_lds(k_half_last<N>, lds);
-    k_half_last<N><<<(a.n2 / kHP) * a.n3, half_threads<N / 2>(), lds, st>>>(a, fft_twiddles(N / 2), fft_twiddles(N));
+    k_half_last<N><<<dim3((a.n2 / kHP) * a.n3, b.count), half_threads<N / 2>(), lds, st>>>(b, fft_twiddles(N / 2),
+                                                                                         fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
-template <int N> void launch_last_first(const HalfIterArgs& a, hipStream_t st)
+template <int N> void launch_last_first(const HalfBatch& b, hipStream_t st)
 {
+    const HalfIterArgs& a = b.job[0];
     const size_t lds = (size_t)kHP * (N / 2 + 1) * sizeof(float2);
     set_lds(k_half_last_first<N>, lds);
     const float2* twM = fft_twiddles(N / 2);
     const float2* twN = fft_twiddles(N);
-    k_half_last_first<N><<<(a.n2 / kHP) * a.n3, half_threads<N / 2>(), lds, st>>>(a, twM, twN, twM, twN);
+    k_half_last_first<N><<<dim3((a.n2 / kHP) * a.n3, b.count), half_threads<N / 2>(), lds, st>>>(b, twM, twN, twM,
+                                                                                               twN);
     GF_LAUNCH_CHECK();
 }
-template <int N> void launch_y(const HalfIterArgs& a, int inverse, hipStream_t st)
+template <int N> void launch_y(const HalfBatch& b, int inverse, hipStream_t st)
 {
+    const HalfIterArgs& a = b.job[0];
     const size_t lds = (size_t)kHP * N * sizeof(float2);
-    const int blocks = (a.n1 / 2 + 1) * (a.n3 / kHP);
+    const dim3 blocks((a.n1 / 2 + 1) * (a.n3 / kHP), b.count);
     if (inverse) {
         set_lds(k_half_y_inv<N>, lds);
-        k_half_y_inv<N><<<blocks, half_threads<N>(), lds, st>>>(a, fft_twiddles(N));
+        k_half_y_inv<N><<<blocks, half_threads<N>(), lds, st>>>(b, fft_twiddles(N));
     } else {
         set_lds(k_half_y_fwd<N>, lds);
-        k_half_y_fwd<N><<<blocks, half_threads<N>(), lds, st>>>(a, fft_twiddles(N));
+        k_half_y_fwd<N><<<blocks, half_threads<N>(), lds, st>>>(b, fft_twiddles(N));
     }
     GF_LAUNCH_CHECK();
 }
-template <int N> void launch_z(const HalfIterArgs& a, hipStream_t st)
+template <int N> void launch_z(const HalfBatch& b, hipStream_t st)
 {
+    const HalfIterArgs& a = b.job[0];
     // tile + one double per wave for the errK partial
     const size_t lds = (size_t)kZP * N * sizeof(float2) + sizeof(double) * (zfused_threads<N>() / 64);
     set_lds(k_half_z_fused<N>, lds);
-    k_half_z_fused<N><<<(a.n1 / 2 + 1) * (a.n2 / kZP), zfused_threads<N>(), lds, st>>>(a, fft_twiddles(N),
-                                                                                      fft_twiddles(N));
+    k_half_z_fused<N><<<dim3((a.n1 / 2 + 1) * (a.n2 / kZP), b.count), zfused_threads<N>(), lds, st>>>(
+        b, fft_twiddles(N), fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
 
@@ -679,37 +693,67 @@ int64_t half_partials(int n1, int n2, int n3) { (void)n3; return (int64_t)(n1 / 
         default: raise(GENFIRE_ERR_UNSUPPORTED, "half-spectrum x length not supported"); \
     }
 
-void half_first(const HalfIterArgs& a, hipStream_t st)
+namespace {
+
+HalfBatch one(const HalfIterArgs& a)
 {
-#define C(NN) launch_first<NN>(a, st)
-    GF_HALF_SWITCH_X(a.n1, C)
+    HalfBatch b{};
+    b.job[0] = a;
+    b.count = 1;
+    return b;
+}
+
+void check_batch(const HalfBatch& b)
+{
+    GF_CHECK(b.count >= 1 && b.count <= kHalfBatchMax, GENFIRE_ERR_ARG, "half-spectrum batch of 1..8 jobs");
+    for (int i = 1; i < b.count; ++i)
+        GF_CHECK(b.job[i].n1 == b.job[0].n1 && b.job[i].n2 == b.job[0].n2 && b.job[i].n3 == b.job[0].n3,
+                 GENFIRE_ERR_ARG, "a half-spectrum batch needs jobs of one grid shape");
+}
+
+}  // namespace
+
+void half_first(const HalfBatch& b, hipStream_t st)
+{
+    check_batch(b);
+#define C(NN) launch_first<NN>(b, st)
+    GF_HALF_SWITCH_X(b.job[0].n1, C)
 #undef C
 }
-void half_y(const HalfIterArgs& a, int inverse, hipStream_t st)
+void half_y(const HalfBatch& b, int inverse, hipStream_t st)
 {
-#define C(NN) launch_y<NN>(a, inverse, st)
-    GF_HALF_SWITCH(a.n2, C)
+    check_batch(b);
+#define C(NN) launch_y<NN>(b, inverse, st)
+    GF_HALF_SWITCH(b.job[0].n2, C)
 #undef C
 }
-void half_z(const HalfIterArgs& a, hipStream_t st)
+void half_z(const HalfBatch& b, hipStream_t st)
 {
-#define C(NN) launch_z<NN>(a, st)
-    GF_HALF_SWITCH(a.n3, C)
+    check_batch(b);
+#define C(NN) launch_z<NN>(b, st)
+    GF_HALF_SWITCH(b.job[0].n3, C)
 #undef C
 }
-void half_last(const HalfIterArgs& a, hipStream_t st)
+void half_last(const HalfBatch& b, hipStream_t st)
 {
-#define C(NN) launch_last<NN>(a, st)
-    GF_HALF_SWITCH_X(a.n1, C)
+    check_batch(b);
+#define C(NN) launch_last<NN>(b, st)
+    GF_HALF_SWITCH_X(b.job[0].n1, C)
+#undef C
+}
+void half_last_first(const HalfBatch& b, hipStream_t st)
+{
+    check_batch(b);
+#define C(NN) launch_last_first<NN>(b, st)
+    GF_HALF_SWITCH_X(b.job[0].n1, C)
 #undef C
 }
 
-void half_last_first(const HalfIterArgs& a, hipStream_t st)
-{
-#define C(NN) launch_last_first<NN>(a, st)
-    GF_HALF_SWITCH_X(a.n1, C)
-#undef C
-}
+void half_first(const HalfIterArgs& a, hipStream_t st) { half_first(one(a), st); }
+void half_y(const HalfIterArgs& a, int inverse, hipStream_t st) { half_y(one(a), inverse, st); }
+void half_z(const HalfIterArgs& a, hipStream_t st) { half_z(one(a), st); }
+void half_last(const HalfIterArgs& a, hipStream_t st) { half_last(one(a), st); }
+void half_last_first(const HalfIterArgs& a, hipStream_t st) { half_last_first(one(a), st); }
 
 void hcsr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st)
 {

@@ -1,0 +1,189 @@
+// gf_job.h -- one reconstruction job (the state of one @GENFIRE_Reconstructor
+// object, GENFIRE_Reconstructor.m:5-71) and the host phases that run it on a
+// HIP stream.  Shared by the C ABI (gf_capi.hip) and the request server
+// (gf_serve.hip).
+#pragma once
+
+#include <atomic>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include "gf_grid.h"
+
+struct genfire_ctx {
+    int device = 0;
+    std::mutex mu;
+    std::string err;
+};
+
+namespace gf {
+
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    void init()
+    {
+        if (!a) {
+            GF_HIP(hipEventCreate(&a));
+            GF_HIP(hipEventCreate(&b));
+        }
+    }
+    EventTimer() = default;
+    EventTimer(const EventTimer&) = delete;
+    EventTimer& operator=(const EventTimer&) = delete;
+    ~EventTimer()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// Per-kernel event pairs for the live roofline measurement.
+struct KStats {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[GENFIRE_KSTAT_COUNT];
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    int64_t launches[GENFIRE_KSTAT_COUNT] = {};
+    double ms[GENFIRE_KSTAT_COUNT] = {};
+    std::pair<hipEvent_t, hipEvent_t> take();
+    void collect();
+    void reset();
+    ~KStats();
+};
+
+double mround(double x);                                  // MATLAB round
+std::vector<int> cutoff_schedule(int N, int m);           // round(linspace(1, N, m))
+std::vector<double> delay_indicators(int mode);           // GENFIRE_Reconstructor.m:156-165
+
+// Iteration state between the phases of reconstruct.m (set up by recon_begin).
+struct ReconState {
+    int N = 0, L = 1, nR = 0;
+    bool half = false, dr = false, monitor = false;
+    int64_t hn = 0, nparts = 0;
+    double den = 0.0;
+    std::vector<int> sched;
+    size_t cur = 0;
+    IterArgs ia{};
+    HalfIterArgs ha{};
+    EventTimer all, loop;
+    std::vector<std::unique_ptr<EventTimer>> rf;   // one pair per R-factor evaluation
+    bool begun = false;
+};
+
+}  // namespace gf
+
+struct genfire_job {
+    genfire_ctx* ctx = nullptr;
+    genfire_params prm{};
+    int dim1 = 0, dim2 = 0, P = 0, n1 = 0, n2 = 0;
+    gf::HalfGrid hg{};
+    std::vector<double> delays;
+    hipStream_t st = nullptr;       // the job's own stream
+    hipStream_t run_st = nullptr;   // stream its iterations run on (st, or a server's group stream)
+
+    // inputs
+    gf::DevBuf proj, support_box, init_box;
+    std::vector<double> angles;  // P x 3 column-major
+    bool have_inputs = false, have_support = false, have_init = false;
+
+    // gridding
+    gf::DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles, Pt, Phi, Plo, Pscale;
+    gf::DevBuf acc, sw, hits, exact;
+    gf::DevBuf M, mask, recifft;
+    std::vector<int64_t> proj_off;
+    int64_t npts = 0;
+    bool gridded = false, have_mask = false;
+
+    // iteration
+    gf::DevBuf x, K, S, best, pfbuf, err_part, errK, den_tmp, den, recon;
+    gf::DevBuf csr_cnt, csr_off, csr_meta, csr_val, csr_mag;
+    int64_t nmeas = 0;
+    gf::DevBuf vol, vol_y, Rmat, rsums, rout;
+    bool rmat_any_y = false;
+    std::vector<float> errK_h, errR_h;
+    std::vector<double> Rarr_h, Rarr2_h;
+    bool reconstructed = false;
+    gf::ReconState rs;
+
+    // timing
+    double t_grid = 0, t_iter = 0, t_rf = 0, t_total = 0;
+    bool prof = false;
+    gf::KStats ks;
+
+    // async
+    std::thread worker;
+    std::atomic<int> async_state{0};  // 0 idle, 1 running, 2 done
+    int async_status = 0;
+    std::string async_err;
+
+    ~genfire_job()
+    {
+        if (worker.joinable()) worker.join();
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace gf {
+
+void set_err(genfire_ctx* ctx, const std::string& m);
+void use_device(genfire_ctx* ctx);
+
+template <class F> int guarded(genfire_ctx* ctx, F&& f)
+{
+    try {
+        f();
+        return GENFIRE_OK;
+    } catch (const Error& e) {
+        set_err(ctx, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_err(ctx, "host allocation failed");
+        return GENFIRE_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_err(ctx, e.what());
+        return GENFIRE_ERR_ARG;
+    }
+}
+
+inline int64_t full_size(const genfire_job* j) { return (int64_t)j->n1 * j->n2 * j->n1; }
+inline int64_t box_size(const genfire_job* j) { return (int64_t)j->dim1 * j->dim2 * j->dim1; }
+
+// The checks of CheckPrepareData (GENFIRE_Reconstructor.m:115-171) and of this
+// build's size limits; raises with the reference's message where it has one.
+void validate_job(const genfire_params* p, int dim1, int dim2, int P);
+// Validates the parameters and fills sizes, schedule and stream of a fresh job.
+void job_setup(genfire_job* j, genfire_ctx* ctx, const genfire_params* p, int dim1, int dim2, int P);
+// Uploads the inputs (readFiles, :82-112); support and initial object may be null.
+void job_set_inputs(genfire_job* j, const float* projections, const double* angles, const uint8_t* support,
+                    const float* initial_object);
+// Device bytes a job of these sizes holds at its peak (an upper estimate).
+int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P);
+
+std::vector<float> proj_matrices(const double* angles, int P, bool* any_y);
+
+// fillInFourierGrid_DFT + CentroSymmetricity + runGridding post-processing.
+void do_grid(genfire_job* j);
+
+// reconstruct.m in phases.  recon_begin sets up (support, initial object,
+// measured-voxel lists, errK denominator); an iteration is recon_pre (schedule
+// stage, first x pass), the transform passes (recon_body for one job,
+// half_body_batch for several of one shape), and recon_post (errK, R-factor
+// monitor); recon_end crops the result and reads the records back.  All launch
+// on j->run_st.
+void recon_begin(genfire_job* j);
+void recon_pre(genfire_job* j, int it);
+void recon_body(genfire_job* j, int it);
+void recon_post(genfire_job* j, int it);
+void recon_end(genfire_job* j);
+void do_reconstruct(genfire_job* j);
+
+// The passes of iteration its[i] of every job (half-spectrum path, one shape)
+// as batched launches on st.
+void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStream_t st);
+
+// Frees what the iterations no longer need (point lists, hits, measuredK).
+void release_grid_extras(genfire_job* j);
+
+}  // namespace gf

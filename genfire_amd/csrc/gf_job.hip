@@ -1,0 +1,744 @@
+// gf_job.hip -- the phases of one reconstruction job: gridding
+// (fillInFourierGrid_DFT.m:85-310 + CentroSymmetricity.m:2-72 + runGridding,
+// GENFIRE_Reconstructor.m:172-186) and the iteration (reconstruct.m:28-194 and
+// the Douglas-Rachford variant), each on the job's HIP stream.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+
+#include "gf_job.h"
+
+namespace gf {
+
+[[noreturn]] void raise(int code, const std::string& msg) { throw Error{code, msg}; }
+
+// MATLAB round: half away from zero.
+double mround(double x) { return x < 0 ? -std::floor(-x + 0.5) : std::floor(x + 0.5); }
+
+// round(linspace(1, N, m)) (reconstruct.m:103) with MATLAB's linspace
+// d1 + (0:n1).*(d2-d1)./n1 and the exact endpoint.
+std::vector<int> cutoff_schedule(int N, int m)
+{
+    std::vector<int> out(m);
+    if (m == 1) { out[0] = N; return out; }
+    const double d1 = 1.0, d2 = (double)N;
+    const int n1 = m - 1;
+    for (int k = 0; k <= n1; ++k) {
+        double y = d1 + ((double)k * (d2 - d1)) / (double)n1;
+        if (k == n1) y = d2;
+        if (d1 == d2) y = d1;
+        out[k] = (int)mround(y);
+    }
+    return out;
+}
+
+namespace {
+
+// MATLAB colon a:d:b with MathWorks' symmetric evaluation (colonop).
+std::vector<double> colon(double a, double d, double b)
+{
+    const double tol = 2.0 * 2.220446049250313e-16 * std::max(std::fabs(a), std::fabs(b));
+    const double sig = d < 0 ? -1.0 : 1.0;
+    int n = (int)mround((b - a) / d);
+    if (sig * (a + n * d - b) > tol) --n;
+    std::vector<double> v;
+    if (n < 0) return v;
+    double c = a + n * d;
+    if (sig * (c - b) > -tol) c = b;
+    v.assign(n + 1, 0.0);
+    for (int k = 0; k <= n / 2; ++k) {
+        v[k] = a + k * d;
+        v[n - k] = c - k * d;
+    }
+    if (n % 2 == 0) v[n / 2] = (a + c) / 2;
+    return v;
+}
+
+}  // namespace
+
+// constraintEnforcementDelayIndicators, GENFIRE_Reconstructor.m:156-165.
+std::vector<double> delay_indicators(int mode)
+{
+    std::vector<double> v;
+    if (mode == 1) {
+        v = colon(0.95, -0.1, -0.15);
+        v.insert(v.end(), {-10.0, -10.0, -10.0});
+        auto up = colon(-0.15, 0.1, 0.95);
+        v.insert(v.end(), up.begin(), up.end());
+    } else if (mode == 2) {
+        v = colon(0.95, -0.1, -0.15);
+        v.insert(v.end(), {-10.0, -10.0});
+    } else if (mode == 3) {
+        v.assign(5, -999.0);
+    } else {
+        raise(GENFIRE_ERR_ARG, "GENFIRE: ERROR! constraintEnforcementMode value " + std::to_string(mode) +
+                                   " not understood");
+    }
+    return v;
+}
+
+std::pair<hipEvent_t, hipEvent_t> KStats::take()
+{
+    if (!pool.empty()) {
+        auto e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t a, b;
+    GF_HIP(hipEventCreate(&a));
+    GF_HIP(hipEventCreate(&b));
+    return {a, b};
+}
+
+void KStats::collect()
+{
+    for (int k = 0; k < GENFIRE_KSTAT_COUNT; ++k) {
+        for (auto& e : pending[k]) {
+            float t = 0.0f;
+            GF_HIP(hipEventSynchronize(e.second));
+            GF_HIP(hipEventElapsedTime(&t, e.first, e.second));
+            ms[k] += t;
+            launches[k] += 1;
+            pool.push_back(e);
+        }
+        pending[k].clear();
+    }
+}
+
+void KStats::reset()
+{
+    for (int k = 0; k < GENFIRE_KSTAT_COUNT; ++k) { launches[k] = 0; ms[k] = 0.0; }
+}
+
+KStats::~KStats()
+{
+    for (auto& e : pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto& v : pending)
+        for (auto& e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+}
+
+void set_err(genfire_ctx* ctx, const std::string& m)
+{
+    if (!ctx) return;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->err = m;
+}
+
+void use_device(genfire_ctx* ctx) { GF_HIP(hipSetDevice(ctx->device)); }
+
+void kstat_begin(genfire_job* j, int k, std::pair<hipEvent_t, hipEvent_t>* ev)
+{
+    if (!j->prof) return;
+    *ev = j->ks.take();
+    GF_HIP(hipEventRecord(ev->first, j->run_st));
+    (void)k;
+}
+
+void kstat_end(genfire_job* j, int k, std::pair<hipEvent_t, hipEvent_t>& ev)
+{
+    if (!j->prof) return;
+    GF_HIP(hipEventRecord(ev.second, j->run_st));
+    j->ks.pending[k].push_back(ev);
+}
+
+// ZYZ matrices of the projector (calculate3Dprojection_RealSpaceinterp.m:13-15),
+// kRStride floats per projection with the y-fastest flag at [9].
+std::vector<float> proj_matrices(const double* angles, int P, bool* any_y)
+{
+    std::vector<float> Rm((size_t)kRStride * P, 0.0f);
+    *any_y = false;
+    for (int p = 0; p < P; ++p) {
+        double R[9];
+        zyz_matrix(angles[p], angles[P + p], angles[2 * P + p], R);
+        float* r = Rm.data() + (size_t)kRStride * p;
+        for (int q = 0; q < 9; ++q) r[q] = (float)R[q];
+        if (project_wants_yfast(r)) { r[9] = 1.0f; *any_y = true; }
+    }
+    return Rm;
+}
+
+// ---------------------------------------------------------------------------
+// fillInFourierGrid_DFT + CentroSymmetricity + runGridding post-processing
+// ---------------------------------------------------------------------------
+void do_grid(genfire_job* j)
+{
+    GF_CHECK(j->have_inputs, GENFIRE_ERR_STATE, "GENFIRE: inputs not set (call genfire_job_set_inputs)");
+    const int P = j->P, n1 = j->n1, n2 = j->n2;
+    const int centro = j->prm.DFT_CentroSymmetricity ? 1 : 0;
+    hipStream_t st = j->st;
+    EventTimer tm;
+    tm.init();
+    GF_HIP(hipEventRecord(tm.a, st));
+
+    // rotation matrices and plane constants (fillInFourierGrid_DFT.m:100-118)
+    std::vector<PlaneConst> pcs(P);
+    int64_t max_cols = 1;
+    for (int p = 0; p < P; ++p) {
+        float Rf[9];
+        rotation_f32(j->prm.vector1, j->prm.vector2, j->prm.vector3, j->angles[p], j->angles[P + p],
+                     j->angles[2 * P + p], Rf);
+        plane_setup(Rf, &pcs[p]);
+        const int a = pcs[p].axis, b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        max_cols = std::max<int64_t>(max_cols, (int64_t)j->hg.len[b] * j->hg.len[c]);
+    }
+    j->planes.alloc(sizeof(PlaneConst) * P);
+    GF_HIP(hipMemcpyAsync(j->planes.p, pcs.data(), sizeof(PlaneConst) * P, hipMemcpyHostToDevice, st));
+
+    const int nthr = index_threads();
+    const int nblk = (int)((max_cols + nthr - 1) / nthr);
+    const int64_t nb_total = (int64_t)nblk * P;
+    j->blk_counts.alloc(sizeof(uint32_t) * nb_total);
+    j->blk_offsets.alloc(sizeof(int64_t) * nb_total);
+    j->total.alloc(sizeof(int64_t));
+    const double cutoff = j->prm.interpolationCutoffDistance;
+    grid_index_count(j->planes.as<PlaneConst>(), P, j->hg, cutoff, n1, n2, nblk,
+                     j->blk_counts.as<uint32_t>(), st);
+    grid_index_scan(j->blk_counts.as<uint32_t>(), nb_total, j->blk_offsets.as<int64_t>(),
+                    j->total.as<int64_t>(), st);
+    std::vector<int64_t> offs(nb_total);
+    int64_t npts = 0;
+    GF_HIP(hipMemcpyAsync(offs.data(), j->blk_offsets.p, sizeof(int64_t) * nb_total, hipMemcpyDeviceToHost, st));
+    GF_HIP(hipMemcpyAsync(&npts, j->total.p, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    j->npts = npts;
+    j->proj_off.assign(P + 1, npts);
+    for (int p = 0; p < P; ++p) j->proj_off[p] = offs[(int64_t)p * nblk];
+
+    const int64_t cap = std::max<int64_t>(npts, 1);
+    j->ind.alloc(sizeof(uint32_t) * cap);
+    j->dist.alloc(sizeof(float) * cap);
+    j->un.alloc(sizeof(float) * cap);
+    j->vn.alloc(sizeof(float) * cap);
+    j->F.alloc(sizeof(float2) * cap);
+    grid_index_emit(j->planes.as<PlaneConst>(), P, j->hg, cutoff, n1, n2, nblk, j->blk_offsets.as<int64_t>(),
+                    j->ind.as<uint32_t>(), j->dist.as<float>(), j->un.as<float>(), j->vn.as<float>(), st);
+
+    // DFT values (:236-248).  Default: the MFMA GEMM form on 64-point tiles in
+    // the job's precision; with FP32, GENFIRE_GRID_KERNEL=valu selects the VALU
+    // kernel (256-point tiles).
+    const int prec = j->prm.precision;
+    const char* gk = std::getenv("GENFIRE_GRID_KERNEL");
+    const bool use_valu = prec == GENFIRE_PREC_FP32 && gk && std::string(gk) == "valu";
+    const int tile_pts = use_valu ? 256 : mfma_tile_points();
+    std::vector<PointTile> tl;
+    for (int p = 0; p < P; ++p)
+        for (int64_t b = j->proj_off[p]; b < j->proj_off[p + 1]; b += tile_pts)
+            tl.push_back({b, (int32_t)std::min<int64_t>(tile_pts, j->proj_off[p + 1] - b), p});
+    if (!tl.empty()) {
+        j->tiles.alloc(sizeof(PointTile) * tl.size());
+        GF_HIP(hipMemcpyAsync(j->tiles.p, tl.data(), sizeof(PointTile) * tl.size(), hipMemcpyHostToDevice, st));
+        if (prec != GENFIRE_PREC_FP32) {
+            const size_t ne = grid_prep_lp_elems(j->dim1, j->dim2, P);
+            j->Phi.alloc(sizeof(uint16_t) * ne);
+            if (prec == GENFIRE_PREC_SPLIT_BF16) j->Plo.alloc(sizeof(uint16_t) * ne);
+            j->Pscale.alloc(sizeof(float) * P);
+            grid_prep_lp(prec, j->proj.as<float>(), j->dim1, j->dim2, P, j->Phi.as<uint16_t>(),
+                         j->Plo.as<uint16_t>(), j->Pscale.as<float>(), st);
+        } else if (!use_valu) {
+            j->Pt.alloc(sizeof(float) * grid_prep_floats(j->dim1, j->dim2, P));
+            grid_prep_proj(j->proj.as<float>(), j->dim1, j->dim2, P, j->Pt.as<float>(), st);
+        }
+        std::pair<hipEvent_t, hipEvent_t> ev;
+        kstat_begin(j, GENFIRE_KSTAT_GRID_VALUES, &ev);
+        if (prec != GENFIRE_PREC_FP32)
+            grid_values_mfma_lp(prec, j->Phi.as<uint16_t>(), j->Plo.as<uint16_t>(), j->Pscale.as<float>(), j->dim1,
+                                j->dim2, j->un.as<float>(), j->vn.as<float>(), j->tiles.as<PointTile>(),
+                                (int)tl.size(), j->F.as<float2>(), st);
+        else if (use_valu)
+            grid_values_fp32(j->proj.as<float>(), j->dim1, j->dim2, j->un.as<float>(), j->vn.as<float>(),
+                             j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
+        else
+            grid_values_mfma_fp32(j->Pt.as<float>(), j->dim1, j->dim2, j->un.as<float>(), j->vn.as<float>(),
+                                  j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
+        kstat_end(j, GENFIRE_KSTAT_GRID_VALUES, ev);
+    }
+
+    // weighted average (:267-291), projection by projection in list order
+    const int64_t ht = j->hg.total();
+    j->acc.alloc(sizeof(double2) * ht);
+    j->sw.alloc(sizeof(double) * ht);
+    j->hits.alloc(sizeof(uint32_t) * ht);
+    j->exact.alloc(sizeof(float2) * ht);
+    GF_HIP(hipMemsetAsync(j->acc.p, 0, j->acc.bytes, st));
+    GF_HIP(hipMemsetAsync(j->sw.p, 0, j->sw.bytes, st));
+    GF_HIP(hipMemsetAsync(j->hits.p, 0, j->hits.bytes, st));
+    for (int p = 0; p < P; ++p)
+        grid_scatter(j->ind.as<uint32_t>(), j->dist.as<float>(), j->F.as<float2>(), j->proj_off[p],
+                     j->proj_off[p + 1] - j->proj_off[p], j->acc.as<double2>(), j->sw.as<double>(),
+                     j->hits.as<uint32_t>(), j->exact.as<float2>(), st);
+
+    // CentroSymmetricity + single cast, straight into FFT order
+    j->M.alloc(sizeof(float2) * full_size(j));
+    grid_finalize(j->hg, n1, n2, centro, j->acc.as<double2>(), j->sw.as<double>(), j->hits.as<uint32_t>(),
+                  j->exact.as<float2>(), j->M.as<float2>(), st);
+    j->have_mask = false;
+    j->mask.release();
+
+    // recIFFT = crop(real(fftshift(ifftn(ifftshift(measuredK))))) (:304)
+    if (j->prm.compute_recIFFT) {
+        j->K.alloc(sizeof(float2) * full_size(j));
+        GF_HIP(hipMemcpyAsync(j->K.p, j->M.p, j->M.bytes, hipMemcpyDeviceToDevice, st));
+        fft3(j->K.as<float2>(), n1, n2, n1, 1, st);
+        j->recifft.alloc(sizeof(float) * box_size(j));
+        crop_centre(j->K.p, 1, n1, n2, n1, j->recifft.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    }
+    GF_HIP(hipEventRecord(tm.b, st));
+    GF_HIP(hipEventSynchronize(tm.b));
+    float ms = 0.0f;
+    GF_HIP(hipEventElapsedTime(&ms, tm.a, tm.b));
+    j->t_grid = ms;
+    // the FP64 accumulators and the GEMM operands are only needed for the values above
+    j->acc.release();
+    j->sw.release();
+    j->exact.release();
+    j->Pt.release();
+    j->Phi.release();
+    j->Plo.release();
+    j->gridded = true;
+    j->reconstructed = false;
+}
+
+// ---------------------------------------------------------------------------
+// reconstruct.m:28-194 (GENFIRE) and the Douglas-Rachford variant
+// ---------------------------------------------------------------------------
+namespace {
+
+// R-factor monitor (reconstruct.m:168-185): model = crop(fftshift(x)) then, for
+// every projection, the sums of R1 = sum|back-pj|/sum|pj| and
+// R2 = ||back-pj||_F/||pj||_F, kept on the device until recon_end.
+void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
+{
+    hipStream_t st = j->run_st;
+    const int P = j->P;
+    EventTimer& tm = *j->rs.rf[slot];
+    GF_HIP(hipEventRecord(tm.a, st));
+    crop_centre(d_model_fft, 0, j->n1, j->n2, j->n1, j->vol.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    if (j->rmat_any_y) volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    const int nblk = project_blocks(j->dim1, j->dim2);
+    ProjArgs pa{};
+    pa.vol = j->vol.as<float>();
+    pa.vol_y = j->rmat_any_y ? j->vol_y.as<float>() : nullptr;
+    pa.d1 = j->dim1; pa.d2 = j->dim2; pa.d3 = j->dim1;
+    pa.R = j->Rmat.as<float>();
+    pa.P = P;
+    pa.out = nullptr;
+    pa.meas = j->proj.as<float>();
+    pa.m1 = j->dim1; pa.m2 = j->dim2;
+    pa.sums = j->rsums.as<double>();
+    pa.post_hoc = 0;
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    kstat_begin(j, GENFIRE_KSTAT_PROJECT, &ev);
+    project(pa, st);
+    kstat_end(j, GENFIRE_KSTAT_PROJECT, ev);
+    project_reduce(j->rsums.as<double>(), P, nblk, j->rout.as<double>() + (size_t)slot * P * 4, st);
+    GF_HIP(hipEventRecord(tm.b, st));
+}
+
+HalfIterArgs p5_args(const genfire_job* j, int it)
+{
+    const ReconState& r = j->rs;
+    HalfIterArgs ha = r.ha;
+    const bool mon_it = r.monitor && (it % r.L == 0);
+    ha.pf_out = mon_it ? const_cast<float*>(j->pfbuf.as<float>()) : nullptr;
+    ha.best = (it + 1 == r.N) ? const_cast<float*>(j->best.as<float>()) : nullptr;
+    return ha;
+}
+
+}  // namespace
+
+void recon_begin(genfire_job* j)
+{
+    GF_CHECK(j->gridded, GENFIRE_ERR_STATE, "GENFIRE: measuredK not available (call genfire_job_grid first)");
+    ReconState& r = j->rs;
+    r.N = j->prm.numIterations;
+    const int N = r.N;
+    const int n1 = j->n1, n2 = j->n2, n3 = j->n1;
+    const int64_t nf = full_size(j);
+    if (!j->run_st) j->run_st = j->st;
+    hipStream_t st = j->run_st;
+    r.dr = j->prm.algorithm == GENFIRE_ALG_DR;
+    r.monitor = j->prm.monitor_R != 0;
+    r.L = std::max(1, j->prm.monitorR_loopLength);
+    const bool dr = r.dr, monitor = r.monitor;
+
+    r.half = half_supported(n1, n2, n3);
+    const bool half = r.half;
+    r.hn = (int64_t)(n1 / 2 + 1) * n2 * n3;   // half-spectrum size
+    const int64_t hn = r.hn;
+    r.all.init();
+    r.loop.init();
+    GF_HIP(hipEventRecord(r.all.a, st));
+    j->x.alloc(sizeof(float) * nf);
+    j->K.alloc(sizeof(float2) * (half ? std::max<int64_t>(2 * hn, nf) : nf));
+    j->S.alloc(nf);
+    j->best.alloc(sizeof(float) * nf);
+    if (monitor && (dr || half)) j->pfbuf.alloc(sizeof(float) * nf);
+    IterArgs& ia = r.ia;
+    ia = IterArgs{};
+    ia.n1 = n1; ia.n2 = n2; ia.n3 = n3;
+    GF_CHECK(n1 <= 2048 && n3 <= 2048, GENFIRE_ERR_UNSUPPORTED, "GENFIRE: grid edge above 2048");
+    r.nparts = half ? half_partials(n1, n2, n3) : iter_fused_partials(ia);
+    j->err_part.alloc(sizeof(double) * r.nparts);
+    j->errK.alloc(sizeof(float) * std::max(N, 1));
+    const int ntmp = 1024;
+    j->den_tmp.alloc(sizeof(double) * ntmp);
+    j->den.alloc(sizeof(double));
+
+    // padded support (reconstruct.m:34) and initial object (:76-80), ifftshifted (:109-110)
+    if (j->have_support) {
+        pad_centre_u8(j->support_box.as<uint8_t>(), j->dim1, j->dim2, j->dim1, j->S.as<uint8_t>(), n1, n2, n3, st);
+    } else {
+        // Support = ones(Dim1,Dim2,Dim1) (GENFIRE_Reconstructor.m:146-148), padded
+        DevBuf ones;
+        ones.alloc(box_size(j));
+        GF_HIP(hipMemsetAsync(ones.p, 1, ones.bytes, st));
+        pad_centre_u8(ones.as<uint8_t>(), j->dim1, j->dim2, j->dim1, j->S.as<uint8_t>(), n1, n2, n3, st);
+        GF_HIP(hipStreamSynchronize(st));
+    }
+    if (j->have_init)
+        pad_centre(j->init_box.as<float>(), j->dim1, j->dim2, j->dim1, j->x.as<float>(), n1, n2, n3, st);
+    else
+        GF_HIP(hipMemsetAsync(j->x.p, 0, j->x.bytes, st));
+
+    r.sched = cutoff_schedule(N, (int)j->delays.size());
+    r.cur = 0;
+    GF_CHECK(N >= 1 && r.sched.back() == N, GENFIRE_ERR_ARG, "GENFIRE: numIterations must be >= 1");
+
+    // measured voxels (errInd, reconstruct.m:99) as per-pencil lists
+    const uint8_t* mask = j->have_mask ? j->mask.as<uint8_t>() : nullptr;
+    const int64_t npen = half ? (int64_t)(n1 / 2 + 1) * n2 : (int64_t)n1 * n2;
+    j->csr_cnt.alloc(sizeof(uint32_t) * npen);
+    j->csr_off.alloc(sizeof(int64_t) * (npen + 1));
+    if (half) hcsr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
+    else csr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
+    grid_index_scan(j->csr_cnt.as<uint32_t>(), npen, j->csr_off.as<int64_t>(), j->csr_off.as<int64_t>() + npen, st);
+    GF_HIP(hipMemcpyAsync(&j->nmeas, j->csr_off.as<int64_t>() + npen, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    const int64_t ne = std::max<int64_t>(j->nmeas, 1);
+    j->csr_meta.alloc(sizeof(uint32_t) * ne);
+    j->csr_val.alloc(sizeof(float2) * ne);
+    if (half) {
+        j->csr_mag.alloc(sizeof(float2) * ne);
+        hcsr_fill(j->M.as<float2>(), mask, n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(),
+                  j->csr_val.as<float2>(), j->csr_mag.as<float2>(), st);
+        hcsr_den(j->csr_mag.as<float2>(), j->csr_meta.as<uint32_t>(), j->nmeas, j->den_tmp.as<double>(), ntmp,
+                 j->den.as<double>(), st);
+    } else {
+        csr_fill(j->M.as<float2>(), mask, n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(),
+                 j->csr_val.as<float2>(), st);
+        csr_den(j->csr_val.as<float2>(), j->nmeas, j->den_tmp.as<double>(), ntmp, j->den.as<double>(), st);
+    }
+    // errK denominator: sum |M(errInd)| (reconstruct.m:135)
+    GF_HIP(hipMemcpyAsync(&r.den, j->den.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+
+    // R-factor monitor buffers and projection matrices
+    r.nR = monitor ? N / r.L : 0;
+    j->errR_h.assign(r.nR, 0.0f);
+    j->Rarr_h.assign((size_t)r.nR * j->P, 0.0);
+    j->Rarr2_h.assign((size_t)r.nR * j->P, 0.0);
+    j->t_rf = 0.0;
+    while ((int)r.rf.size() < r.nR) {
+        r.rf.emplace_back(new EventTimer());
+        r.rf.back()->init();
+    }
+    if (r.nR > 0) {
+        std::vector<float> Rm = proj_matrices(j->angles.data(), j->P, &j->rmat_any_y);
+        j->Rmat.alloc(sizeof(float) * Rm.size());
+        GF_HIP(hipMemcpyAsync(j->Rmat.p, Rm.data(), sizeof(float) * Rm.size(), hipMemcpyHostToDevice, st));
+        j->vol.alloc(sizeof(float) * box_size(j));
+        if (j->rmat_any_y) j->vol_y.alloc(sizeof(float) * box_size(j));
+        j->rsums.alloc(sizeof(double) * (size_t)j->P * 4 * project_blocks(j->dim1, j->dim2));
+        j->rout.alloc(sizeof(double) * (size_t)r.nR * j->P * 4);
+        GF_HIP(hipStreamSynchronize(st));
+    }
+
+    ia.x = j->x.as<float>();
+    ia.K = j->K.as<float2>();
+    ia.S = j->S.as<uint8_t>();
+    ia.csr_off = j->csr_off.as<int64_t>();
+    ia.csr_meta = j->csr_meta.as<uint32_t>();
+    ia.csr_val = j->csr_val.as<float2>();
+    ia.err_part = j->err_part.as<double>();
+    ia.positivity = j->prm.constraintPositivity ? 1 : 0;
+    ia.support = j->prm.constraintSupport ? 1 : 0;
+    ia.dr = dr ? 1 : 0;
+    HalfIterArgs& ha = r.ha;
+    ha = HalfIterArgs{};
+    ha.x = ia.x;
+    ha.A = j->K.as<float2>();
+    ha.B = j->K.as<float2>() + hn;
+    ha.S = ia.S;
+    ha.csr_off = ia.csr_off;
+    ha.csr_meta = ia.csr_meta;
+    ha.csr_val = ia.csr_val;
+    ha.csr_mag = j->csr_mag.as<float2>();
+    ha.err_part = ia.err_part;
+    ha.n1 = n1; ha.n2 = n2; ha.n3 = n3;
+    ha.positivity = ia.positivity;
+    ha.support = ia.support;
+    ha.dr = ia.dr;
+    GF_HIP(hipEventRecord(r.loop.a, st));
+    r.begun = true;
+}
+
+void recon_pre(genfire_job* j, int it)
+{
+    ReconState& r = j->rs;
+    hipStream_t st = j->run_st;
+    const int n1 = j->n1, n2 = j->n2, n3 = j->n1;
+    if (r.cur < r.sched.size() && it == r.sched[r.cur]) {          // :118-124
+        size_t last = r.cur;
+        for (size_t q = r.cur; q < r.sched.size(); ++q)
+            if (r.sched[q] == it) last = q;
+        if (r.half) hcsr_enforce(n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(), j->delays[last], st);
+        else csr_enforce(n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(), j->delays[last], st);
+        r.cur = last + 1;
+    }
+    // The schedule always ends with a change at it == N, which resets bestErr
+    // (:123), so the returned reconstruction is P_R(x) of the last iteration:
+    // it is written there and nowhere else.  P1 runs on its own only in the
+    // first iteration; later ones get it from the fused P5 + P1 pass that ends
+    // the iteration before.
+    if (r.half && it == 1) {
+        HalfIterArgs ha = r.ha;
+        ha.best = (r.N == 1) ? j->best.as<float>() : nullptr;
+        ha.pf_out = nullptr;
+        half_first(ha, st);
+    }
+}
+
+void recon_body(genfire_job* j, int it)
+{
+    ReconState& r = j->rs;
+    hipStream_t st = j->run_st;
+    const int n1 = j->n1, n2 = j->n2, n3 = j->n1;
+    std::pair<hipEvent_t, hipEvent_t> e_pass, e_fused;
+    if (r.half) {
+        kstat_begin(j, GENFIRE_KSTAT_FFT_PASS, &e_pass);
+        half_y(r.ha, 0, st);
+        kstat_end(j, GENFIRE_KSTAT_FFT_PASS, e_pass);
+        kstat_begin(j, GENFIRE_KSTAT_FFT_FUSED, &e_fused);
+        half_z(r.ha, st);
+        kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
+        half_y(r.ha, 1, st);
+        if (it < r.N) half_last_first(p5_args(j, it), st);
+        else half_last(p5_args(j, it), st);
+    } else {
+        IterArgs ia = r.ia;
+        const bool mon_it = r.monitor && (it % r.L == 0);
+        ia.best = (it == r.N) ? j->best.as<float>() : nullptr;
+        ia.pf_out = (mon_it && r.dr) ? j->pfbuf.as<float>() : nullptr;
+        iter_first(ia, st);
+        kstat_begin(j, GENFIRE_KSTAT_FFT_PASS, &e_pass);
+        fft_axis(j->K.as<float2>(), n1, n2, n3, 1, 0, st);
+        kstat_end(j, GENFIRE_KSTAT_FFT_PASS, e_pass);
+        kstat_begin(j, GENFIRE_KSTAT_FFT_FUSED, &e_fused);
+        iter_fused(ia, st);
+        kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
+        fft_axis(j->K.as<float2>(), n1, n2, n3, 1, 1, st);
+        iter_last(ia, st);
+    }
+}
+
+void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStream_t st)
+{
+    for (int c0 = 0; c0 < count; c0 += kHalfBatchMax) {
+        const int c1 = std::min(count, c0 + kHalfBatchMax);
+        HalfBatch all{}, cont{}, fin{};
+        for (int i = c0; i < c1; ++i) {
+            const genfire_job* j = jobs[i];
+            GF_CHECK(j->rs.half, GENFIRE_ERR_ARG, "half_body_batch: job is not on the half-spectrum path");
+            all.job[all.count++] = j->rs.ha;
+            if (its[i] < j->rs.N) cont.job[cont.count++] = p5_args(j, its[i]);
+            else fin.job[fin.count++] = p5_args(j, its[i]);
+        }
+        half_y(all, 0, st);
+        half_z(all, st);
+        half_y(all, 1, st);
+        if (cont.count) half_last_first(cont, st);
+        if (fin.count) half_last(fin, st);
+    }
+}
+
+void recon_post(genfire_job* j, int it)
+{
+    ReconState& r = j->rs;
+    iter_errk_reduce(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>() + (it - 1), j->run_st);
+    if (r.monitor && (it % r.L == 0) && it / r.L - 1 < r.nR) {
+        const float* model = (r.dr || r.half) ? j->pfbuf.as<float>() : j->x.as<float>();
+        run_rfactor(j, model, it / r.L - 1);
+    }
+}
+
+void recon_end(genfire_job* j)
+{
+    ReconState& r = j->rs;
+    hipStream_t st = j->run_st;
+    const int N = r.N;
+    GF_HIP(hipEventRecord(r.loop.b, st));
+    // reconstruction = My_stripzero(fftshift(best), [Dim1 Dim2 Dim1]) (:192)
+    j->recon.alloc(sizeof(float) * box_size(j));
+    crop_centre(j->best.p, 0, j->n1, j->n2, j->n1, j->recon.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    j->errK_h.assign(N, 0.0f);
+    GF_HIP(hipMemcpyAsync(j->errK_h.data(), j->errK.p, sizeof(float) * N, hipMemcpyDeviceToHost, st));
+    std::vector<double> h((size_t)r.nR * j->P * 4);
+    if (r.nR > 0) GF_HIP(hipMemcpyAsync(h.data(), j->rout.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    GF_HIP(hipEventRecord(r.all.b, st));
+    GF_HIP(hipEventSynchronize(r.all.b));
+    const int P = j->P;
+    j->t_rf = 0.0;
+    for (int slot = 0; slot < r.nR; ++slot) {
+        double mean = 0.0;
+        const double* s = h.data() + (size_t)slot * P * 4;
+        for (int p = 0; p < P; ++p) {
+            const double r1 = s[4 * p + 0] / s[4 * p + 2];
+            const double r2 = std::sqrt(s[4 * p + 1]) / std::sqrt(s[4 * p + 3]);
+            j->Rarr_h[(size_t)slot * P + p] = r1;
+            j->Rarr2_h[(size_t)slot * P + p] = r2;
+            mean += r1;
+        }
+        j->errR_h[slot] = (float)(mean / P);
+        float ms = 0.0f;
+        GF_HIP(hipEventElapsedTime(&ms, r.rf[slot]->a, r.rf[slot]->b));
+        j->t_rf += ms;
+    }
+    float ms = 0.0f;
+    GF_HIP(hipEventElapsedTime(&ms, r.loop.a, r.loop.b));
+    j->t_iter = ms - j->t_rf;
+    GF_HIP(hipEventElapsedTime(&ms, r.all.a, r.all.b));
+    j->t_total = j->t_grid + ms;
+    if (j->prof) j->ks.collect();
+    j->reconstructed = true;
+    r.begun = false;
+}
+
+void do_reconstruct(genfire_job* j)
+{
+    j->run_st = j->st;
+    recon_begin(j);
+    std::pair<hipEvent_t, hipEvent_t> e_it;
+    for (int it = 1; it <= j->rs.N; ++it) {
+        kstat_begin(j, GENFIRE_KSTAT_ITERATION, &e_it);
+        recon_pre(j, it);
+        recon_body(j, it);
+        iter_errk_reduce(j->err_part.as<double>(), j->rs.nparts, j->rs.den, j->errK.as<float>() + (it - 1), j->run_st);
+        kstat_end(j, GENFIRE_KSTAT_ITERATION, e_it);
+        if (j->rs.monitor && (it % j->rs.L == 0) && it / j->rs.L - 1 < j->rs.nR) {
+            const float* model = (j->rs.dr || j->rs.half) ? j->pfbuf.as<float>() : j->x.as<float>();
+            run_rfactor(j, model, it / j->rs.L - 1);
+        }
+    }
+    recon_end(j);
+}
+
+void release_grid_extras(genfire_job* j)
+{
+    j->ind.release();
+    j->dist.release();
+    j->un.release();
+    j->vn.release();
+    j->F.release();
+    j->tiles.release();
+    j->hits.release();
+    j->blk_counts.release();
+    j->blk_offsets.release();
+    j->recifft.release();
+    j->M.release();
+    j->mask.release();
+    j->csr_cnt.release();
+    j->den_tmp.release();
+}
+
+// ---------------------------------------------------------------------------
+// Job set-up (CheckPrepareData, GENFIRE_Reconstructor.m:115-171) and inputs
+// ---------------------------------------------------------------------------
+void validate_job(const genfire_params* p, int dim1, int dim2, int P)
+{
+    GF_CHECK(dim1 >= 2 && dim2 >= 2 && P >= 1, GENFIRE_ERR_ARG,
+             "GENFIRE: projections must be at least 2 x 2 and P >= 1");
+    GF_CHECK(p->numIterations >= 1, GENFIRE_ERR_ARG, "GENFIRE: numIterations must be >= 1");
+    GF_CHECK(p->oversamplingRatio > 0, GENFIRE_ERR_ARG, "GENFIRE: oversamplingRatio must be positive");
+    GF_CHECK(p->algorithm == GENFIRE_ALG_GENFIRE || p->algorithm == GENFIRE_ALG_DR, GENFIRE_ERR_ARG,
+             "GENFIRE: unknown algorithm");
+    GF_CHECK(p->precision == GENFIRE_PREC_FP32 || p->precision == GENFIRE_PREC_SPLIT_BF16 ||
+                 p->precision == GENFIRE_PREC_FP16,
+             GENFIRE_ERR_ARG, "GENFIRE: unknown gridding precision");
+    const int n1 = (int)mround(dim1 * p->oversamplingRatio), n2 = (int)mround(dim2 * p->oversamplingRatio);
+    GF_CHECK(n1 >= dim1 && n2 >= dim2, GENFIRE_ERR_ARG, "GENFIRE: oversampled size smaller than projection");
+    GF_CHECK(fft_supported(n1) && fft_supported(n2), GENFIRE_ERR_UNSUPPORTED,
+             "GENFIRE: oversampled grid " + std::to_string(n1) + "x" + std::to_string(n2) +
+                 " has an FFT length this build does not support");
+    (void)delay_indicators(p->constraintEnforcementMode);   // raises the reference's message for a bad mode
+}
+
+void job_setup(genfire_job* j, genfire_ctx* ctx, const genfire_params* p, int dim1, int dim2, int P)
+{
+    validate_job(p, dim1, dim2, P);
+    j->ctx = ctx;
+    j->prm = *p;
+    j->dim1 = dim1;
+    j->dim2 = dim2;
+    j->P = P;
+    j->n1 = (int)mround(dim1 * p->oversamplingRatio);   // :168
+    j->n2 = (int)mround(dim2 * p->oversamplingRatio);   // :169
+    j->hg = half_grid(j->n1, j->n2, p->DFT_CentroSymmetricity ? 1 : 0);
+    j->delays = delay_indicators(p->constraintEnforcementMode);
+    GF_HIP(hipStreamCreateWithFlags(&j->st, hipStreamNonBlocking));
+    j->run_st = j->st;
+}
+
+void job_set_inputs(genfire_job* j, const float* projections, const double* angles, const uint8_t* support,
+                    const float* initial_object)
+{
+    GF_CHECK(projections && angles, GENFIRE_ERR_ARG, "GENFIRE: projections and angles are required");
+    const size_t np = (size_t)j->dim1 * j->dim2 * j->P;
+    j->proj.alloc(sizeof(float) * np);
+    GF_HIP(hipMemcpyAsync(j->proj.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, j->st));
+    j->angles.assign(angles, angles + 3 * (size_t)j->P);
+    j->have_support = support != nullptr;
+    if (support) {
+        j->support_box.alloc(box_size(j));
+        GF_HIP(hipMemcpyAsync(j->support_box.p, support, box_size(j), hipMemcpyHostToDevice, j->st));
+    }
+    j->have_init = initial_object != nullptr;
+    if (initial_object) {
+        j->init_box.alloc(sizeof(float) * box_size(j));
+        GF_HIP(hipMemcpyAsync(j->init_box.p, initial_object, sizeof(float) * box_size(j), hipMemcpyHostToDevice,
+                              j->st));
+    }
+    GF_HIP(hipStreamSynchronize(j->st));
+    j->Rmat.release();
+    j->have_inputs = true;
+    j->gridded = false;
+    j->reconstructed = false;
+}
+
+// Upper estimate of the device bytes a job holds at its peak: the larger of the
+// gridding phase (point lists, FP64 half-grid accumulators, measuredK and the
+// recIFFT transform) and the iteration phase (iterate, half or full spectrum,
+// support, best, monitor model, measured-voxel lists), plus the inputs.
+int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P)
+{
+    const int64_t n1 = (int64_t)mround(dim1 * p.oversamplingRatio), n2 = (int64_t)mround(dim2 * p.oversamplingRatio);
+    const int64_t n3 = n1, nf = n1 * n2 * n3, box = (int64_t)dim1 * dim2 * dim1;
+    const int64_t half_grid_vox = (n1 / 2 + 1) * (n2 + 1) * (n3 + 1);
+    const double cut = std::max(0.05, p.interpolationCutoffDistance);
+    const int64_t pts = (int64_t)((1.3 * cut + 0.05) * (double)(n1 * n2) * P) + 4096;
+    const int64_t kp = (dim1 + 63) / 64 * 64, np = (dim2 + 15) / 16 * 16;
+    const int64_t inputs = (int64_t)4 * dim1 * dim2 * P + box * 5;
+    const int64_t grid = pts * 28 + kp * np * P * 4 + half_grid_vox * 36 + nf * (8 + 1 + 8) + box * 4;
+    const bool half = half_supported((int)n1, (int)n2, (int)n3);
+    const int64_t hn = (n1 / 2 + 1) * n2 * n3;
+    const int64_t spectrum = 8 * (half ? std::max<int64_t>(2 * hn, nf) : nf);
+    const int64_t meas = std::min<int64_t>(nf, 2 * pts);
+    const int64_t iter = nf * (4 + 1 + 4 + 4 + 8) + spectrum + meas * 20 + n1 * n2 * 12 + box * 8 +
+                         (int64_t)P * 4 * 8 * (1 + p.numIterations / std::max(1, p.monitorR_loopLength));
+    return inputs + std::max(grid, iter) + (64ll << 20);
+}
+
+}  // namespace gf

@@ -176,6 +176,66 @@ int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2
 int genfire_fftn(genfire_ctx* ctx, float* data, int32_t n1, int32_t n2, int32_t n3,
                  int32_t inverse);
 
+/* ------------------------------------------------------------------------
+ * Request server.  No counterpart in the reference, which reconstructs one
+ * object per MATLAB session (Genfire_test.m:41-44); this is the serving form of
+ * the same job for callers that submit many reconstructions: requests are
+ * admitted while a slot and their HBM footprint are free, prepared (upload,
+ * gridding, set-up) on worker threads, and iterated by one scheduler thread
+ * that launches the transform passes of all iterating jobs of one grid shape
+ * together.  Each result equals the one genfire_job_* gives for the same
+ * inputs; errR and the R-factor records follow genfire_job_get_errR.
+ * ------------------------------------------------------------------------ */
+typedef struct genfire_server genfire_server;
+
+typedef struct {
+    int64_t hbm_budget_bytes;   /* device bytes for resident jobs; <= 0: 90% of the free memory */
+    int32_t max_active;         /* jobs resident at once (slots), 1..64                      */
+    int32_t max_batch;          /* jobs per batched launch, 1..8                              */
+    int32_t lookahead;          /* batched iterations queued ahead of the GPU per shape, 1..16 */
+    int32_t prep_threads;       /* worker threads preparing jobs, 1..8                        */
+} genfire_server_config;
+
+#define GENFIRE_REQ_QUEUED 0
+#define GENFIRE_REQ_PREPARING 1
+#define GENFIRE_REQ_ITERATING 2
+#define GENFIRE_REQ_FINISHING 3
+#define GENFIRE_REQ_DONE 4
+#define GENFIRE_REQ_FAILED 5
+
+typedef struct {
+    int64_t submitted, completed, failed, queued, active;
+    int64_t batched_launches;      /* scheduler iterations launched (all groups)       */
+    int64_t batched_jobs;          /* job-iterations they carried                      */
+    int64_t peak_footprint_bytes;  /* largest sum of admitted footprints               */
+    int64_t hbm_budget_bytes;
+    int32_t max_active_seen;
+    int32_t pad_;
+    double p50_ms, p99_ms, mean_ms; /* submission to result, completed requests        */
+    double jobs_per_hour;           /* completed / (last completion - first submission) */
+} genfire_server_stats;
+
+int genfire_server_config_default(genfire_server_config* cfg);
+int genfire_server_create(genfire_ctx* ctx, const genfire_server_config* cfg, genfire_server** out);
+/* waits for every submitted request, then stops the threads */
+int genfire_server_destroy(genfire_server* srv);
+/* copies the inputs (as genfire_job_set_inputs; support may be NULL) and queues
+ * the request; fails at once on invalid parameters or a footprint above the budget */
+int genfire_server_submit(genfire_server* srv, const genfire_params* params, int32_t dim1, int32_t dim2,
+                          int32_t num_projs, const float* projections, const double* angles,
+                          const uint8_t* support, int64_t* request_id);
+int genfire_server_status(genfire_server* srv, int64_t request_id, int32_t* state);
+int genfire_server_wait(genfire_server* srv, int64_t request_id);
+int genfire_server_drain(genfire_server* srv);
+/* reconstruction Dim1 x Dim2 x Dim1, errK[numIterations], errR[numIterations /
+ * monitorR_loopLength] when monitor_R; latency_ms = {queued, admitted to done,
+ * submission to done}; any output may be NULL */
+int genfire_server_result(genfire_server* srv, int64_t request_id, float* reconstruction, float* errK,
+                          float* errR, double latency_ms[3]);
+/* forgets a finished request */
+int genfire_server_release(genfire_server* srv, int64_t request_id);
+int genfire_server_get_stats(genfire_server* srv, genfire_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,132 @@
+"""The request server (genfire_server_*): continuous batching of reconstruction
+jobs on one GPU.
+
+Bar: every request's reconstruction, errK and errR equal those of the same job
+run alone through genfire_job_* (bit-exact: batching shares launches, not
+arithmetic); admission respects the HBM budget and the slot count; invalid
+requests fail at submission with the reference's message."""
+import numpy as np
+import pytest
+
+from genfire_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _phantom(dim, P, seed, single_axis):
+    rng = np.random.default_rng(seed)
+    k = np.arange(dim) - dim // 2
+    vol = np.zeros((dim, dim, dim), np.float32)
+    for _ in range(max(4, dim ** 3 // 800)):
+        c = rng.uniform(-dim / 3, dim / 3, 3)
+        g = [np.exp(-((k - q) ** 2) / 3.0) for q in c]
+        vol += rng.uniform(0.5, 1.0) * (g[0][:, None, None] * g[1][None, :, None] * g[2][None, None, :])
+    if single_axis:
+        ang = np.stack([np.zeros(P), np.linspace(-70, 70, P), np.zeros(P)], 1)
+    else:
+        ang = np.stack([rng.uniform(-5, 5, P), np.linspace(-70, 70, P), rng.uniform(-5, 5, P)], 1)
+    sup = (k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2) <= (0.42 * dim) ** 2
+    return vol, ang, sup
+
+
+def _params(iters, dr, monitor, prec=_lib.PREC_FP32):
+    p = _lib.default_params()
+    p.numIterations = iters
+    p.oversamplingRatio = 2.0
+    p.interpolationCutoffDistance = 0.3
+    p.constraintEnforcementMode = 3 if dr else 1
+    p.algorithm = _lib.ALG_DR if dr else _lib.ALG_GENFIRE
+    p.monitor_R = 1 if monitor else 0
+    p.monitorR_loopLength = 5
+    p.precision = prec
+    for i in range(3):
+        p.vector3[i] = (0.0, 0.0, 1.0)[i]
+    return p
+
+
+# (dim, P, iterations, DR, monitor, seed): dim 64 -> 128^3 half-spectrum path
+# (three same-shape jobs batch together), dim 32 -> 64^3 and dim 48 -> 96^3 on
+# the full-spectrum path
+MIX = [
+    (64, 21, 30, True, True, 1),
+    (64, 17, 45, True, False, 2),
+    (64, 25, 20, False, True, 3),
+    (32, 13, 25, True, True, 4),
+    (48, 15, 15, False, False, 5),
+    (32, 9, 35, False, True, 6),
+]
+
+
+@pytest.fixture(scope="module")
+def mix_inputs(ctx):
+    out = []
+    for dim, P, iters, dr, mon, seed in MIX:
+        vol, ang, sup = _phantom(dim, P, seed, single_axis=seed % 2 == 0)
+        projs = np.stack([ctx.project(vol, *a) for a in ang], 2).astype(np.float32)
+        out.append((projs, ang, sup, _params(iters, dr, mon)))
+    return out
+
+
+def _alone(ctx, projs, ang, sup, p):
+    d1, d2, P = projs.shape
+    job = _lib.Job(ctx, p, d1, d2, P)
+    job.set_inputs(projs, ang, sup)
+    job.grid()
+    job.reconstruct()
+    return job.reconstruction(), job.errK(), job.errR()[0]
+
+
+def test_results_same_alone_and_in_mix(ctx, mix_inputs):
+    print("[serve] mix of 6 requests, 4 slots", flush=True)
+    alone = [_alone(ctx, *m) for m in mix_inputs]
+    print("[serve] alone runs done", flush=True)
+    cfg = _lib.default_server_config()
+    cfg.max_active = 4
+    cfg.lookahead = 2
+    srv = _lib.Server(ctx, cfg)
+    ids = [srv.submit(p, projs, ang, sup) for projs, ang, sup, p in mix_inputs]
+    srv.drain()
+    for rid, (rec0, ek0, er0), m in zip(ids, alone, MIX):
+        assert srv.status(rid) == _lib.REQ_DONE, m
+        rec, ek, er, lat = srv.result(rid)
+        print(f"[serve] {m}: latency {lat['total']:.1f} ms", flush=True)
+        np.testing.assert_array_equal(rec, rec0)
+        np.testing.assert_array_equal(ek, ek0)
+        np.testing.assert_array_equal(er, er0)
+    st = srv.stats()
+    print(f"[serve] stats {st}", flush=True)
+    assert st["completed"] == len(MIX) and st["failed"] == 0
+    assert st["max_active_seen"] <= 4
+    # the three 128^3 jobs iterate together at least part of the time
+    assert st["batched_jobs"] > st["batched_launches"]
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()
+
+
+def test_budget_admits_one_at_a_time(ctx, mix_inputs):
+    print("[serve] budget for one job", flush=True)
+    cfg = _lib.default_server_config()
+    cfg.max_active = 8
+    cfg.hbm_budget_bytes = 200_000_000   # fits one 128^3 job (about 150 MB estimated), not two
+    srv = _lib.Server(ctx, cfg)
+    ids = [srv.submit(m[3], *m[:3]) for m in mix_inputs[:3]]
+    for rid in ids:
+        srv.wait(rid)
+    st = srv.stats()
+    assert st["completed"] == 3
+    assert st["max_active_seen"] == 1, st
+    assert st["peak_footprint_bytes"] <= cfg.hbm_budget_bytes
+    srv.close()
+
+
+def test_invalid_request_fails_at_submission(ctx, mix_inputs):
+    print("[serve] invalid request", flush=True)
+    srv = _lib.Server(ctx)
+    projs, ang, sup, p = mix_inputs[0]
+    p.constraintEnforcementMode = 4
+    with pytest.raises(_lib.GenfireError, match="constraintEnforcementMode value 4 not understood"):
+        srv.submit(p, projs, ang, sup)
+    p.constraintEnforcementMode = 3
+    st = srv.stats()
+    assert st["submitted"] == 0
+    srv.close()
