@@ -123,6 +123,99 @@ def grid_value_error(job, projs, idx, npts=2000):
     return (num / den) ** 0.5
 
 
+def serve_tilt(ctx, p, w, projs, ang, sup, njobs, precision, max_active):
+    """The served form on the same workload: njobs identical requests submitted at
+    once to the request server (continuous batching, same-shape iterations in
+    shared launches); throughput is njobs over the time from the first
+    submission to the last result."""
+    from genfire_amd import _lib
+    cfg = _lib.default_server_config()
+    cfg.max_active = max_active
+    srv = _lib.Server(ctx, cfg)
+    p.precision = precision
+    req = _lib.Server.request_inputs(projs, ang, sup)
+    # one untimed request first (kernel and allocator warm-up)
+    srv.wait(srv.submit(p, req))
+    srv.close()
+    srv = _lib.Server(ctx, cfg)
+    t0 = time.perf_counter()
+    ids = [srv.submit(p, req) for _ in range(njobs)]
+    srv.drain()
+    span = time.perf_counter() - t0
+    st = srv.stats()
+    res = [srv.result(i) for i in ids]
+    errR = [r[2] for r in res]
+    srv.close()
+    return {
+        "jobs": njobs, "max_active": max_active, "recon_per_hour": njobs / span * 3600.0,
+        "makespan_s": span, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+        "batched_launches": st["batched_launches"], "batched_jobs": st["batched_jobs"],
+        "peak_footprint_gb": st["peak_footprint_bytes"] / 1e9,
+        "rfactor_final": float(errR[0][-1]) if len(errR[0]) else None,
+        "max_active_seen": st["max_active_seen"],
+        "phases_ms": [{k: round(v, 1) for k, v in r[3].items()} for r in res],
+    }
+
+
+def serve_mix(ctx, nreq, rate, seed, max_active):
+    """The serving mix: a seeded Poisson stream of requests of uneven size
+    (phantom.serving_mix) submitted at their arrival times; reports jobs/hour
+    and p50/p99 time to reconstruction."""
+    from genfire_amd import _lib, phantom
+    reqs = phantom.serving_mix(nreq, rate, seed)
+    pm = phantom.gpu_project_many(ctx)
+    t0 = time.perf_counter()
+    vols = {}
+    inputs = []
+    for r in reqs:
+        w = phantom.mix_workload(r)
+        if r.dim not in vols:   # one phantom per size; tilts, noise and counts differ per request
+            rng = np.random.Generator(np.random.PCG64(1000 + r.dim))
+            vols[r.dim] = phantom.blob_phantom(r.dim, w.n_blobs, w.sigma, w.radius, rng)
+        rng = np.random.Generator(np.random.PCG64(r.seed))
+        a = phantom.tilt_angles(w, rng)
+        pj = phantom.add_noise(np.asarray(pm(vols[r.dim], a), np.float32), w.noise, rng)
+        prm = _lib.default_params()
+        prm.numIterations = w.iterations
+        prm.oversamplingRatio = w.oversampling
+        prm.interpolationCutoffDistance = w.cutoff
+        prm.constraintEnforcementMode = w.mode
+        prm.algorithm = _lib.ALG_DR
+        for i, v in enumerate(w.vectors):
+            for k in range(3):
+                getattr(prm, f"vector{i + 1}")[k] = v[k]
+        inputs.append((prm, _lib.Server.request_inputs(pj, a, phantom.spherical_support(r.dim, w.support_radius))))
+    log(f"serving mix: {nreq} requests prepared in {time.perf_counter() - t0:.1f} s")
+    cfg = _lib.default_server_config()
+    cfg.max_active = max_active
+    srv = _lib.Server(ctx, cfg)
+    t0 = time.perf_counter()
+    for r, (prm, req) in zip(reqs, inputs):
+        dt = r.arrival_s - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        srv.submit(prm, req)
+    srv.drain()
+    span = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    sizes = {}
+    for r in reqs:
+        sizes[r.dim] = sizes.get(r.dim, 0) + 1
+    return {
+        "metric": "serving mix: jobs/hour with p50/p99 time to reconstruction",
+        "value": st["jobs_per_hour"], "unit": "jobs/hour", "n_gpus": 1, "higher_is_better": True,
+        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+        "requests": nreq, "arrival_rate_per_s": rate, "span_s": span, "completed": st["completed"],
+        "failed": st["failed"], "max_active": max_active, "max_active_seen": st["max_active_seen"],
+        "batched_launches": st["batched_launches"], "batched_jobs": st["batched_jobs"],
+        "peak_footprint_gb": st["peak_footprint_bytes"] / 1e9, "sizes": sizes,
+        "data": "synthetic (one seeded blob phantom per size, per-request tilts, counts and Poisson noise)",
+        "config": {"workload": "serving mix", "sizes": list(phantom.MIX_SIZES), "projections": [31, 91],
+                   "iterations": [50, 200], "algorithm": "dr", "oversampling": 2.0, "seed": seed},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +229,12 @@ def main():
                     help="override the workload's iteration count (short profiling runs only)")
     ap.add_argument("--precision", choices=["fp32", "split_bf16", "fp16"], default="fp32",
                     help="gridding GEMM precision (FFT and constraint steps stay FP32)")
+    ap.add_argument("--served-jobs", type=int, default=6,
+                    help="requests of the served form on the same workload (0: skip)")
+    ap.add_argument("--serve-mix", type=int, default=0,
+                    help="run only the serving mix with this many requests and print its line")
+    ap.add_argument("--mix-rate", type=float, default=4.0)
+    ap.add_argument("--max-active", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
@@ -154,6 +253,11 @@ def main():
         torch.cuda.synchronize()
         log("calibration copy: 3 x 1 GiB read + 1 GiB write")
         del src, dst
+
+    if args.serve_mix:
+        ctx = _lib.Context(0)
+        print(json.dumps(serve_mix(ctx, args.serve_mix, args.mix_rate, 7, args.max_active)))
+        return
 
     w = phantom.WORKLOADS[args.workload]
     if args.iterations:
@@ -250,6 +354,12 @@ def main():
                      "kernel": f"k_half_y_fwd<{n}> (y-forward pass), {pass_bytes / 1e9:.3f} GB algorithmic "
                                f"per launch, {pass_ms:.3f} ms mean over {nfp} launches"},
     }
+    if args.served_jobs > 0:
+        out["served"] = serve_tilt(ctx, p, w, projs, ang, sup, args.served_jobs, _lib.PREC_SPLIT_BF16,
+                                   args.max_active)
+        out["served"]["precision"] = "split_bf16"
+        out["served"]["vs_this_line"] = out["served"]["recon_per_hour"] / out["value"]
+        p.precision = prec
     if prec != _lib.PREC_FP32:
         # the same job with FP32 gridding, untimed: volume and R-factor deltas
         p.precision = _lib.PREC_FP32

@@ -404,12 +404,24 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, params: GenfireParams, projections, angles, support=None) -> int:
+    @staticmethod
+    def request_inputs(projections, angles, support=None):
+        """The inputs of a request in the library's memory layout (column-major
+        projections and support, P x 3 angles), for repeated submission."""
         d1, d2, P = projections.shape
         pj = np.ascontiguousarray(np.asarray(projections, np.float32).ravel(order="F"))
         ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(P, 3).ravel(order="F"))
         sup = None if support is None else np.ascontiguousarray(
             (np.asarray(support) != 0).astype(np.uint8).ravel(order="F"))
+        return (d1, d2, P), pj, ang, sup
+
+    def submit(self, params: GenfireParams, projections, angles=None, support=None) -> int:
+        """Queue a request: (projections, angles, support) arrays, or the tuple
+        request_inputs returns as the second argument."""
+        if isinstance(projections, tuple):
+            (d1, d2, P), pj, ang, sup = projections
+        else:
+            (d1, d2, P), pj, ang, sup = self.request_inputs(projections, angles, support)
         rid = ctypes.c_int64()
         self.ctx.check(self.L.genfire_server_submit(self.h, ctypes.byref(params), d1, d2, P,
                                                     _ptr(pj, ctypes.c_float), _ptr(ang, ctypes.c_double),
@@ -431,17 +443,17 @@ class Server:
         self.ctx.check(self.L.genfire_server_drain(self.h), "genfire_server_drain")
 
     def result(self, rid: int):
-        """(reconstruction, errK, errR, latency_ms {queued, running, total})"""
+        """(reconstruction, errK, errR, latency_ms {queued, preparing, running, total})"""
         d1, d2, N, nR = self._shape[rid]
         rec = np.zeros(d1 * d2 * d1, np.float32)
         ek = np.zeros(N, np.float32)
         er = np.zeros(nR, np.float32)
-        lat = np.zeros(3)
+        lat = np.zeros(4)
         self.ctx.check(self.L.genfire_server_result(self.h, int(rid), _ptr(rec, ctypes.c_float),
                                                     _ptr(ek, ctypes.c_float), _ptr(er, ctypes.c_float),
                                                     _ptr(lat, ctypes.c_double)), "genfire_server_result")
         return (rec.reshape(d1, d2, d1, order="F"), ek, er,
-                dict(queued=lat[0], running=lat[1], total=lat[2]))
+                dict(queued=lat[0], preparing=lat[1], running=lat[2], total=lat[3]))
 
     def release(self, rid: int):
         self.ctx.check(self.L.genfire_server_release(self.h, int(rid)), "genfire_server_release")

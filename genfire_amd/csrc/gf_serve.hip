@@ -461,7 +461,7 @@ int genfire_server_drain(genfire_server* s)
 }
 
 int genfire_server_result(genfire_server* s, int64_t id, float* reconstruction, float* errK, float* errR,
-                          double latency_ms[3])
+                          double latency_ms[4])
 {
     if (!s) return GENFIRE_ERR_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
@@ -477,8 +477,9 @@ int genfire_server_result(genfire_server* s, int64_t id, float* reconstruction, 
     if (errR) std::memcpy(errR, r->errR.data(), sizeof(float) * r->errR.size());
     if (latency_ms) {
         latency_ms[0] = (r->t_admit - r->t_submit) * 1e3;   // queued
-        latency_ms[1] = (r->t_done - r->t_admit) * 1e3;     // admitted to done
-        latency_ms[2] = (r->t_done - r->t_submit) * 1e3;    // time to reconstruction
+        latency_ms[1] = (r->t_ready - r->t_admit) * 1e3;    // preparing (upload, gridding, set-up)
+        latency_ms[2] = (r->t_done - r->t_ready) * 1e3;     // iterating and read-back
+        latency_ms[3] = (r->t_done - r->t_submit) * 1e3;    // time to reconstruction
     }
     return GENFIRE_OK;
 }

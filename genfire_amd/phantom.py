@@ -138,6 +138,42 @@ def make_inputs(w: Workload, project_many, seed: int | None = None):
     return vol, projs, ang, sup
 
 
+# The serving mix: a Poisson stream of reconstruction requests of uneven size.
+MIX_SIZES = (64, 96, 128, 192, 256, 320)
+
+
+@dataclass
+class MixRequest:
+    arrival_s: float
+    dim: int
+    num_projs: int
+    three_euler: bool
+    iterations: int
+    seed: int
+
+
+def serving_mix(n_requests: int, rate_per_s: float = 4.0, seed: int = 7):
+    """Seeded request stream: exponential inter-arrival times at rate_per_s; volume
+    N uniform over MIX_SIZES, projections uniform in [31, 91], single-axis or
+    three-Euler tilts within +-70 degrees, 50-200 DR iterations."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for i in range(n_requests):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        out.append(MixRequest(t, int(rng.choice(MIX_SIZES)), int(rng.integers(31, 92)), bool(rng.integers(0, 2)),
+                              int(rng.integers(50, 201)), seed * 1000 + i))
+    return out
+
+
+def mix_workload(r: MixRequest) -> Workload:
+    """Workload of one serving-mix request: 1 blob per 800 voxels (sigma 1.5)
+    inside a sphere of radius 0.39 N, support radius 0.43 N, Poisson noise."""
+    N = r.dim
+    return Workload(f"mix{N}", N, max(1, N ** 3 // 800), (1.5, 1.5), 0.39 * N, r.num_projs, r.three_euler, 70.0,
+                    "poisson1e4", 2.0, 0.3, r.iterations, "dr", 0.43 * N, seed=r.seed)
+
+
 def gpu_project_many(ctx):
     """project_many for make_inputs using the library's batched projector."""
     def f(vol, ang):

@@ -228,10 +228,10 @@ int genfire_server_status(genfire_server* srv, int64_t request_id, int32_t* stat
 int genfire_server_wait(genfire_server* srv, int64_t request_id);
 int genfire_server_drain(genfire_server* srv);
 /* reconstruction Dim1 x Dim2 x Dim1, errK[numIterations], errR[numIterations /
- * monitorR_loopLength] when monitor_R; latency_ms = {queued, admitted to done,
- * submission to done}; any output may be NULL */
+ * monitorR_loopLength] when monitor_R; latency_ms = {queued, preparing,
+ * iterating and read-back, submission to result}; any output may be NULL */
 int genfire_server_result(genfire_server* srv, int64_t request_id, float* reconstruction, float* errK,
-                          float* errR, double latency_ms[3]);
+                          float* errR, double latency_ms[4]);
 /* forgets a finished request */
 int genfire_server_release(genfire_server* srv, int64_t request_id);
 int genfire_server_get_stats(genfire_server* srv, genfire_server_stats* out);
