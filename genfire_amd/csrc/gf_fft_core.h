@@ -1,6 +1,6 @@
 // gf_fft_core.h -- device building blocks of the FFT kernels: complex helpers,
-// radix-2/3/4/5/8/16 butterflies and the register-resident Stockham plan for
-// power-of-two lengths 64..1024.
+// radix-2/3/4/5/6/8/10/16 butterflies and the register-resident Stockham plans
+// for lengths 64..1024.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -102,6 +102,57 @@ template <bool INV> struct Dft<5, INV> {
     }
 };
 
+// Composite radices as P x Q (input n = Q*a + b, output k = c + P*d):
+// X[c + P d] = sum_b W_Q^(bd) [W_N^(bc) sum_a x[Q a + b] W_P^(ac)].
+template <int P, int Q, bool INV>
+__device__ __forceinline__ void dft_pq(float2* v, const float2 (&w)[P][Q])
+{
+    float2 t[P][Q];
+#pragma unroll
+    for (int b = 0; b < Q; ++b) {
+        float2 q[P];
+#pragma unroll
+        for (int a = 0; a < P; ++a) q[a] = v[Q * a + b];
+        Dft<P, INV>::run(q);
+#pragma unroll
+        for (int c = 0; c < P; ++c) t[c][b] = (c == 0 || b == 0) ? q[c] : cmul(q[c], w[c][b]);
+    }
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+        float2 q[Q];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) q[b] = t[c][b];
+        Dft<Q, INV>::run(q);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) v[c + P * d] = q[d];
+    }
+}
+
+// 6 = 2 x 3, twiddles W_6^b (b = 1, 2) = cos(pi b/3) -+ i sin(pi b/3)
+template <bool INV> struct Dft<6, INV> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        const float s3 = 0.86602540378443864676f;
+        const float sg = INV ? 1.0f : -1.0f;
+        const float2 w[2][3] = {{{1.0f, 0.0f}, {1.0f, 0.0f}, {1.0f, 0.0f}},
+                                {{1.0f, 0.0f}, {0.5f, sg * s3}, {-0.5f, sg * s3}}};
+        dft_pq<2, 3, INV>(v, w);
+    }
+};
+
+// 10 = 2 x 5, twiddles W_10^b = cos(pi b/5) -+ i sin(pi b/5), b = 1..4
+template <bool INV> struct Dft<10, INV> {
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        const float c1 = 0.80901699437494742410f, s1 = 0.58778525229247312917f;
+        const float c2 = 0.30901699437494742410f, s2 = 0.95105651629515357212f;
+        const float sg = INV ? 1.0f : -1.0f;
+        const float2 w[2][5] = {{{1.0f, 0.0f}, {1.0f, 0.0f}, {1.0f, 0.0f}, {1.0f, 0.0f}, {1.0f, 0.0f}},
+                                {{1.0f, 0.0f}, {c1, sg * s1}, {c2, sg * s2}, {-c2, sg * s2}, {-c1, sg * s1}}};
+        dft_pq<2, 5, INV>(v, w);
+    }
+};
+
 constexpr int pick_radix(int rem)
 {
     return rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : rem % 5 == 0 ? 5 : rem;
@@ -151,7 +202,7 @@ __device__ __forceinline__ int swz(int col, int p) { return col ^ (((col >> 3) ^
 // natural-order spectrum of the tile's z-pencils [x0, x0+valid) at row y.  The
 // measured voxels come from the per-pencil list (meta = z | x<<11 | enforced<<31).
 // ============================================================================
-// Register kernel (power-of-two lengths 64..1024).  A pencil is owned by
+// Register kernel (lengths 64..1024 with factors 2, 3, 5 from the plans below).  A pencil is owned by
 // T = N/E lanes holding E elements each; every radix stage is computed in
 // registers and only the two exchanges between the three stages go through
 // LDS.  The first stage's inputs and the last stage's outputs are the elements
@@ -161,14 +212,27 @@ __device__ __forceinline__ int swz(int col, int p) { return col ^ (((col >> 3) ^
 // (TP * 8 bytes contiguous per row).
 // ============================================================================
 template <int N> struct RegPlan { static constexpr bool ok = false; };
-template <> struct RegPlan<64> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1, TP = 32; };
-template <> struct RegPlan<128> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 2, TP = 16; };
-template <> struct RegPlan<256> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 4, TP = 16; };
-template <> struct RegPlan<512> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 8, TP = 16; };
-template <> struct RegPlan<1024> { static constexpr bool ok = true; static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 4, TP = 8; };
+// Power-of-two lengths: radix 8/16 stages.  Lengths with a factor 3 or 5: the
+// odd (or 6/10) radix last, so that every earlier stage's span LS stays a power
+// of two, and E = lcm of the radices.
+template <> struct RegPlan<64> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 1, R3 = 1, TP = 32; };
+template <> struct RegPlan<96> { static constexpr bool ok = true; static constexpr int E = 12, R0 = 4, R1 = 4, R2 = 6, R3 = 1, TP = 16; };
+template <> struct RegPlan<128> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 2, R3 = 1, TP = 16; };
+template <> struct RegPlan<160> { static constexpr bool ok = true; static constexpr int E = 20, R0 = 4, R1 = 4, R2 = 10, R3 = 1, TP = 16; };
+template <> struct RegPlan<192> { static constexpr bool ok = true; static constexpr int E = 12, R0 = 4, R1 = 4, R2 = 4, R3 = 3, TP = 16; };
+template <> struct RegPlan<256> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 4, R3 = 1, TP = 16; };
+template <> struct RegPlan<320> { static constexpr bool ok = true; static constexpr int E = 20, R0 = 4, R1 = 4, R2 = 4, R3 = 5, TP = 16; };
+template <> struct RegPlan<384> { static constexpr bool ok = true; static constexpr int E = 12, R0 = 4, R1 = 4, R2 = 4, R3 = 6, TP = 16; };
+template <> struct RegPlan<512> { static constexpr bool ok = true; static constexpr int E = 8, R0 = 8, R1 = 8, R2 = 8, R3 = 1, TP = 16; };
+template <> struct RegPlan<640> { static constexpr bool ok = true; static constexpr int E = 20, R0 = 4, R1 = 4, R2 = 4, R3 = 10, TP = 8; };
+template <> struct RegPlan<768> { static constexpr bool ok = true; static constexpr int E = 24, R0 = 8, R1 = 8, R2 = 4, R3 = 3, TP = 8; };
+template <> struct RegPlan<1024> { static constexpr bool ok = true; static constexpr int E = 16, R0 = 16, R1 = 16, R2 = 4, R3 = 1, TP = 8; };
 
 template <int N> constexpr int reg_threads() { return RegPlan<N>::TP * (N / RegPlan<N>::E); }
-template <int N> constexpr int reg_last_radix() { return RegPlan<N>::R2 > 1 ? RegPlan<N>::R2 : RegPlan<N>::R1; }
+template <int N> constexpr int reg_last_radix()
+{
+    return RegPlan<N>::R3 > 1 ? RegPlan<N>::R3 : RegPlan<N>::R2 > 1 ? RegPlan<N>::R2 : RegPlan<N>::R1;
+}
 
 template <int N, int E, int R, int LS, bool INV>
 __device__ __forceinline__ void reg_stage(float2 (&v)[E], int j, const float2* __restrict__ tw)
@@ -217,7 +281,11 @@ template <int N, bool INV>
 __device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row, int j, int p, const float2* tw)
 {
     using PL = RegPlan<N>;
-    constexpr int E = PL::E, R0 = PL::R0, R1 = PL::R1, R2 = PL::R2;
+    constexpr int E = PL::E, R0 = PL::R0, R1 = PL::R1, R2 = PL::R2, R3 = PL::R3;
+    static_assert(R0 * R1 * R2 * R3 == N && E % R0 == 0 && E % R1 == 0 && E % R2 == 0 && E % R3 == 0,
+                  "register FFT plan");
+    static_assert(((R0 * R1) & (R0 * R1 - 1)) == 0 && (R3 == 1 || ((R0 * R1 * R2) & (R0 * R1 * R2 - 1)) == 0),
+                  "the span of every stage but the last must be a power of two");
     reg_stage<N, E, R0, 1, INV>(v, j, tw);
     reg_write<N, E, R0, 1>(row, v, j, p);
     __syncthreads();
@@ -230,6 +298,13 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row,
         reg_read<N, E, R2>(row, v, j, p);
         __syncthreads();
         reg_stage<N, E, R2, R0 * R1, INV>(v, j, tw);
+    }
+    if constexpr (R3 > 1) {
+        reg_write<N, E, R2, R0 * R1>(row, v, j, p);
+        __syncthreads();
+        reg_read<N, E, R3>(row, v, j, p);
+        __syncthreads();
+        reg_stage<N, E, R3, R0 * R1 * R2, INV>(v, j, tw);
     }
 }
 

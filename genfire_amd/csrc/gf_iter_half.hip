@@ -103,8 +103,8 @@ __device__ __forceinline__ void r2c_store(const HalfIterArgs& a, float2* lds, fl
     const float2* rp = lds + pp * M;
     float2* out = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
     auto split = [&](int kx) {
-        const float2 zk = rp[swz(kx & (M - 1), pp)];
-        const float2 zm = conjf2(rp[swz((M - kx) & (M - 1), pp)]);
+        const float2 zk = rp[swz(kx == M ? 0 : kx, pp)];          // Z[k mod M]
+        const float2 zm = conjf2(rp[swz(kx == 0 ? 0 : M - kx, pp)]);  // conj Z[(M - k) mod M]
         const float2 ev = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
         const float2 od = cscale(mul_mi<false>(csub(zk, zm)), 0.5f);
         return cadd(ev, cmul(twN[kx], od));
@@ -668,8 +668,15 @@ template <int N> void launch_z(const HalfBatch& b, hipStream_t st)
 
 bool half_supported(int n1, int n2, int n3)
 {
-    auto p2 = [](int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; };
-    return p2(n1, 128, 1024) && p2(n2, 64, 1024) && p2(n3, 64, 1024) && n1 <= 2048 && n2 <= 2048;
+    // lengths with a register plan: 64, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024
+    auto reg = [](int n) {
+        switch (n) {
+            case 64: case 96: case 128: case 160: case 192: case 256: case 320: case 384: case 512: case 640:
+            case 768: case 1024: return true;
+            default: return false;
+        }
+    };
+    return n1 % 2 == 0 && n1 >= 128 && reg(n1 / 2) && reg(n2) && reg(n3);
 }
 
 int64_t half_partials(int n1, int n2, int n3) { (void)n3; return (int64_t)(n1 / 2 + 1) * (n2 / kZP); }
@@ -677,18 +684,31 @@ int64_t half_partials(int n1, int n2, int n3) { (void)n3; return (int64_t)(n1 / 
 #define GF_HALF_SWITCH(n, CALL)                                               \
     switch (n) {                                                              \
         case 64: CALL(64); break;                                             \
+        case 96: CALL(96); break;                                             \
         case 128: CALL(128); break;                                           \
+        case 160: CALL(160); break;                                           \
+        case 192: CALL(192); break;                                           \
         case 256: CALL(256); break;                                           \
+        case 320: CALL(320); break;                                           \
+        case 384: CALL(384); break;                                           \
         case 512: CALL(512); break;                                           \
+        case 640: CALL(640); break;                                           \
+        case 768: CALL(768); break;                                           \
         case 1024: CALL(1024); break;                                         \
         default: raise(GENFIRE_ERR_UNSUPPORTED, "half-spectrum length not supported"); \
     }
 
+// x lengths n1: the complex transform runs at n1/2
 #define GF_HALF_SWITCH_X(n, CALL)                                             \
     switch (n) {                                                              \
         case 128: CALL(128); break;                                           \
+        case 192: CALL(192); break;                                           \
         case 256: CALL(256); break;                                           \
+        case 320: CALL(320); break;                                           \
+        case 384: CALL(384); break;                                           \
         case 512: CALL(512); break;                                           \
+        case 640: CALL(640); break;                                           \
+        case 768: CALL(768); break;                                           \
         case 1024: CALL(1024); break;                                         \
         default: raise(GENFIRE_ERR_UNSUPPORTED, "half-spectrum x length not supported"); \
     }

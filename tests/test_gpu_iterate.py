@@ -67,13 +67,16 @@ def test_iteration_matches_oracle(ctx, O, alg, mode):
 
 
 @pytest.mark.parametrize("alg", ["genfire", "dr"])
-@pytest.mark.parametrize("shape", [(64, 64), (64, 32)])
+@pytest.mark.parametrize("shape", [(64, 64, 20), (64, 32, 20), (96, 80, 12), (160, 32, 10), (192, 32, 8),
+                                   (320, 32, 5)])
 def test_half_spectrum_iteration_matches_oracle(ctx, O, alg, shape):
-    """128^3 and 128 x 64 x 128 grids run the half-spectrum (R2C/C2R) passes."""
-    d1, d2 = shape
+    """Grids n1 x n2 x n1 with n = 2 dim run the half-spectrum (R2C/C2R) passes:
+    128^3 and 128 x 64 x 128 (radix 8), 192 x 160 x 192 (x and z with radix 6
+    and 3, y with radix 10), 320 x 64 x 320 (radix 10 / 5), 384 x 64 x 384
+    (radix 3 / 6), 640 x 64 x 640 (radix 5 / 10)."""
+    d1, d2, iters = shape
     print(f"[iter] half spectrum alg={alg} dims={shape}", flush=True)
     vol, projs, ang, mk, sup = _setup(O, dim=d1, dim2=d2, P=13, seed=3)
-    iters = 20
     job = _job(ctx, mk, projs, ang, sup, iters, mode=1, alg=alg, monitor=1, L=10)
     ref = O.reconstruct(mk, mk != 0, sup, d1, d2, iters, mode=1, algorithm=alg, precision="fp64",
                         monitor_R=True, loop_length=10, projections=projs, angles=ang)
