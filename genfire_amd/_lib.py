@@ -34,7 +34,7 @@ EXPORTS = [
     "genfire_job_set_measuredK", "genfire_job_reconstruct", "genfire_job_get_reconstruction",
     "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_submit", "genfire_job_poll",
     "genfire_job_wait", "genfire_job_timing", "genfire_job_set_profiling",
-    "genfire_job_kernel_stats", "genfire_project", "genfire_rfactor", "genfire_fftn",
+    "genfire_job_kernel_stats", "genfire_job_graph_replays", "genfire_project", "genfire_rfactor", "genfire_fftn",
     "genfire_server_config_default", "genfire_server_create", "genfire_server_destroy",
     "genfire_server_submit", "genfire_server_status", "genfire_server_wait", "genfire_server_drain",
     "genfire_server_result", "genfire_server_release", "genfire_server_get_stats",
@@ -144,6 +144,7 @@ def load():
         "genfire_job_timing": ([vp, pd], ctypes.c_int),
         "genfire_job_set_profiling": ([vp, i32], ctypes.c_int),
         "genfire_job_kernel_stats": ([vp, i32, pi64, pd], ctypes.c_int),
+        "genfire_job_graph_replays": ([vp, pi64], ctypes.c_int),
         "genfire_project": ([vp, pf, i32, i32, i32, f64, f64, f64, pf], ctypes.c_int),
         "genfire_rfactor": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pf], ctypes.c_int),
         "genfire_fftn": ([vp, pf, i32, i32, i32, i32], ctypes.c_int),
@@ -366,6 +367,11 @@ class Job:
 
     def set_profiling(self, on: bool):
         self._c(self.L.genfire_job_set_profiling(self.h, 1 if on else 0), "genfire_job_set_profiling")
+
+    def graph_replays(self) -> int:
+        n = ctypes.c_int64()
+        self._c(self.L.genfire_job_graph_replays(self.h, ctypes.byref(n)), "genfire_job_graph_replays")
+        return n.value
 
     def kernel_stats(self, k: int):
         n = ctypes.c_int64()

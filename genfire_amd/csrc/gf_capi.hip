@@ -304,6 +304,13 @@ int genfire_job_timing(const genfire_job* j, double ms[4])
     return GENFIRE_OK;
 }
 
+int genfire_job_graph_replays(const genfire_job* j, int64_t* replays)
+{
+    if (!j || !replays) return GENFIRE_ERR_ARG;
+    *replays = j->rs.graph_launches;
+    return GENFIRE_OK;
+}
+
 int genfire_job_set_profiling(genfire_job* j, int32_t on)
 {
     if (!j) return GENFIRE_ERR_ARG;

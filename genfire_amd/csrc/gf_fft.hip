@@ -349,7 +349,9 @@ void launch(int n, const PassArgs& a, hipStream_t st)
 int axis_len(const IterArgs& it, int axis) { return axis == 0 ? it.n1 : (axis == 1 ? it.n2 : it.n3); }
 
 // errK[it] = sum(part) / den, one workgroup, fixed summation order.
-__global__ __launch_bounds__(1024) void k_err_reduce(const double* part, int64_t n, double den, float* out)
+// out[0], or out[*ctr] with *ctr advanced by one when ctr is set (the iteration
+// index kept on the device, so that one captured launch serves every iteration)
+__global__ __launch_bounds__(1024) void k_err_reduce(const double* part, int64_t n, double den, float* out, int* ctr)
 {
     __shared__ double s[1024];
     double t = 0.0;
@@ -360,7 +362,14 @@ __global__ __launch_bounds__(1024) void k_err_reduce(const double* part, int64_t
         if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = (float)(s[0] / den);
+    if (threadIdx.x == 0) {
+        if (ctr) {
+            out[*ctr] = (float)(s[0] / den);
+            *ctr += 1;
+        } else {
+            *out = (float)(s[0] / den);
+        }
+    }
 }
 
 __device__ __forceinline__ double kspace_coord(int c, int n)
@@ -555,7 +564,13 @@ void iter_last(const IterArgs& it, hipStream_t st)
 
 void iter_errk_reduce(const double* d_part, int64_t n, double den, float* d_errK_it, hipStream_t st)
 {
-    k_err_reduce<<<1, 1024, 0, st>>>(d_part, n, den, d_errK_it);
+    k_err_reduce<<<1, 1024, 0, st>>>(d_part, n, den, d_errK_it, nullptr);
+    GF_LAUNCH_CHECK();
+}
+
+void iter_errk_reduce_next(const double* d_part, int64_t n, double den, float* d_errK, int* d_ctr, hipStream_t st)
+{
+    k_err_reduce<<<1, 1024, 0, st>>>(d_part, n, den, d_errK, d_ctr);
     GF_LAUNCH_CHECK();
 }
 

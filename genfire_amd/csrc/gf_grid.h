@@ -105,6 +105,8 @@ void iter_fused(const IterArgs& a, hipStream_t st);
 void iter_last(const IterArgs& a, hipStream_t st);
 // errK[it] = sum(partials) / den.
 void iter_errk_reduce(const double* d_part, int64_t n, double den, float* d_errK_it, hipStream_t st);
+// errK[*ctr] = sum(part) / den, then ++*ctr (device-side iteration index)
+void iter_errk_reduce_next(const double* d_part, int64_t n, double den, float* d_errK, int* d_ctr, hipStream_t st);
 // measured-voxel lists: per-pencil counts, fill (after an exclusive scan of the
 // counts), enforcement bits for a schedule stage ((1-Q) > delay, reconstruct.m:42,120)
 // and the errK denominator sum |M(errInd)|.

@@ -70,6 +70,22 @@ struct ReconState {
     EventTimer all, loop;
     std::vector<std::unique_ptr<EventTimer>> rf;   // one pair per R-factor evaluation
     bool begun = false;
+    // one captured steady-state iteration (no schedule stage, no monitor, no
+    // first or last pass), replayed for every such iteration
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_launches = 0;
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+    ReconState() = default;
+    ReconState(const ReconState&) = delete;
+    ReconState& operator=(const ReconState&) = delete;
+    ~ReconState() { drop_graph(); }
 };
 
 }  // namespace gf
@@ -97,7 +113,7 @@ struct genfire_job {
     bool gridded = false, have_mask = false;
 
     // iteration
-    gf::DevBuf x, K, S, best, pfbuf, err_part, errK, den_tmp, den, recon;
+    gf::DevBuf x, K, S, best, pfbuf, err_part, errK, errctr, den_tmp, den, recon;
     gf::DevBuf csr_cnt, csr_off, csr_meta, csr_val, csr_mag;
     int64_t nmeas = 0;
     gf::DevBuf vol, vol_y, Rmat, rsums, rout;

@@ -384,6 +384,8 @@ void recon_begin(genfire_job* j)
     r.nparts = half ? half_partials(n1, n2, n3) : iter_fused_partials(ia);
     j->err_part.alloc(sizeof(double) * r.nparts);
     j->errK.alloc(sizeof(float) * std::max(N, 1));
+    j->errctr.alloc(sizeof(int));
+    GF_HIP(hipMemsetAsync(j->errctr.p, 0, sizeof(int), st));
     const int ntmp = 1024;
     j->den_tmp.alloc(sizeof(double) * ntmp);
     j->den.alloc(sizeof(double));
@@ -568,7 +570,8 @@ void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStr
 void recon_post(genfire_job* j, int it)
 {
     ReconState& r = j->rs;
-    iter_errk_reduce(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>() + (it - 1), j->run_st);
+    iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(), j->errctr.as<int>(),
+                          j->run_st);
     if (r.monitor && (it % r.L == 0) && it / r.L - 1 < r.nR) {
         const float* model = (r.dr || r.half) ? j->pfbuf.as<float>() : j->x.as<float>();
         run_rfactor(j, model, it / r.L - 1);
@@ -617,16 +620,42 @@ void recon_end(genfire_job* j)
     r.begun = false;
 }
 
+// Steady-state iterations replay one captured HIP graph (the transform passes
+// and the errK reduction, whose output slot is the device-side iteration
+// index) unless per-kernel profiling is on or GENFIRE_GRAPHS=0.
 void do_reconstruct(genfire_job* j)
 {
     j->run_st = j->st;
     recon_begin(j);
+    ReconState& r = j->rs;
+    r.drop_graph();
+    r.graph_launches = 0;
+    const char* ge = std::getenv("GENFIRE_GRAPHS");
+    const bool graphs = !j->prof && !(ge && std::string(ge) == "0");
+    hipStream_t st = j->run_st;
     std::pair<hipEvent_t, hipEvent_t> e_it;
-    for (int it = 1; it <= j->rs.N; ++it) {
+    for (int it = 1; it <= r.N; ++it) {
+        const bool stage = r.cur < r.sched.size() && it == r.sched[r.cur];
+        const bool mon_it = r.monitor && (it % r.L == 0);
+        const bool steady = graphs && it > 1 && it + 1 < r.N && !stage && !mon_it;
         kstat_begin(j, GENFIRE_KSTAT_ITERATION, &e_it);
-        recon_pre(j, it);
-        recon_body(j, it);
-        iter_errk_reduce(j->err_part.as<double>(), j->rs.nparts, j->rs.den, j->errK.as<float>() + (it - 1), j->run_st);
+        if (steady) {
+            if (!r.gexec) {
+                GF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                recon_body(j, it);
+                iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(),
+                                      j->errctr.as<int>(), st);
+                GF_HIP(hipStreamEndCapture(st, &r.graph));
+                GF_HIP(hipGraphInstantiate(&r.gexec, r.graph, nullptr, nullptr, 0));
+            }
+            GF_HIP(hipGraphLaunch(r.gexec, st));
+            ++r.graph_launches;
+        } else {
+            recon_pre(j, it);
+            recon_body(j, it);
+            iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(),
+                                  j->errctr.as<int>(), st);
+        }
         kstat_end(j, GENFIRE_KSTAT_ITERATION, e_it);
         if (j->rs.monitor && (it % j->rs.L == 0) && it / j->rs.L - 1 < j->rs.nR) {
             const float* model = (j->rs.dr || j->rs.half) ? j->pfbuf.as<float>() : j->x.as<float>();
@@ -634,6 +663,7 @@ void do_reconstruct(genfire_job* j)
         }
     }
     recon_end(j);
+    r.drop_graph();
 }
 
 void release_grid_extras(genfire_job* j)

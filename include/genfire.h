@@ -153,6 +153,10 @@ int genfire_job_timing(const genfire_job* job, double ms[4]);
 #define GENFIRE_KSTAT_PROJECT 4       /* projector + R sums, all projections      */
 #define GENFIRE_KSTAT_COUNT 5
 int genfire_job_set_profiling(genfire_job* job, int32_t on);
+/* Iterations of the last genfire_job_reconstruct that replayed the captured
+ * HIP graph of a steady-state iteration (no enforcement stage, no R-factor
+ * monitor, not the first or last two); 0 with profiling on or GENFIRE_GRAPHS=0. */
+int genfire_job_graph_replays(const genfire_job* job, int64_t* replays);
 int genfire_job_kernel_stats(const genfire_job* job, int32_t kernel, int64_t* launches,
                              double* total_ms);
 
