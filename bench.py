@@ -58,7 +58,7 @@ def cpu_baseline(w, projs, ang, sup, mk_centred, model, n_cores):
         O.dft64(projs[:, :, p], un, vn)
         npts += len(ind)
     t_grid_proj = (time.perf_counter() - t0) / 2
-    # iteration: 2 DR iterations' worth of FFT + constraint + replacement on 512^3
+    # iteration: 2 DR iterations' worth of FFT + constraint + replacement on the n^3 grid
     shape = (n, n, n)
     x = np.zeros(shape, np.float32)
     S = np.fft.ifftshift(O.paddzero(sup.astype(np.float32), shape))
@@ -75,7 +75,7 @@ def cpu_baseline(w, projs, ang, sup, mk_centred, model, n_cores):
         x = x + pf - r
     t_iter = (time.perf_counter() - t0) / 2
     del x, S, M, meas, k, pf, r, y
-    # R-factor: 2 projections of the 256^3 model with the FP64 trilinear projector
+    # R-factor: 2 projections of the Dim^3 model with the FP64 trilinear projector
     t0 = time.perf_counter()
     for p in (0, 1):
         O.project(model, *ang[p])
@@ -87,11 +87,11 @@ def cpu_baseline(w, projs, ang, sup, mk_centred, model, n_cores):
         "unit": "recon/hour",
         "cores": n_cores,
         "kind": "port",
-        "sample": (f"oracle (C/OpenMP + numpy/pocketfft) on 2 of 91 projections gridded "
+        "sample": (f"oracle (C/OpenMP + numpy/pocketfft) on 2 of {w.num_projs} projections gridded "
                    f"({t_grid_proj:.2f} s each, full-grid FP32 index scan + FP64 direct DFT), "
-                   f"2 DR iterations on the 512^3 grid ({t_iter:.2f} s each), 2 of 91 trilinear "
-                   f"projections ({t_proj:.2f} s each); extrapolated linearly to 91 projections, "
-                   f"200 iterations and {n_eval} R-factor evaluations = {total:.0f} s per job"),
+                   f"2 DR iterations on the {n}^3 grid ({t_iter:.2f} s each), 2 of {w.num_projs} trilinear "
+                   f"projections ({t_proj:.2f} s each); extrapolated linearly to {w.num_projs} projections, "
+                   f"{w.iterations} iterations and {n_eval} R-factor evaluations = {total:.0f} s per job"),
     }
 
 
@@ -338,7 +338,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded Gaussian-blob phantom, trilinear projections, Poisson 1e4 counts)",
-        "config": {"workload": "tilt256", "volume": 256, "projections": 91, "projection_size": 256,
+        "config": {"workload": w.name, "volume": w.dim, "projections": w.num_projs, "projection_size": w.dim,
                    "grid": n, "iterations": w.iterations, "algorithm": w.algorithm,
                    "rfactor_every": w.monitor_every, "gridding_precision": args.precision,
                    "cutoff": w.cutoff, "oversampling": w.oversampling},
