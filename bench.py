@@ -318,8 +318,8 @@ def main():
     nfp, msfp = stats[_lib.KSTAT_FFT_PASS]
     nit, msit = stats[_lib.KSTAT_ITERATION]
     pass_ms = msfp / max(nfp, 1)
-    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
-    it_ms = msit / max(nit, 1)
+    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if nfp else float("nan")
+    it_ms = msit / max(nit, 1) if nit else tm["iterate_ms"] / w.iterations
     # algorithmic bytes per iteration with half-spectrum storage:
     # 16 n^3 + 12 N_meas,half + n^3/8, N_meas,half ~ N_meas / 2
     nmeas = int(np.count_nonzero(job.measuredK()))
