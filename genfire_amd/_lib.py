@@ -29,7 +29,7 @@ KSTAT_GRID_VALUES, KSTAT_FFT_PASS, KSTAT_FFT_FUSED, KSTAT_ITERATION, KSTAT_PROJE
 EXPORTS = [
     "genfire_version", "genfire_params_default", "genfire_ctx_create", "genfire_ctx_destroy",
     "genfire_last_error", "genfire_job_create", "genfire_job_destroy", "genfire_job_sizes",
-    "genfire_job_set_inputs", "genfire_job_grid", "genfire_job_grid_count", "genfire_job_get_points",
+    "genfire_job_set_inputs", "genfire_job_set_inputs_f16", "genfire_job_grid", "genfire_job_grid_count", "genfire_job_get_points",
     "genfire_job_get_measuredK", "genfire_job_get_hits", "genfire_job_get_recIFFT",
     "genfire_job_set_measuredK", "genfire_job_reconstruct", "genfire_job_get_reconstruction",
     "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_submit", "genfire_job_poll",
@@ -127,6 +127,7 @@ def load():
         "genfire_job_destroy": ([vp], ctypes.c_int),
         "genfire_job_sizes": ([vp, pi32, pi32, pi64], ctypes.c_int),
         "genfire_job_set_inputs": ([vp, pf, pd, pu8, pf], ctypes.c_int),
+        "genfire_job_set_inputs_f16": ([vp, P(ctypes.c_uint16), pf, pd, pu8, pf], ctypes.c_int),
         "genfire_job_grid": ([vp], ctypes.c_int),
         "genfire_job_grid_count": ([vp, pi64], ctypes.c_int),
         "genfire_job_get_points": ([vp, pi64, pi32, pf, pf, pf, pf], ctypes.c_int),
@@ -282,6 +283,20 @@ class Job:
         self._c(self.L.genfire_job_set_inputs(self.h, _ptr(pj, ctypes.c_float), _ptr(ang, ctypes.c_double),
                                               _ptr(sup, ctypes.c_uint8), _ptr(ini, ctypes.c_float)),
                 "genfire_job_set_inputs")
+
+    def set_inputs_f16(self, projections_f16, scales, angles, support=None, initial_object=None):
+        """Projections as float16 (Dim1 x Dim2 x P) with one float scale per projection."""
+        pj = np.ascontiguousarray(np.asarray(projections_f16, np.float16).ravel(order="F")).view(np.uint16)
+        sc = np.ascontiguousarray(np.asarray(scales, np.float32).reshape(self.P))
+        ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(self.P, 3).ravel(order="F"))
+        sup = None if support is None else np.ascontiguousarray(
+            (np.asarray(support) != 0).astype(np.uint8).ravel(order="F"))
+        ini = None if initial_object is None else np.ascontiguousarray(
+            np.asarray(initial_object, np.float32).ravel(order="F"))
+        self._keep = (pj, sc, ang, sup, ini)
+        self._c(self.L.genfire_job_set_inputs_f16(self.h, _ptr(pj, ctypes.c_uint16), _ptr(sc, ctypes.c_float),
+                                                  _ptr(ang, ctypes.c_double), _ptr(sup, ctypes.c_uint8),
+                                                  _ptr(ini, ctypes.c_float)), "genfire_job_set_inputs_f16")
 
     def grid(self):
         self._c(self.L.genfire_job_grid(self.h), "genfire_job_grid")

@@ -110,6 +110,20 @@ int genfire_job_set_inputs(genfire_job* j, const float* projections, const doubl
     });
 }
 
+int genfire_job_set_inputs_f16(genfire_job* j, const uint16_t* projections_f16, const float* proj_scale,
+                               const double* angles, const uint8_t* support, const float* initial_object)
+{
+    if (!j) return GENFIRE_ERR_ARG;
+    if (!projections_f16 || !proj_scale || !angles) {
+        set_err(j->ctx, "GENFIRE: projections, their scales and angles are required");
+        return GENFIRE_ERR_ARG;
+    }
+    return guarded(j->ctx, [&] {
+        use_device(j->ctx);
+        job_set_inputs(j, nullptr, angles, support, initial_object, projections_f16, proj_scale);
+    });
+}
+
 int genfire_job_grid(genfire_job* j)
 {
     if (!j) return GENFIRE_ERR_ARG;

@@ -57,6 +57,8 @@ constexpr uint32_t kExactBit = 0x80000000u;
 // ---- layout helpers (gf_grid_values.hip) ----
 // Centred <-> FFT-order conversion of an n1 x n2 x n3 complex grid
 // (to_centred = 1: out = fftshift(in), 0: out = ifftshift(in)).
+// out[t] = half(h[t]) * scale[t / per_proj] for P projections of per_proj values
+void expand_f16(const uint16_t* d_h, const float* d_scale, int64_t per_proj, int P, float* d_out, hipStream_t st);
 void shift_complex(const float2* d_in, float2* d_out, int n1, int n2, int n3, int to_centred,
                    hipStream_t st);
 // Real crop of the centred d1 x d2 x d3 box out of an FFT-order n1 x n2 x n3

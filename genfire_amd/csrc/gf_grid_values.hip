@@ -212,6 +212,13 @@ __global__ void k_pad(const T* __restrict__ src, int d1, int d2, int d3, T* __re
     }
 }
 
+__global__ void k_expand_f16(const uint16_t* __restrict__ h, const float* __restrict__ scale, int64_t per_proj,
+                             int64_t total, float* __restrict__ out)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = (float)__builtin_bit_cast(_Float16, h[t]) * scale[t / per_proj];
+}
+
 int grid_blocks(int64_t n, int threads)
 {
     int64_t b = (n + threads - 1) / threads;
@@ -248,6 +255,13 @@ void grid_finalize(const HalfGrid& hg, int n1, int n2, int centro, const double2
     const int64_t total = (int64_t)n1 * n2 * n1;
     k_finalize<<<grid_blocks(total, 256), 256, 0, st>>>(hg, n1, n2, centro, d_acc, d_sw, d_hits,
                                                         d_exact, d_M);
+    GF_LAUNCH_CHECK();
+}
+
+void expand_f16(const uint16_t* d_h, const float* d_scale, int64_t per_proj, int P, float* d_out, hipStream_t st)
+{
+    const int64_t total = per_proj * P;
+    k_expand_f16<<<grid_blocks(total, 256), 256, 0, st>>>(d_h, d_scale, per_proj, total, d_out);
     GF_LAUNCH_CHECK();
 }
 

@@ -171,9 +171,11 @@ inline int64_t box_size(const genfire_job* j) { return (int64_t)j->dim1 * j->dim
 void validate_job(const genfire_params* p, int dim1, int dim2, int P);
 // Validates the parameters and fills sizes, schedule and stream of a fresh job.
 void job_setup(genfire_job* j, genfire_ctx* ctx, const genfire_params* p, int dim1, int dim2, int P);
-// Uploads the inputs (readFiles, :82-112); support and initial object may be null.
+// Uploads the inputs (readFiles, :82-112); support and initial object may be
+// null; projections either FP32, or binary16 with one scale per projection.
 void job_set_inputs(genfire_job* j, const float* projections, const double* angles, const uint8_t* support,
-                    const float* initial_object);
+                    const float* initial_object, const uint16_t* proj_f16 = nullptr,
+                    const float* proj_scale = nullptr);
 // Device bytes a job of these sizes holds at its peak (an upper estimate).
 int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P);
 

@@ -723,12 +723,23 @@ void job_setup(genfire_job* j, genfire_ctx* ctx, const genfire_params* p, int di
 }
 
 void job_set_inputs(genfire_job* j, const float* projections, const double* angles, const uint8_t* support,
-                    const float* initial_object)
+                    const float* initial_object, const uint16_t* proj_f16, const float* proj_scale)
 {
-    GF_CHECK(projections && angles, GENFIRE_ERR_ARG, "GENFIRE: projections and angles are required");
+    GF_CHECK((projections || (proj_f16 && proj_scale)) && angles, GENFIRE_ERR_ARG,
+             "GENFIRE: projections and angles are required");
     const size_t np = (size_t)j->dim1 * j->dim2 * j->P;
     j->proj.alloc(sizeof(float) * np);
-    GF_HIP(hipMemcpyAsync(j->proj.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, j->st));
+    if (projections) {
+        GF_HIP(hipMemcpyAsync(j->proj.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, j->st));
+    } else {
+        DevBuf h, sc;
+        h.alloc(sizeof(uint16_t) * np);
+        sc.alloc(sizeof(float) * j->P);
+        GF_HIP(hipMemcpyAsync(h.p, proj_f16, sizeof(uint16_t) * np, hipMemcpyHostToDevice, j->st));
+        GF_HIP(hipMemcpyAsync(sc.p, proj_scale, sizeof(float) * j->P, hipMemcpyHostToDevice, j->st));
+        expand_f16(h.as<uint16_t>(), sc.as<float>(), (int64_t)j->dim1 * j->dim2, j->P, j->proj.as<float>(), j->st);
+        GF_HIP(hipStreamSynchronize(j->st));
+    }
     j->angles.assign(angles, angles + 3 * (size_t)j->P);
     j->have_support = support != nullptr;
     if (support) {

@@ -12,6 +12,10 @@ Workloads (sizes fixed here, every one seeded with numpy PCG64):
   pixel; oversampling 2, cutoff 0.3, 200 DR iterations, R-factor every 10.
 * ``tilt512`` -- as tilt256 with a 512^3 phantom of 40000 blobs (radius 200) and
   512 x 512 projections.
+* ``tilt512x121`` -- 512^3 phantom of 60000 blobs (radius 200), 121 three-Euler
+  projections 512 x 512 with theta in +-72 degrees, 300 DR iterations, R-factor
+  every 20 (the projections are meant to be stored as FP16 with a scale per
+  projection).
 
 The tilt series are computed by the library's projector (the trilinear
 calculate3Dprojection_RealSpaceinterp) from the voxelised phantom.
@@ -58,6 +62,8 @@ WORKLOADS = {
                         0.3, 200, "dr", 110.0, monitor_R=1, monitor_every=10, seed=2),
     "tilt512": Workload("tilt512", 512, 40000, (1.5, 1.5), 200.0, 91, True, 70.0, "poisson1e4", 2.0,
                         0.3, 200, "dr", 220.0, monitor_R=1, monitor_every=10, seed=3),
+    "tilt512x121": Workload("tilt512x121", 512, 60000, (1.5, 1.5), 200.0, 121, True, 72.0, "poisson1e4", 2.0,
+                            0.3, 300, "dr", 220.0, monitor_R=1, monitor_every=20, seed=4),
 }
 
 

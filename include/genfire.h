@@ -100,6 +100,14 @@ int genfire_job_sizes(const genfire_job* job, int32_t* n1, int32_t* n2, int64_t 
 int genfire_job_set_inputs(genfire_job* job, const float* projections, const double* angles,
                            const uint8_t* support, const float* initial_object);
 
+/* As genfire_job_set_inputs with the projections stored as IEEE binary16 with
+ * one scale per projection: value(a, b, p) = half(a, b, p) * proj_scale[p].  No
+ * counterpart in the reference (which reads double projections,
+ * GENFIRE_Reconstructor.m:82-112); for callers that keep large tilt series in
+ * half precision.  The projections are expanded to FP32 on the device. */
+int genfire_job_set_inputs_f16(genfire_job* job, const uint16_t* projections_f16, const float* proj_scale,
+                               const double* angles, const uint8_t* support, const float* initial_object);
+
 /* runGridding with griddingMethod = 2 (GENFIRE_Reconstructor.m:172-186):
  * fillInFourierGrid_DFT + CentroSymmetricity + single cast + measuredK_mask. */
 int genfire_job_grid(genfire_job* job);

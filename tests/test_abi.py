@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     src = open(os.path.join(ROOT, "include", "genfire.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(genfire_[A-Za-z_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(genfire_[A-Za-z0-9_]+)\s*\(", src)))
 
 
 def test_header_and_binding_agree():

@@ -165,3 +165,26 @@ def test_reduced_precision_values(ctx, O, name, prec):
     np.testing.assert_array_equal(mka != 0, mkb != 0)
     err = np.linalg.norm((mkb - mka).astype(np.complex128)) / np.linalg.norm(mka.astype(np.complex128))
     assert err < LP_TOL[prec], err
+
+
+def test_fp16_projection_storage(ctx):
+    """genfire_job_set_inputs_f16 grids exactly what FP32 inputs of the expanded
+    values (half * scale, exact in FP32) grid."""
+    print("[grid] fp16 projection storage", flush=True)
+    d1, d2, P = 48, 48, 5
+    rng = np.random.default_rng(13)
+    projs = rng.uniform(0.0, 50.0, (d1, d2, P)).astype(np.float32)
+    scale = np.abs(projs).reshape(-1, P, order="F").max(axis=0).astype(np.float32)
+    h = (projs / scale[None, None, :]).astype(np.float16)
+    expanded = (h.astype(np.float32) * scale[None, None, :]).astype(np.float32)
+    ang = np.stack([np.zeros(P), np.linspace(-60, 60, P), np.zeros(P)], 1)
+    out = []
+    for f16 in (True, False):
+        job = _lib.Job(ctx, _params(ratio=2.0, cutoff=0.3), d1, d2, P)
+        if f16:
+            job.set_inputs_f16(h, scale, ang)
+        else:
+            job.set_inputs(expanded, ang)
+        job.grid()
+        out.append(job.measuredK())
+    np.testing.assert_array_equal(out[0], out[1])
