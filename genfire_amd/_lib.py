@@ -32,7 +32,7 @@ EXPORTS = [
     "genfire_job_set_inputs", "genfire_job_set_inputs_f16", "genfire_job_grid", "genfire_job_grid_count", "genfire_job_get_points",
     "genfire_job_get_measuredK", "genfire_job_get_hits", "genfire_job_get_recIFFT",
     "genfire_job_set_measuredK", "genfire_job_reconstruct", "genfire_job_get_reconstruction",
-    "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_submit", "genfire_job_poll",
+    "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_get_Rfree", "genfire_job_submit", "genfire_job_poll",
     "genfire_job_wait", "genfire_job_timing", "genfire_job_set_profiling",
     "genfire_job_kernel_stats", "genfire_job_graph_replays", "genfire_project", "genfire_rfactor", "genfire_fftn",
     "genfire_server_config_default", "genfire_server_create", "genfire_server_destroy",
@@ -59,6 +59,10 @@ class GenfireParams(ctypes.Structure):
         ("algorithm", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("compute_recIFFT", ctypes.c_int32),
+        ("calculate_Rfree", ctypes.c_int32),
+        ("numBinsRfree", ctypes.c_int32),
+        ("percentValuesForRfree", ctypes.c_double),
+        ("RandomSeed", ctypes.c_int64),
     ]
 
 
@@ -139,6 +143,7 @@ def load():
         "genfire_job_get_reconstruction": ([vp, pf], ctypes.c_int),
         "genfire_job_get_errK": ([vp, pf], ctypes.c_int),
         "genfire_job_get_errR": ([vp, pf, pd, pd], ctypes.c_int),
+        "genfire_job_get_Rfree": ([vp, pf], ctypes.c_int),
         "genfire_job_submit": ([vp], ctypes.c_int),
         "genfire_job_poll": ([vp, pi32], ctypes.c_int),
         "genfire_job_wait": ([vp], ctypes.c_int),
@@ -363,6 +368,13 @@ class Job:
         self._c(self.L.genfire_job_get_errR(self.h, _ptr(e, ctypes.c_float), _ptr(r1, ctypes.c_double),
                                             _ptr(r2, ctypes.c_double)), "genfire_job_get_errR")
         return e, r1.reshape(self.P, nR, order="F"), r2.reshape(self.P, nR, order="F")
+
+    def Rfree(self) -> np.ndarray:
+        """Rfree_complex: numBinsRfree x numIterations (reconstruct.m:137-147)."""
+        nb, N = self.params.numBinsRfree, self.params.numIterations
+        out = np.zeros(nb * N, np.float32)
+        self._c(self.L.genfire_job_get_Rfree(self.h, _ptr(out, ctypes.c_float)), "genfire_job_get_Rfree")
+        return out.reshape(nb, N, order="F")
 
     def submit(self):
         self._c(self.L.genfire_job_submit(self.h), "genfire_job_submit")

@@ -35,6 +35,10 @@ int genfire_params_default(genfire_params* p)
     p->algorithm = GENFIRE_ALG_GENFIRE;
     p->precision = GENFIRE_PREC_FP32;
     p->compute_recIFFT = 1;
+    p->calculate_Rfree = 0;
+    p->numBinsRfree = 10;
+    p->percentValuesForRfree = 0.05;
+    p->RandomSeed = 0;
     return GENFIRE_OK;
 }
 
@@ -271,6 +275,18 @@ int genfire_job_get_errR(genfire_job* j, float* errR, double* Rarr, double* Rarr
     // stored slot-major (slot*P + p): exactly the column-major P x nR layout
     if (Rarr) std::memcpy(Rarr, j->Rarr_h.data(), sizeof(double) * j->Rarr_h.size());
     if (Rarr2) std::memcpy(Rarr2, j->Rarr2_h.data(), sizeof(double) * j->Rarr2_h.size());
+    return GENFIRE_OK;
+}
+
+int genfire_job_get_Rfree(genfire_job* j, float* out)
+{
+    if (!j || !out) return GENFIRE_ERR_ARG;
+    if (!j->reconstructed) return GENFIRE_ERR_STATE;
+    if (j->rf_bins <= 0) {
+        set_err(j->ctx, "GENFIRE: Rfree was not calculated (calculate_Rfree = 0)");
+        return GENFIRE_ERR_STATE;
+    }
+    std::memcpy(out, j->Rfree_h.data(), sizeof(float) * j->Rfree_h.size());
     return GENFIRE_OK;
 }
 

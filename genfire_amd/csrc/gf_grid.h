@@ -135,9 +135,20 @@ struct HalfIterArgs {
     float* best;
     float* pf_out;
     double* err_part;
+    // Rfree (withheld voxels, per half pencil: meta z | ky<<11, value as stored
+    // or conjugated for a mate-stored voxel); null when Rfree is off
+    const int64_t* rf_off;
+    const uint32_t* rf_meta;
+    const float2* rf_val;
+    float* rf_terms;   // |k - value| per entry, written by the z pass
     int n1, n2, n3;
     int positivity, support, dr;
 };
+// Rfree[s + nbins * (*ctr)] = sum of terms of shell s (entries order[shell_off[s]
+// .. shell_off[s+1]), fixed order) / den[s], or -1 for an empty shell; *ctr is
+// the iteration index the errK reduction advances afterwards.
+void rfree_reduce(const float* d_terms, const int32_t* d_order, const int32_t* d_shell_off, const double* d_den,
+                  int nbins, float* d_Rfree, const int* d_ctr, hipStream_t st);
 constexpr uint32_t kHalfTermA = 1u << 28;     // voxel measured: errK term |k| vs mag.x
 constexpr uint32_t kHalfTermB = 1u << 29;     // mate measured: errK term |k| vs mag.y
 constexpr uint32_t kHalfAverage = 1u << 30;   // only one of the pair measured: k = (V + k)/2

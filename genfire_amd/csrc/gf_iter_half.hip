@@ -239,6 +239,19 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
 #pragma unroll
         for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
     __syncthreads();
+    // Rfree terms (reconstruct.m:137-147) on the transform before replacement
+    if (a.rf_off) {
+        const int rb = (int)a.rf_off[r0], re = (int)a.rf_off[r0 + kZP];
+        for (int e = rb + t; e < re; e += NT) {
+            const uint32_t m = a.rf_meta[e];
+            const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
+            const float2 kv = lds[pl * N + swz(z, pl)];
+            const float2 V = a.rf_val[e];
+            const float dx = kv.x - V.x, dy = kv.y - V.y;
+            a.rf_terms[e] = sqrtf(dx * dx + dy * dy);
+        }
+        __syncthreads();
+    }
     // errK numerator (:135) and replacement (:163) from the half-spectrum lists;
     // the numerator is reduced here (LDS past the tile) so that it is not live
     // through the inverse transform
@@ -592,6 +605,25 @@ __global__ __launch_bounds__(256) void k_hcsr_den(const float2* __restrict__ mag
     if (threadIdx.x == 0) part[blockIdx.x] = r[0];
 }
 
+__global__ __launch_bounds__(256) void k_rfree_reduce(const float* __restrict__ terms,
+                                                     const int32_t* __restrict__ order,
+                                                     const int32_t* __restrict__ shell_off,
+                                                     const double* __restrict__ den, int nbins,
+                                                     float* __restrict__ out, const int* __restrict__ ctr)
+{
+    __shared__ double red[256];
+    const int sh = blockIdx.x;
+    double t = 0.0;
+    for (int e = shell_off[sh] + (int)threadIdx.x; e < shell_off[sh + 1]; e += 256) t += (double)terms[order[e]];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[sh + (int64_t)nbins * (*ctr)] = den[sh] > 0.0 ? (float)(red[0] / den[sh]) : -1.0f;
+}
+
 __global__ void k_hsum(const double* part, int n, double* out)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -774,6 +806,13 @@ void half_y(const HalfIterArgs& a, int inverse, hipStream_t st) { half_y(one(a),
 void half_z(const HalfIterArgs& a, hipStream_t st) { half_z(one(a), st); }
 void half_last(const HalfIterArgs& a, hipStream_t st) { half_last(one(a), st); }
 void half_last_first(const HalfIterArgs& a, hipStream_t st) { half_last_first(one(a), st); }
+
+void rfree_reduce(const float* d_terms, const int32_t* d_order, const int32_t* d_shell_off, const double* d_den,
+                  int nbins, float* d_Rfree, const int* d_ctr, hipStream_t st)
+{
+    k_rfree_reduce<<<nbins, 256, 0, st>>>(d_terms, d_order, d_shell_off, d_den, nbins, d_Rfree, d_ctr);
+    GF_LAUNCH_CHECK();
+}
 
 void hcsr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st)
 {

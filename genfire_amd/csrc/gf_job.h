@@ -117,6 +117,10 @@ struct genfire_job {
     gf::DevBuf csr_cnt, csr_off, csr_meta, csr_val, csr_mag;
     int64_t nmeas = 0;
     gf::DevBuf vol, vol_y, Rmat, rsums, rout;
+    // Rfree: withheld voxels per half pencil, their terms, shell order and denominators
+    gf::DevBuf rf_off, rf_meta, rf_val, rf_terms, rf_order, rf_shell_off, rf_den, Rfree;
+    int rf_bins = 0;
+    std::vector<float> Rfree_h;
     bool rmat_any_y = false;
     std::vector<float> errK_h, errR_h;
     std::vector<double> Rarr_h, Rarr2_h;
@@ -200,6 +204,9 @@ void do_reconstruct(genfire_job* j);
 // The passes of iteration its[i] of every job (half-spectrum path, one shape)
 // as batched launches on st.
 void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStream_t st);
+
+// The Rfree reduction of the job's current iteration (no-op without Rfree).
+void rfree_step(genfire_job* j, hipStream_t st);
 
 // Frees what the iterations no longer need (point lists, hits, measuredK).
 void release_grid_extras(genfire_job* j);

@@ -338,6 +338,139 @@ void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
     GF_HIP(hipEventRecord(tm.b, st));
 }
 
+// The build's seeded permutation for the withheld points (the reference's
+// randperm after rng(RandomSeed), reconstruct.m:52-55, is MATLAB's generator):
+// candidates are ordered by splitmix64(seed ^ splitmix64(index)), index the
+// centred column-major linear index, ties by index.
+uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// reconstruct.m:46-69: per resolution shell [edge_s - 0.05, edge_s+1) of
+// Q = make_Kspace_indices, withhold floor(count * percent) (at least one) of
+// the measured voxels in the permuted order from measuredK_mask, and keep their
+// values.  Builds the device mask and the per-half-pencil evaluation lists.
+void rfree_select(genfire_job* j)
+{
+    hipStream_t st = j->run_st;
+    const int n1 = j->n1, n2 = j->n2, n3 = j->n1;
+    const int64_t nf = full_size(j);
+    const int nb = j->prm.numBinsRfree;
+    const double pct = j->prm.percentValuesForRfree;
+    GF_CHECK(nb >= 1 && nb <= 1024, GENFIRE_ERR_ARG, "GENFIRE: numBinsRfree must be 1..1024");
+    GF_CHECK(pct > 0.0 && pct <= 1.0, GENFIRE_ERR_ARG, "GENFIRE: percentValuesForRfree must be in (0, 1]");
+    std::vector<float2> M(nf);
+    std::vector<uint8_t> mask(nf, 1);
+    GF_HIP(hipMemcpyAsync(M.data(), j->M.p, sizeof(float2) * nf, hipMemcpyDeviceToHost, st));
+    if (j->have_mask) GF_HIP(hipMemcpyAsync(mask.data(), j->mask.p, nf, hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    // shell edges: linspace(0, 1, nb + 1) (MATLAB's, endpoint exact)
+    std::vector<double> edge(nb + 1);
+    for (int k = 0; k <= nb; ++k) edge[k] = (k == nb) ? 1.0 : 0.0 + ((double)k * (1.0 - 0.0)) / (double)nb;
+    const double res_range = -0.05;
+    auto coord = [](int c, int n) {   // make_Kspace_indices.m vector entry of centred index c
+        if (n == 1) return 0.0;
+        if (n % 2 == 0) return (double)(c - n / 2) / (double)(n / 2);
+        return (double)(c - (n - 1) / 2) / (double)((n - 1) / 2);
+    };
+    const uint64_t seed = splitmix64((uint64_t)j->prm.RandomSeed);
+    struct Cand {
+        uint64_t key;
+        int64_t c, f;
+    };
+    std::vector<std::vector<Cand>> shells(nb);
+    for (int64_t f = 0; f < nf; ++f) {
+        const float2 m = M[f];
+        if (m.x == 0.0f && m.y == 0.0f) continue;   // measuredK ~= 0 (:50)
+        const int x = (int)(f % n1), y = (int)((f / n1) % n2), z = (int)(f / ((int64_t)n1 * n2));
+        const int c1 = (x + n1 / 2) % n1, c2 = (y + n2 / 2) % n2, c3 = (z + n3 / 2) % n3;   // FFT order -> centred
+        const double v1 = coord(c1, n1), v2 = coord(c2, n2), v3 = coord(c3, n3);
+        const double Q = std::sqrt((v2 * v2 + v1 * v1) + v3 * v3);
+        const int64_t c = c1 + (int64_t)n1 * (c2 + (int64_t)n2 * c3);
+        for (int sh = 0; sh < nb; ++sh)
+            if (Q >= edge[sh] + res_range && Q < edge[sh + 1])
+                shells[sh].push_back({splitmix64(seed ^ splitmix64((uint64_t)c)), c, f});
+    }
+    struct Ent {
+        int64_t pencil;
+        uint32_t meta;
+        float2 val;
+        int shell;
+    };
+    std::vector<Ent> ents;
+    std::vector<double> den(nb, 0.0);
+    for (int sh = 0; sh < nb; ++sh) {
+        auto& v = shells[sh];
+        if (v.empty()) continue;
+        size_t cut = (size_t)std::floor((double)v.size() * pct);
+        if (cut == 0) cut = 1;
+        std::partial_sort(v.begin(), v.begin() + cut, v.end(), [](const Cand& a, const Cand& b) {
+            return a.key != b.key ? a.key < b.key : a.c < b.c;
+        });
+        for (size_t q = 0; q < cut; ++q) {
+            const int64_t f = v[q].f;
+            mask[f] = 0;   // measuredK_mask(R_freeInd) = false (:66)
+            float2 val = M[f];
+            den[sh] += std::sqrt((double)val.x * val.x + (double)val.y * val.y);
+            int x = (int)(f % n1), y = (int)((f / n1) % n2), z = (int)(f / ((int64_t)n1 * n2));
+            if (2 * x > n1) {   // stored as its mate: k(s) = conj(k(-s))
+                x = (n1 - x) % n1;
+                y = (n2 - y) % n2;
+                z = (n3 - z) % n3;
+                val.y = -val.y;
+            }
+            ents.push_back({(int64_t)y + (int64_t)n2 * x, (uint32_t)z | ((uint32_t)y << 11), val, sh});
+        }
+    }
+    // per half pencil lists (stable counting order), and the shell order for the sums
+    const int64_t npen = (int64_t)(n1 / 2 + 1) * n2;
+    std::vector<int64_t> off(npen + 1, 0);
+    for (const Ent& e : ents) off[e.pencil + 1]++;
+    for (int64_t q = 0; q < npen; ++q) off[q + 1] += off[q];
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    const size_t ne = std::max<size_t>(ents.size(), 1);
+    std::vector<uint32_t> meta(ne, 0);
+    std::vector<float2> val(ne, make_float2(0.0f, 0.0f));
+    std::vector<int32_t> slot_of(ents.size());
+    for (size_t q = 0; q < ents.size(); ++q) {
+        const int64_t sl = pos[ents[q].pencil]++;
+        meta[sl] = ents[q].meta;
+        val[sl] = ents[q].val;
+        slot_of[q] = (int32_t)sl;
+    }
+    std::vector<int32_t> shell_off(nb + 1, 0), order(ne, 0);
+    for (const Ent& e : ents) shell_off[e.shell + 1]++;
+    for (int sh = 0; sh < nb; ++sh) shell_off[sh + 1] += shell_off[sh];
+    {
+        std::vector<int32_t> sp(shell_off.begin(), shell_off.end() - 1);
+        for (size_t q = 0; q < ents.size(); ++q) order[sp[ents[q].shell]++] = slot_of[q];
+    }
+    j->rf_bins = nb;
+    j->rf_off.alloc(sizeof(int64_t) * (npen + 1));
+    j->rf_meta.alloc(sizeof(uint32_t) * ne);
+    j->rf_val.alloc(sizeof(float2) * ne);
+    j->rf_terms.alloc(sizeof(float) * ne);
+    j->rf_order.alloc(sizeof(int32_t) * ne);
+    j->rf_shell_off.alloc(sizeof(int32_t) * (nb + 1));
+    j->rf_den.alloc(sizeof(double) * nb);
+    j->Rfree.alloc(sizeof(float) * (size_t)nb * std::max(j->prm.numIterations, 1));
+    if (!j->mask.p) j->mask.alloc(nf);
+    GF_HIP(hipMemcpyAsync(j->mask.p, mask.data(), nf, hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(j->rf_off.p, off.data(), sizeof(int64_t) * (npen + 1), hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(j->rf_meta.p, meta.data(), sizeof(uint32_t) * ne, hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(j->rf_val.p, val.data(), sizeof(float2) * ne, hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(j->rf_order.p, order.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(j->rf_shell_off.p, shell_off.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice,
+                          st));
+    GF_HIP(hipMemcpyAsync(j->rf_den.p, den.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    GF_HIP(hipStreamSynchronize(st));
+    j->have_mask = true;
+}
+
 HalfIterArgs p5_args(const genfire_job* j, int it)
 {
     const ReconState& r = j->rs;
@@ -409,6 +542,12 @@ void recon_begin(genfire_job* j)
     r.sched = cutoff_schedule(N, (int)j->delays.size());
     r.cur = 0;
     GF_CHECK(N >= 1 && r.sched.back() == N, GENFIRE_ERR_ARG, "GENFIRE: numIterations must be >= 1");
+
+    j->rf_bins = 0;
+    if (j->prm.calculate_Rfree) {
+        GF_CHECK(half, GENFIRE_ERR_UNSUPPORTED, "GENFIRE: Rfree needs a grid on the half-spectrum path");
+        rfree_select(j);
+    }
 
     // measured voxels (errInd, reconstruct.m:99) as per-pencil lists
     const uint8_t* mask = j->have_mask ? j->mask.as<uint8_t>() : nullptr;
@@ -484,6 +623,12 @@ void recon_begin(genfire_job* j)
     ha.positivity = ia.positivity;
     ha.support = ia.support;
     ha.dr = ia.dr;
+    if (j->rf_bins > 0) {
+        ha.rf_off = j->rf_off.as<int64_t>();
+        ha.rf_meta = j->rf_meta.as<uint32_t>();
+        ha.rf_val = j->rf_val.as<float2>();
+        ha.rf_terms = j->rf_terms.as<float>();
+    }
     GF_HIP(hipEventRecord(r.loop.a, st));
     r.begun = true;
 }
@@ -514,6 +659,15 @@ void recon_pre(genfire_job* j, int it)
     }
 }
 
+// Rfree of this iteration from the z pass's terms (slot = the device-side
+// iteration index, advanced later by the errK reduction)
+void rfree_step(genfire_job* j, hipStream_t st)
+{
+    if (j->rf_bins <= 0) return;
+    rfree_reduce(j->rf_terms.as<float>(), j->rf_order.as<int32_t>(), j->rf_shell_off.as<int32_t>(),
+                 j->rf_den.as<double>(), j->rf_bins, j->Rfree.as<float>(), j->errctr.as<int>(), st);
+}
+
 void recon_body(genfire_job* j, int it)
 {
     ReconState& r = j->rs;
@@ -527,6 +681,7 @@ void recon_body(genfire_job* j, int it)
         kstat_begin(j, GENFIRE_KSTAT_FFT_FUSED, &e_fused);
         half_z(r.ha, st);
         kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
+        rfree_step(j, st);
         half_y(r.ha, 1, st);
         if (it < r.N) half_last_first(p5_args(j, it), st);
         else half_last(p5_args(j, it), st);
@@ -561,6 +716,7 @@ void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStr
         }
         half_y(all, 0, st);
         half_z(all, st);
+        for (int i = c0; i < c1; ++i) rfree_step(jobs[i], st);
         half_y(all, 1, st);
         if (cont.count) half_last_first(cont, st);
         if (fin.count) half_last(fin, st);
@@ -589,6 +745,10 @@ void recon_end(genfire_job* j)
     crop_centre(j->best.p, 0, j->n1, j->n2, j->n1, j->recon.as<float>(), j->dim1, j->dim2, j->dim1, st);
     j->errK_h.assign(N, 0.0f);
     GF_HIP(hipMemcpyAsync(j->errK_h.data(), j->errK.p, sizeof(float) * N, hipMemcpyDeviceToHost, st));
+    j->Rfree_h.assign((size_t)j->rf_bins * N, -1.0f);
+    if (j->rf_bins > 0)
+        GF_HIP(hipMemcpyAsync(j->Rfree_h.data(), j->Rfree.p, sizeof(float) * j->Rfree_h.size(),
+                              hipMemcpyDeviceToHost, st));
     std::vector<double> h((size_t)r.nR * j->P * 4);
     if (r.nR > 0) GF_HIP(hipMemcpyAsync(h.data(), j->rout.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
     GF_HIP(hipEventRecord(r.all.b, st));

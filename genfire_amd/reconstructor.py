@@ -93,6 +93,10 @@ class GENFIRE_Reconstructor:
         self.griddingMethod = 1
         self.allowMultipleGridMatches = 1
         self.calculate_Rfree = 0
+        # Rfree properties reconstruct.m:47-58 reads (undeclared in the reference class)
+        self.numBinsRfree = 10
+        self.percentValuesForRfree = 0.05
+        self.RandomSeed = None
         # axes vectors (:50-52)
         self.vector1 = [0, 0, 1]
         self.vector2 = [0, 1, 0]
@@ -228,6 +232,10 @@ class GENFIRE_Reconstructor:
         p.precision = {"fp32": _lib.PREC_FP32, "split_bf16": _lib.PREC_SPLIT_BF16,
                        "fp16": _lib.PREC_FP16}[self.precision]
         p.compute_recIFFT = 1
+        p.calculate_Rfree = int(bool(self.calculate_Rfree))
+        p.numBinsRfree = int(self.numBinsRfree)
+        p.percentValuesForRfree = float(self.percentValuesForRfree)
+        p.RandomSeed = int(self.RandomSeed) if self.RandomSeed is not None else 0
         return p
 
     def _ensure_job(self):
@@ -279,6 +287,8 @@ class GENFIRE_Reconstructor:
         self.errK = job.errK()
         if self.monitor_R:
             self.errR, self.Rarr_record, self.Rarr2_record = job.errR()
+        if self.calculate_Rfree:
+            self.Rfree_complex = job.Rfree()
         self.timing.update(job.timing())
         return self
 
@@ -291,11 +301,22 @@ class GENFIRE_Reconstructor:
         return self
 
     def SaveResults(self):
-        """GENFIRE_Reconstructor.m:205-207 (as .npz next to filename_Results)."""
-        base = os.path.splitext(self.filename_Results)[0]
-        os.makedirs(os.path.dirname(base) or ".", exist_ok=True)
-        np.savez(base + ".npz", reconstruction=self.reconstruction, errK=self.errK,
-                 errR=self.errR if self.errR is not None else np.zeros(0))
+        """GENFIRE_Reconstructor.m:205-207: save(filename_Results, 'obj') -- here a
+        MAT v5 file (scipy.io) holding a struct 'obj' with the result fields and
+        the parameters, readable by MATLAB's load."""
+        import scipy.io
+        os.makedirs(os.path.dirname(self.filename_Results) or ".", exist_ok=True)
+        fields = {
+            "reconstruction": self.reconstruction, "errK": self.errK, "errR": self.errR,
+            "Rarr_record": self.Rarr_record, "Rarr2_record": self.Rarr2_record,
+            "Rfree_complex": self.Rfree_complex, "InputAngles": self.InputAngles,
+            "numIterations": self.numIterations, "oversamplingRatio": self.oversamplingRatio,
+            "interpolationCutoffDistance": self.interpolationCutoffDistance,
+            "constraintEnforcementMode": self.constraintEnforcementMode,
+            "griddingMethod": self.griddingMethod, "calculate_Rfree": self.calculate_Rfree,
+        }
+        obj = {k: (np.zeros((0, 0)) if v is None else np.asarray(v)) for k, v in fields.items()}
+        scipy.io.savemat(self.filename_Results, {"obj": obj}, do_compression=True)
 
     def set_parameters(self, *args):
         """GENFIRE_Reconstructor.m:210-225."""

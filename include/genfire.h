@@ -52,7 +52,10 @@ typedef struct genfire_ctx genfire_ctx;
 typedef struct genfire_job genfire_job;
 
 /* Mirrors the GENFIRE_Reconstructor properties (GENFIRE_Reconstructor.m:30-70),
- * same names and defaults; the last three fields are additions of this library. */
+ * same names and defaults; algorithm, precision and compute_recIFFT are
+ * additions of this library, and the Rfree fields define the properties
+ * reconstruct.m:46-69 reads but the class never declares (numBinsRfree,
+ * percentValuesForRfree, RandomSeed). */
 typedef struct genfire_params {
     int32_t numIterations;              /* 50    (:42) */
     double oversamplingRatio;           /* 3     (:43) */
@@ -69,6 +72,10 @@ typedef struct genfire_params {
     int32_t algorithm;                  /* GENFIRE_ALG_GENFIRE */
     int32_t precision;                  /* GENFIRE_PREC_FP32 */
     int32_t compute_recIFFT;            /* 1: also form recIFFT (fillInFourierGrid_DFT.m:304) */
+    int32_t calculate_Rfree;            /* 0     (:47) withhold points and monitor Rfree */
+    int32_t numBinsRfree;               /* 10    resolution shells (reconstruct.m:47) */
+    double percentValuesForRfree;       /* 0.05  fraction withheld per shell (:58) */
+    int64_t RandomSeed;                 /* 0     seed of the withheld-point permutation (:52-55) */
 } genfire_params;
 
 /* Library version, major*10000 + minor*100 + patch. */
@@ -140,6 +147,12 @@ int genfire_job_get_errK(genfire_job* job, float* errK);
 /* errR (floor(N/L)), Rarr_record and Rarr2_record (P x floor(N/L), column-major),
  * reconstruct.m:168-185.  Any pointer may be NULL. */
 int genfire_job_get_errR(genfire_job* job, float* errR, double* Rarr, double* Rarr2);
+
+/* Rfree_complex (reconstruct.m:137-147): numBinsRfree x numIterations
+ * column-major, entry (shell, iteration) = sum|k(withheld) - M(withheld)| /
+ * sum|M(withheld)| over the shell's withheld points, k the transform errK uses;
+ * -1 where a shell has no measured point.  Requires calculate_Rfree. */
+int genfire_job_get_Rfree(genfire_job* job, float* Rfree);
 
 /* Asynchronous form for serving: grid + reconstruct enqueued on the job's
  * worker; poll returns 1 in *done when finished; wait blocks. */

@@ -469,13 +469,56 @@ class ReconResult:
     Rarr_record: np.ndarray | None = None
     Rarr2_record: np.ndarray | None = None
     last_iterate: np.ndarray | None = None
+    Rfree: np.ndarray | None = None
+
+
+_U64 = np.uint64
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    x = np.asarray(x, _U64) + _U64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> _U64(30))) * _U64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> _U64(27))) * _U64(0x94D049BB133111EB)
+    return x ^ (x >> _U64(31))
+
+
+def rfree_select(measuredK: np.ndarray, num_bins: int, percent: float, seed: int):
+    """reconstruct.m:46-69 with the build's seeded permutation in place of
+    MATLAB's randperm after rng(RandomSeed) (MATLAB's generator is not
+    available): per shell s, the candidates find(measuredK ~= 0 & Q >= edge_s -
+    0.05 & Q < edge_s+1) are ordered by splitmix64(splitmix64(seed) ^
+    splitmix64(index)) (ties by index) and the first floor(count * percent),
+    at least one, are withheld.  Returns [(centred linear indices, values)] per
+    shell (empty shells give empty arrays)."""
+    shape = measuredK.shape
+    Q = make_kspace_indices(shape)
+    edges = matlab_linspace(0.0, 1.0, num_bins + 1)                                  # :47
+    nz = np.asarray(measuredK) != 0
+    sd = splitmix64(np.array([seed], dtype=np.int64).astype(_U64))[0]
+    Mf = np.asarray(measuredK).ravel(order="F")
+    out = []
+    with np.errstate(over="ignore"):
+        for s in range(num_bins):
+            cand = find(nz & (Q >= edges[s] + (-0.05)) & (Q < edges[s + 1]))          # :50
+            if cand.size == 0:
+                out.append((cand, Mf[cand]))
+                continue
+            keys = splitmix64(sd ^ splitmix64(cand.astype(_U64)))
+            perm = cand[np.lexsort((cand, keys))]                                       # :55-57
+            cut = int(np.floor(cand.size * percent))                                    # :58
+            if cut == 0:
+                cut = 1                                                                 # :59-61
+            sel = np.sort(perm[:cut])
+            out.append((sel, Mf[sel]))                                                  # :62, :67
+    return out
 
 
 def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, dim1: int,
                 dim2: int, num_iterations: int, mode: int = 3, positivity: bool = True,
                 support_on: bool = True, algorithm: str = "genfire", precision: str = "fp32",
                 initial_object: np.ndarray | None = None, monitor_R: bool = False,
-                loop_length: int = 10, projections=None, angles=None) -> ReconResult:
+                loop_length: int = 10, projections=None, angles=None, rfree=None) -> ReconResult:
     """reconstruct.m:28-194 (GENFIRE, algorithm='genfire').
 
     algorithm='dr' is this build's Douglas-Rachford variant composed of the same two
@@ -487,6 +530,8 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
     pinned only by its fixed-point and convex-feasibility properties.
 
     precision: 'fp32' (MATLAB single, as when the support constraint is on) or 'fp64'.
+    rfree: None or (num_bins, percent, seed): withhold points per shell
+    (rfree_select, :46-69) and record Rfree_complex (:137-147) in ``Rfree``.
     """
     cdt = np.complex64 if precision == "fp32" else np.complex128
     rdt = np.float32 if precision == "fp32" else np.float64
@@ -497,6 +542,15 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
     measured = (M != 0) & np.asarray(mask, bool)
     indicators = np.zeros(shape)                                                      # :39
     indicators[measured] = 1 - Q[measured]                                            # :42
+    rf = None
+    if rfree is not None:
+        rf = rfree_select(M, *rfree)
+        mflat = measured.ravel(order="F").copy()
+        for sel, _ in rf:
+            mflat[sel] = False                                                        # :66
+        measured = mflat.reshape(shape, order="F")
+        rf = [(iffshift3_ind(shape, sel), vals) for sel, vals in rf]                  # :65
+        Rfree = np.full((len(rf), num_iterations), -1.0, np.float32)                  # :92-94
     delays = delay_indicators(mode)
     if initial_object is None:
         x = np.zeros(shape, rdt)                                                      # :77
@@ -547,6 +601,11 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
         kf = k.reshape(-1, order="F")
         e = np.abs(np.abs(kf[err_sh]).astype(np.float64) - np.abs(M_err).astype(np.float64))
         errK[it - 1] = np.float32(np.sum(e) / den)                                    # :135
+        if rf is not None:                                                            # :137-147
+            for sh, (ind_sh, vals) in enumerate(rf):
+                if ind_sh.size:
+                    num = np.sum(np.abs(kf[ind_sh].astype(np.complex128) - vals.astype(np.complex128)))
+                    Rfree[sh, it - 1] = np.float32(num / np.sum(np.abs(vals.astype(np.complex128))))
         if errK[it - 1] < best_err:                                                   # :149-153
             best_err = errK[it - 1]
             recon = r.copy()
@@ -569,4 +628,4 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
                        errR=errR if monitor_R else None,
                        Rarr_record=Rrec if monitor_R else None,
                        Rarr2_record=R2rec if monitor_R else None,
-                       last_iterate=x)
+                       last_iterate=x, Rfree=Rfree if rf is not None else None)

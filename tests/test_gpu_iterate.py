@@ -124,3 +124,38 @@ def test_graph_replays_match_direct_launches(ctx, O, dims, monkeypatch):
     assert out["1"][3] > 10 and out["0"][3] == 0, (out["1"][3], out["0"][3])
     for a, b in zip(out["1"][:3], out["0"][:3]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("alg", ["genfire", "dr"])
+@pytest.mark.parametrize("shape", [(64, 64, 15), (96, 80, 10)])
+def test_rfree_matches_oracle(ctx, O, alg, shape):
+    """Rfree (reconstruct.m:46-69,137-147): the same withheld set as the oracle's
+    restatement (same seeded permutation), the masked reconstruction and errK,
+    and every Rfree_complex(shell, iteration) within 1e-4."""
+    d1, d2, iters = shape
+    print(f"[iter] Rfree alg={alg} dims={shape}", flush=True)
+    vol, projs, ang, mk, sup = _setup(O, dim=d1, dim2=d2, P=13, seed=5)
+    p = _lib.default_params()
+    p.numIterations = iters
+    p.oversamplingRatio = mk.shape[0] / d1
+    p.interpolationCutoffDistance = 0.3
+    p.constraintEnforcementMode = 1
+    p.algorithm = _lib.ALG_DR if alg == "dr" else _lib.ALG_GENFIRE
+    p.calculate_Rfree = 1
+    p.numBinsRfree = 8
+    p.percentValuesForRfree = 0.05
+    p.RandomSeed = 11
+    job = _lib.Job(ctx, p, d1, d2, projs.shape[2])
+    job.set_inputs(projs, ang, sup)
+    job.set_measuredK(mk)
+    job.reconstruct()
+    ref = O.reconstruct(mk, mk != 0, sup, d1, d2, iters, mode=1, algorithm=alg, precision="fp64",
+                        rfree=(8, 0.05, 11))
+    rec = job.reconstruction()
+    err = np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction)
+    assert err < 1e-4, err
+    np.testing.assert_allclose(job.errK(), ref.errK, atol=2e-5, rtol=0)
+    rf = job.Rfree()
+    assert rf.shape == ref.Rfree.shape
+    np.testing.assert_array_equal(rf < 0, ref.Rfree < 0)
+    np.testing.assert_allclose(rf, ref.Rfree, atol=1e-4, rtol=0)

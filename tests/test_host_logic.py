@@ -71,3 +71,30 @@ def test_fft_gridding_is_refused():
 def test_volumn_index_matches_oracle(O):
     for big, small in [((128, 128, 128), (64, 64, 64)), ((45, 46, 45), (15, 16, 15)), ((9, 8, 9), (4, 3, 4))]:
         np.testing.assert_array_equal(My_volumn_index(big, small), O.volumn_index(big, small))
+
+
+def test_save_results_mat_roundtrip(tmp_path):
+    """SaveResults (GENFIRE_Reconstructor.m:205-207) writes a MAT v5 struct 'obj'
+    that loadmat (as MATLAB's load) reads back field by field."""
+    import scipy.io
+    G = GENFIRE_Reconstructor()
+    G.filename_Results = str(tmp_path / "res" / "GENFIRE_rec.mat")
+    rng = np.random.default_rng(0)
+    G.reconstruction = rng.random((4, 5, 4)).astype(np.float32)
+    G.errK = rng.random(7).astype(np.float32)
+    G.Rfree_complex = rng.random((3, 7)).astype(np.float32)
+    G.InputAngles = rng.random((5, 3))
+    G.SaveResults()
+    d = scipy.io.loadmat(G.filename_Results, squeeze_me=False)["obj"][0, 0]
+    np.testing.assert_array_equal(d["reconstruction"], G.reconstruction)
+    np.testing.assert_array_equal(d["errK"].ravel(), G.errK)
+    np.testing.assert_array_equal(d["Rfree_complex"], G.Rfree_complex)
+    np.testing.assert_array_equal(d["InputAngles"], G.InputAngles)
+    assert d["errR"].size == 0 and int(d["numIterations"].ravel()[0]) == G.numIterations
+
+
+def test_rfree_parameters_reach_the_library():
+    G = GENFIRE_Reconstructor()
+    G.calculate_Rfree, G.numBinsRfree, G.percentValuesForRfree, G.RandomSeed = 1, 6, 0.1, 42
+    p = G._params()
+    assert (p.calculate_Rfree, p.numBinsRfree, p.percentValuesForRfree, p.RandomSeed) == (1, 6, 0.1, 42)

@@ -119,3 +119,44 @@ def test_rfactor_zero_for_self_consistent(O):
     projs = np.stack([O.project(v, *a) for a in ang], 2)
     R1, R2 = O.rfactors(v, projs, ang)
     assert np.all(R1 < 1e-12) and np.all(R2 < 1e-12)
+
+
+def test_rfree_selection(O):
+    """reconstruct.m:46-69 restated: per shell [edge - 0.05, edge'), floor(count
+    * percent) withheld (at least one), all among the measured voxels and inside
+    the shell, the same set for the same seed and a different one for another."""
+    rng = np.random.default_rng(4)
+    n = 16
+    M = (rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))).astype(np.complex64)
+    M[rng.random(M.shape) < 0.4] = 0
+    Q = O.make_kspace_indices(M.shape).ravel(order="F")
+    nz = np.flatnonzero(M.ravel(order="F") != 0)
+    edges = O.matlab_linspace(0.0, 1.0, 11)
+    a = O.rfree_select(M, 10, 0.05, 7)
+    b = O.rfree_select(M, 10, 0.05, 7)
+    c = O.rfree_select(M, 10, 0.05, 8)
+    assert len(a) == 10
+    differ = False
+    for s, ((sel, vals), (sel_b, _), (sel_c, _)) in enumerate(zip(a, b, c)):
+        cand = nz[(Q[nz] >= edges[s] - 0.05) & (Q[nz] < edges[s + 1])]
+        assert sel.size == (max(1, int(np.floor(cand.size * 0.05))) if cand.size else 0)
+        assert np.all(np.isin(sel, cand))
+        np.testing.assert_array_equal(vals, M.ravel(order="F")[sel])
+        np.testing.assert_array_equal(sel, sel_b)
+        differ |= not np.array_equal(sel, sel_c)
+    assert differ
+
+
+@pytest.mark.parametrize("alg", ["genfire", "dr"])
+def test_rfree_vanishes_at_fixed_point(O, alg):
+    """With M = F(x*) for a positive supported x*, the withheld points are
+    predicted by the iterate: Rfree of every non-empty shell falls to ~0, while
+    it starts at 1 (k = 0 at the zero initial object).  An error in the index
+    shift or in the value pairing would leave it large."""
+    obj, M = _obj()
+    r = O.reconstruct(M, np.ones(M.shape, bool), obj > 0, 8, 8, 60, mode=3, algorithm=alg, precision="fp64",
+                      rfree=(5, 0.1, 3))
+    filled = r.Rfree[:, 0] >= 0
+    assert filled.any()
+    np.testing.assert_allclose(r.Rfree[filled, 0], 1.0, atol=1e-6)
+    assert np.all(r.Rfree[filled, -1] < 1e-3), r.Rfree[:, -1]
