@@ -35,6 +35,7 @@ EXPORTS = [
     "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_get_Rfree", "genfire_job_submit", "genfire_job_poll",
     "genfire_job_wait", "genfire_job_timing", "genfire_job_set_profiling",
     "genfire_job_kernel_stats", "genfire_job_graph_replays", "genfire_project", "genfire_rfactor", "genfire_fftn",
+    "genfire_project_general", "genfire_rfactor_general",
     "genfire_server_config_default", "genfire_server_create", "genfire_server_destroy",
     "genfire_server_submit", "genfire_server_status", "genfire_server_wait", "genfire_server_drain",
     "genfire_server_result", "genfire_server_release", "genfire_server_get_stats",
@@ -154,6 +155,9 @@ def load():
         "genfire_project": ([vp, pf, i32, i32, i32, f64, f64, f64, pf], ctypes.c_int),
         "genfire_rfactor": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pf], ctypes.c_int),
         "genfire_fftn": ([vp, pf, i32, i32, i32, i32], ctypes.c_int),
+        "genfire_project_general": ([vp, pf, i32, i32, i32, f64, f64, f64, pd, pd, pd, pf], ctypes.c_int),
+        "genfire_rfactor_general": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pd, pd, pd, pf],
+                                    ctypes.c_int),
         "genfire_server_config_default": ([P(ServerConfig)], ctypes.c_int),
         "genfire_server_create": ([vp, P(ServerConfig), P(vp)], ctypes.c_int),
         "genfire_server_destroy": ([vp], ctypes.c_int),
@@ -225,6 +229,35 @@ class Context:
                                           float(theta), float(psi), _ptr(out, ctypes.c_float)),
                    "genfire_project")
         return out.reshape(d1, d2, order="F")
+
+    def project_general(self, vol: np.ndarray, phi: float, theta: float, psi: float, v1, v2, v3) -> np.ndarray:
+        """calculate3Dprojection_RealSpaceinterp_general (trilinear)."""
+        d1, d2, d3 = vol.shape
+        v = np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F"))
+        vec = [np.ascontiguousarray(np.asarray(x, np.float64).reshape(3)) for x in (v1, v2, v3)]
+        out = np.zeros(d1 * d2, np.float32)
+        self.check(self.L.genfire_project_general(self.h, _ptr(v, ctypes.c_float), d1, d2, d3, float(phi),
+                                                  float(theta), float(psi), *[_ptr(x, ctypes.c_double) for x in vec],
+                                                  _ptr(out, ctypes.c_float)), "genfire_project_general")
+        return out.reshape(d1, d2, order="F")
+
+    def rfactor_general(self, model: np.ndarray, projections: np.ndarray, angles: np.ndarray, v1, v2, v3):
+        """Tian_calc_Rfactor_realspace with the general-axis projector."""
+        d1, d2, d3 = model.shape
+        p1, p2, P = projections.shape
+        v = np.ascontiguousarray(np.asarray(model, np.float32).ravel(order="F"))
+        m = np.ascontiguousarray(np.asarray(projections, np.float32).ravel(order="F"))
+        ang = np.ascontiguousarray(np.asarray(angles, np.float64).ravel(order="F"))
+        vec = [np.ascontiguousarray(np.asarray(x, np.float64).reshape(3)) for x in (v1, v2, v3)]
+        rt = np.zeros(1)
+        ra = np.zeros(P)
+        sim = np.zeros(p1 * p2 * P, np.float32)
+        self.check(self.L.genfire_rfactor_general(self.h, _ptr(v, ctypes.c_float), d1, d2, d3,
+                                                  _ptr(m, ctypes.c_float), p1, p2, P, _ptr(ang, ctypes.c_double),
+                                                  *[_ptr(x, ctypes.c_double) for x in vec],
+                                                  _ptr(rt, ctypes.c_double), _ptr(ra, ctypes.c_double),
+                                                  _ptr(sim, ctypes.c_float)), "genfire_rfactor_general")
+        return float(rt[0]), ra, sim.reshape(p1, p2, P, order="F")
 
     def rfactor(self, model: np.ndarray, projections: np.ndarray, angles: np.ndarray):
         """Tian_calc_Rfactor_realspace(model, Projs, Angles) -> (Rtotal, Rarr, simuProjs)."""

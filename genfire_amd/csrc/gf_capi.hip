@@ -357,42 +357,114 @@ int genfire_job_kernel_stats(const genfire_job* j, int32_t k, int64_t* launches,
     return GENFIRE_OK;
 }
 
+namespace {
+
+void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, const double ang[3],
+                 const double* vectors, float* proj)
+{
+    use_device(ctx);
+    GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: volume dimensions must be >= 2");
+    hipStream_t st;
+    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    DevBuf v, vy, r, o;
+    const size_t nv = (size_t)d1 * d2 * d3;
+    v.alloc(sizeof(float) * nv);
+    o.alloc(sizeof(float) * (size_t)d1 * d2);
+    bool any_y = false;
+    std::vector<float> Rf = proj_matrices(ang, 1, &any_y, vectors);
+    r.alloc(sizeof(float) * Rf.size());
+    GF_HIP(hipMemcpyAsync(v.p, vol, sizeof(float) * nv, hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
+    if (any_y) {
+        vy.alloc(sizeof(float) * nv);
+        volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
+    }
+    ProjArgs pa{};
+    pa.vol = v.as<float>();
+    pa.vol_y = vy.as<float>();
+    pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
+    pa.R = r.as<float>();
+    pa.P = 1;
+    pa.out = o.as<float>();
+    pa.m1 = d1; pa.m2 = d2;
+    project(pa, st);
+    GF_HIP(hipMemcpyAsync(proj, o.p, o.bytes, hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    GF_HIP(hipStreamDestroy(st));
+}
+
+void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, const float* projections, int p1,
+                 int p2, int P, const double* angles, const double* vectors, double* Rtotal, double* Rarr,
+                 float* simu)
+{
+    use_device(ctx);
+    GF_CHECK(p1 <= d1 && p2 <= d2, GENFIRE_ERR_ARG, "GENFIRE: projection larger than the model");
+    hipStream_t st;
+    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    DevBuf v, vy, r, o, m, s, s4;
+    const size_t nv = (size_t)d1 * d2 * d3, np = (size_t)p1 * p2 * P;
+    v.alloc(sizeof(float) * nv);
+    m.alloc(sizeof(float) * np);
+    o.alloc(sizeof(float) * np);
+    bool any_y = false;
+    std::vector<float> Rf = proj_matrices(angles, P, &any_y, vectors);
+    r.alloc(sizeof(float) * Rf.size());
+    GF_HIP(hipMemcpyAsync(v.p, model, sizeof(float) * nv, hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(m.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, st));
+    GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
+    if (any_y) {
+        vy.alloc(sizeof(float) * nv);
+        volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
+    }
+    const int nblk = project_blocks(p1, p2);
+    s.alloc(sizeof(double) * (size_t)P * 4 * nblk);
+    s4.alloc(sizeof(double) * (size_t)P * 4);
+    ProjArgs pa{};
+    pa.vol = v.as<float>();
+    pa.vol_y = vy.as<float>();
+    pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
+    pa.R = r.as<float>();
+    pa.P = P;
+    pa.out = o.as<float>();
+    pa.meas = m.as<float>();
+    pa.m1 = p1; pa.m2 = p2;
+    pa.sums = s.as<double>();
+    pa.post_hoc = 1;
+    project(pa, st);
+    project_reduce(s.as<double>(), P, nblk, s4.as<double>(), st);
+    std::vector<double> h((size_t)P * 4);
+    GF_HIP(hipMemcpyAsync(h.data(), s4.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    if (simu) GF_HIP(hipMemcpyAsync(simu, o.p, o.bytes, hipMemcpyDeviceToHost, st));
+    GF_HIP(hipStreamSynchronize(st));
+    GF_HIP(hipStreamDestroy(st));
+    double mean = 0.0;
+    for (int p = 0; p < P; ++p) {
+        const double rr = h[4 * p] / h[4 * p + 2];
+        if (Rarr) Rarr[p] = rr;
+        mean += rr;
+    }
+    if (Rtotal) *Rtotal = mean / P;
+}
+
+}  // namespace
+
 int genfire_project(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3, double phi,
                     double theta, double psi, float* proj)
 {
     if (!ctx || !vol || !proj) return GENFIRE_ERR_ARG;
-    return guarded(ctx, [&] {
-        use_device(ctx);
-        GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: volume dimensions must be >= 2");
-        hipStream_t st;
-        GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        DevBuf v, vy, r, o;
-        const size_t nv = (size_t)d1 * d2 * d3;
-        v.alloc(sizeof(float) * nv);
-        o.alloc(sizeof(float) * (size_t)d1 * d2);
-        const double ang[3] = {phi, theta, psi};
-        bool any_y = false;
-        std::vector<float> Rf = proj_matrices(ang, 1, &any_y);
-        r.alloc(sizeof(float) * Rf.size());
-        GF_HIP(hipMemcpyAsync(v.p, vol, sizeof(float) * nv, hipMemcpyHostToDevice, st));
-        GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
-        if (any_y) {
-            vy.alloc(sizeof(float) * nv);
-            volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
-        }
-        ProjArgs pa{};
-        pa.vol = v.as<float>();
-        pa.vol_y = vy.as<float>();
-        pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
-        pa.R = r.as<float>();
-        pa.P = 1;
-        pa.out = o.as<float>();
-        pa.m1 = d1; pa.m2 = d2;
-        project(pa, st);
-        GF_HIP(hipMemcpyAsync(proj, o.p, o.bytes, hipMemcpyDeviceToHost, st));
-        GF_HIP(hipStreamSynchronize(st));
-        GF_HIP(hipStreamDestroy(st));
-    });
+    const double ang[3] = {phi, theta, psi};
+    return guarded(ctx, [&] { project_one(ctx, vol, d1, d2, d3, ang, nullptr, proj); });
+}
+
+int genfire_project_general(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3, double phi,
+                            double theta, double psi, const double vector1[3], const double vector2[3],
+                            const double vector3[3], float* proj)
+{
+    if (!ctx || !vol || !proj || !vector1 || !vector2 || !vector3) return GENFIRE_ERR_ARG;
+    const double ang[3] = {phi, theta, psi};
+    double vec[9];
+    for (int i = 0; i < 3; ++i) { vec[i] = vector1[i]; vec[3 + i] = vector2[i]; vec[6 + i] = vector3[i]; }
+    return guarded(ctx, [&] { project_one(ctx, vol, d1, d2, d3, ang, vec, proj); });
 }
 
 int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
@@ -401,53 +473,21 @@ int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2
 {
     if (!ctx || !model || !projections || !angles || P < 1) return GENFIRE_ERR_ARG;
     return guarded(ctx, [&] {
-        use_device(ctx);
-        GF_CHECK(p1 <= d1 && p2 <= d2, GENFIRE_ERR_ARG, "GENFIRE: projection larger than the model");
-        hipStream_t st;
-        GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        DevBuf v, vy, r, o, m, s, s4;
-        const size_t nv = (size_t)d1 * d2 * d3, np = (size_t)p1 * p2 * P;
-        v.alloc(sizeof(float) * nv);
-        m.alloc(sizeof(float) * np);
-        o.alloc(sizeof(float) * np);
-        bool any_y = false;
-        std::vector<float> Rf = proj_matrices(angles, P, &any_y);
-        r.alloc(sizeof(float) * Rf.size());
-        GF_HIP(hipMemcpyAsync(v.p, model, sizeof(float) * nv, hipMemcpyHostToDevice, st));
-        GF_HIP(hipMemcpyAsync(m.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, st));
-        GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
-        if (any_y) {
-            vy.alloc(sizeof(float) * nv);
-            volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
-        }
-        const int nblk = project_blocks(p1, p2);
-        s.alloc(sizeof(double) * (size_t)P * 4 * nblk);
-        s4.alloc(sizeof(double) * (size_t)P * 4);
-        ProjArgs pa{};
-        pa.vol = v.as<float>();
-        pa.vol_y = vy.as<float>();
-        pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
-        pa.R = r.as<float>();
-        pa.P = P;
-        pa.out = o.as<float>();
-        pa.meas = m.as<float>();
-        pa.m1 = p1; pa.m2 = p2;
-        pa.sums = s.as<double>();
-        pa.post_hoc = 1;
-        project(pa, st);
-        project_reduce(s.as<double>(), P, nblk, s4.as<double>(), st);
-        std::vector<double> h((size_t)P * 4);
-        GF_HIP(hipMemcpyAsync(h.data(), s4.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
-        if (simu) GF_HIP(hipMemcpyAsync(simu, o.p, o.bytes, hipMemcpyDeviceToHost, st));
-        GF_HIP(hipStreamSynchronize(st));
-        GF_HIP(hipStreamDestroy(st));
-        double mean = 0.0;
-        for (int p = 0; p < P; ++p) {
-            const double rr = h[4 * p] / h[4 * p + 2];
-            if (Rarr) Rarr[p] = rr;
-            mean += rr;
-        }
-        if (Rtotal) *Rtotal = mean / P;
+        rfactor_all(ctx, model, d1, d2, d3, projections, p1, p2, P, angles, nullptr, Rtotal, Rarr, simu);
+    });
+}
+
+int genfire_rfactor_general(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
+                            const float* projections, int32_t p1, int32_t p2, int32_t P, const double* angles,
+                            const double vector1[3], const double vector2[3], const double vector3[3],
+                            double* Rtotal, double* Rarr, float* simu)
+{
+    if (!ctx || !model || !projections || !angles || P < 1 || !vector1 || !vector2 || !vector3)
+        return GENFIRE_ERR_ARG;
+    double vec[9];
+    for (int i = 0; i < 3; ++i) { vec[i] = vector1[i]; vec[3 + i] = vector2[i]; vec[6 + i] = vector3[i]; }
+    return guarded(ctx, [&] {
+        rfactor_all(ctx, model, d1, d2, d3, projections, p1, p2, P, angles, vec, Rtotal, Rarr, simu);
     });
 }
 

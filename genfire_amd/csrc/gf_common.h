@@ -89,5 +89,7 @@ void rotation_f32(const double v1[3], const double v2[3], const double v3[3], do
                   double theta, double psi, float Rf[9]);
 void plane_setup(const float R[9], PlaneConst* pc);
 void zyz_matrix(double phi, double theta, double psi, double R[9]);
+void rotation_f64(const double v1[3], const double v2[3], const double v3[3], double phi, double theta, double psi,
+                  double R[9]);
 
 }  // namespace gf

@@ -56,6 +56,19 @@ void sincosd(double x, double* s, double* c)
 
 }  // namespace
 
+// R = Q(v1,phi)*Q(v2,theta)*Q(v3,psi) in FP64
+// (src/calculate3Dprojection_RealSpaceinterp_general.m:3-7).
+void rotation_f64(const double v1[3], const double v2[3], const double v3[3], double phi, double theta, double psi,
+                  double R[9])
+{
+    double A[9], B[9], C[9], AB[9];
+    quat_rot(v1, phi, A);
+    quat_rot(v2, theta, B);
+    quat_rot(v3, psi, C);
+    mul3(A, B, AB);
+    mul3(AB, C, R);
+}
+
 // fillInFourierGrid_DFT.m:108-114: R = Q(v1,phi)*Q(v2,theta)*Q(v3,psi) in FP64,
 // stored as single (rotMATs, :101).
 void rotation_f32(const double v1[3], const double v2[3], const double v3[3], double phi,

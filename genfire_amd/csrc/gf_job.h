@@ -183,7 +183,10 @@ void job_set_inputs(genfire_job* j, const float* projections, const double* angl
 // Device bytes a job of these sizes holds at its peak (an upper estimate).
 int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P);
 
-std::vector<float> proj_matrices(const double* angles, int P, bool* any_y);
+// Projector matrices, kRStride floats per projection: the ZYZ matrix of
+// calculate3Dprojection_RealSpaceinterp.m:13-15, or with vectors (v1, v2, v3,
+// 9 doubles) the quaternion product of ..._general.m:3-7.
+std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const double* vectors = nullptr);
 
 // fillInFourierGrid_DFT + CentroSymmetricity + runGridding post-processing.
 void do_grid(genfire_job* j);

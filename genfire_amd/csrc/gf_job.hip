@@ -147,13 +147,16 @@ void kstat_end(genfire_job* j, int k, std::pair<hipEvent_t, hipEvent_t>& ev)
 
 // ZYZ matrices of the projector (calculate3Dprojection_RealSpaceinterp.m:13-15),
 // kRStride floats per projection with the y-fastest flag at [9].
-std::vector<float> proj_matrices(const double* angles, int P, bool* any_y)
+std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const double* vectors)
 {
     std::vector<float> Rm((size_t)kRStride * P, 0.0f);
     *any_y = false;
     for (int p = 0; p < P; ++p) {
         double R[9];
-        zyz_matrix(angles[p], angles[P + p], angles[2 * P + p], R);
+        if (vectors)   // src/calculate3Dprojection_RealSpaceinterp_general.m:3-7
+            rotation_f64(vectors, vectors + 3, vectors + 6, angles[p], angles[P + p], angles[2 * P + p], R);
+        else
+            zyz_matrix(angles[p], angles[P + p], angles[2 * P + p], R);
         float* r = Rm.data() + (size_t)kRStride * p;
         for (int q = 0; q < 9; ++q) r[q] = (float)R[q];
         if (project_wants_yfast(r)) { r[9] = 1.0f; *any_y = true; }

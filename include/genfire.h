@@ -195,6 +195,21 @@ int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2
                     const float* projections, int32_t p1, int32_t p2, int32_t num_projs,
                     const double* angles, double* Rtotal, double* Rarr, float* simuProjs);
 
+/* calculate3Dprojection_RealSpaceinterp_general(Vol, phi, theta, psi, vector1,
+ * vector2, vector3) (src/calculate3Dprojection_RealSpaceinterp_general.m):
+ * R = Q(vector1, phi) Q(vector2, theta) Q(vector3, psi) (MatrixQuaternionRot),
+ * trilinear as genfire_project. */
+int genfire_project_general(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3,
+                            double phi, double theta, double psi, const double vector1[3],
+                            const double vector2[3], const double vector3[3], float* proj);
+
+/* genfire_rfactor with the general-axis projector above (Tian's post-hoc R for
+ * tilt series recorded with the gridding's own rotation axes). */
+int genfire_rfactor_general(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
+                            const float* projections, int32_t p1, int32_t p2, int32_t num_projs,
+                            const double* angles, const double vector1[3], const double vector2[3],
+                            const double vector3[3], double* Rtotal, double* Rarr, float* simuProjs);
+
 /* fftn (inverse = 0) or ifftn (inverse = 1, scaled by 1/N) of an n1 x n2 x n3
  * interleaved-complex array in place, with this library's 3D FFT (MATLAB's
  * fftn/ifftn at reconstruct.m:133,165 and src/my_ifft.m:11). */

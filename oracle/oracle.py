@@ -428,6 +428,30 @@ def project(vol: np.ndarray, phi: float, theta: float, psi: float) -> np.ndarray
     return out.reshape(d1, d2, order="F")
 
 
+def quat_rot(vec, theta_deg: float) -> np.ndarray:
+    """MatrixQuaternionRot (src/MatrixQuaternionRot.m), FP64, R[i, j] = row i."""
+    th = theta_deg * np.pi / 180.0
+    a = np.asarray(vec, np.float64)
+    a = a / np.linalg.norm(a)
+    w, s = np.cos(th / 2.0), np.sin(th / 2.0)
+    x, y, z = -s * a[0], -s * a[1], -s * a[2]
+    return np.array([[1 - 2 * y * y - 2 * z * z, 2 * x * y + 2 * w * z, 2 * x * z - 2 * w * y],
+                     [2 * x * y - 2 * w * z, 1 - 2 * x * x - 2 * z * z, 2 * y * z + 2 * w * x],
+                     [2 * x * z + 2 * w * y, 2 * y * z - 2 * w * x, 1 - 2 * x * x - 2 * y * y]])
+
+
+def project_general(vol: np.ndarray, phi: float, theta: float, psi: float, v1, v2, v3) -> np.ndarray:
+    """src/calculate3Dprojection_RealSpaceinterp_general.m with R =
+    Q(v1,phi) Q(v2,theta) Q(v3,psi) (:3-7) and the trilinear interpolant of
+    ``project`` (FP64)."""
+    d1, d2, d3 = vol.shape
+    R = quat_rot(v1, phi) @ quat_rot(v2, theta) @ quat_rot(v3, psi)
+    out = np.zeros(d1 * d2, np.float64)
+    lib().go_project(np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F")),
+                     d1, d2, d3, np.ascontiguousarray(R.ravel(order="F")), out)
+    return out.reshape(d1, d2, order="F")
+
+
 def rfactors(model: np.ndarray, projections: np.ndarray, angles: np.ndarray):
     """R-factor monitor body, reconstruct.m:171-178: per projection
     R1 = sum|back-pj|/sum|pj| and R2 = ||back-pj||_F/||pj||_F."""

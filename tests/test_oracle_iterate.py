@@ -160,3 +160,34 @@ def test_rfree_vanishes_at_fixed_point(O, alg):
     assert filled.any()
     np.testing.assert_allclose(r.Rfree[filled, 0], 1.0, atol=1e-6)
     assert np.all(r.Rfree[filled, -1] < 1e-3), r.Rfree[:, -1]
+
+
+def test_general_projector_reduces_to_zyz(O):
+    """calculate3Dprojection_RealSpaceinterp.m:3-12 states that its ZYZ matrix is
+    Q([0 0 1],phi) Q([0 1 0],theta) Q([0 0 1],psi); the general projector with
+    those axes gives the ZYZ projection."""
+    rng = np.random.default_rng(8)
+    vol = rng.uniform(0, 1, (12, 10, 11)).astype(np.float32)
+    for ang in [(0.0, 0.0, 0.0), (20.0, -35.0, 50.0), (-7.0, 64.0, 3.0)]:
+        Rq = O.quat_rot((0, 0, 1), ang[0]) @ O.quat_rot((0, 1, 0), ang[1]) @ O.quat_rot((0, 0, 1), ang[2])
+        np.testing.assert_allclose(Rq, O.zyz(*ang), atol=1e-14)
+        np.testing.assert_allclose(O.project_general(vol, *ang, (0, 0, 1), (0, 1, 0), (0, 0, 1)),
+                                   O.project(vol, *ang), rtol=0, atol=1e-9)
+
+
+def test_general_projector_quarter_turn_about_x(O):
+    """Q([1 0 0], 90): the sum along z of the rotated volume equals an axis sum
+    of the unrotated one (closed form, integer grid points only)."""
+    rng = np.random.default_rng(9)
+    n = 9
+    vol = np.zeros((n, n, n), np.float32)   # zero faces: samples on the box faces may round outside
+    vol[1:-1, 1:-1, 1:-1] = rng.uniform(0, 1, (n - 2, n - 2, n - 2))
+    R = O.quat_rot((1, 0, 0), 90.0)
+    assert np.allclose(np.abs(R), [[1, 0, 0], [0, 0, 1], [0, 1, 0]], atol=1e-15)
+    got = O.project_general(vol, 90.0, 0.0, 0.0, (1, 0, 0), (0, 1, 0), (0, 0, 1))
+    # sample (X, Y, Z) reads vol at R (X, Y, Z): y' = +-Z, z' = +-Y, so the sum over Z runs over volume y
+    s = np.sign(R[1, 2])
+    expect = vol.sum(axis=1)            # (x, z')
+    expect = expect if np.sign(R[2, 1]) > 0 else expect[:, ::-1]
+    np.testing.assert_allclose(got, expect, rtol=0, atol=1e-5)
+    assert s != 0
