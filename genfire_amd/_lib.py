@@ -64,6 +64,8 @@ class GenfireParams(ctypes.Structure):
         ("numBinsRfree", ctypes.c_int32),
         ("percentValuesForRfree", ctypes.c_double),
         ("RandomSeed", ctypes.c_int64),
+        ("griddingMethod", ctypes.c_int32),
+        ("allowMultipleGridMatches", ctypes.c_int32),
     ]
 
 
@@ -364,8 +366,15 @@ class Job:
         return out.view(np.complex64).reshape(self.n1, self.n2, self.n1, order="F")
 
     def hits(self) -> np.ndarray:
-        out = np.zeros(int(np.prod(self.half_shape)), np.int32)
+        """Per-voxel hit counts: flat over the half grid (DFT gridding), or the
+        centred n1 x n1 x n1 grid before hermitianSymmetrize (FFT gridding)."""
+        if self.params.griddingMethod == 1:
+            out = np.zeros(self.n1 ** 3, np.int32)
+        else:
+            out = np.zeros(int(np.prod(self.half_shape)), np.int32)
         self._c(self.L.genfire_job_get_hits(self.h, _ptr(out, ctypes.c_int32)), "genfire_job_get_hits")
+        if self.params.griddingMethod == 1:
+            return out.reshape(self.n1, self.n1, self.n1, order="F")
         return out
 
     def recIFFT(self) -> np.ndarray:

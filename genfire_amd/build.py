@@ -3,7 +3,7 @@
     python -m genfire_amd.build
 
 Every .hip source in csrc/ is compiled for gfx950 with packed-FP32 instructions
-disabled; the gridding index kernels additionally with -ffp-contract=off (their
+disabled; the gridding index and FFT-gridding kernels additionally with -ffp-contract=off (their
 FP32 expressions must round once per operation, in the reference's order).  The
 host-only set-up math is compiled without contraction as well.
 """
@@ -25,7 +25,7 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", f"-I{INCLUDE}"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics"]
-NO_CONTRACT = {"gf_grid_index.hip", "gf_host_math.cpp"}
+NO_CONTRACT = {"gf_grid_index.hip", "gf_grid_fft.hip", "gf_host_math.cpp"}
 
 
 def _hipcc() -> str:

@@ -39,6 +39,8 @@ int genfire_params_default(genfire_params* p)
     p->numBinsRfree = 10;
     p->percentValuesForRfree = 0.05;
     p->RandomSeed = 0;
+    p->griddingMethod = 1;
+    p->allowMultipleGridMatches = 1;
     return GENFIRE_OK;
 }
 
@@ -142,6 +144,10 @@ int genfire_job_grid_count(const genfire_job* j, int64_t* n)
 {
     if (!j || !n) return GENFIRE_ERR_ARG;
     if (!j->gridded) return GENFIRE_ERR_STATE;
+    if (j->prm.griddingMethod == 1) {
+        set_err(j->ctx, "GENFIRE: FFT gridding forms no point list");
+        return GENFIRE_ERR_STATE;
+    }
     *n = j->npts;
     return GENFIRE_OK;
 }
@@ -152,6 +158,7 @@ int genfire_job_get_points(genfire_job* j, int64_t* ind, int32_t* proj, float* d
     if (!j) return GENFIRE_ERR_ARG;
     return guarded(j->ctx, [&] {
         GF_CHECK(j->gridded, GENFIRE_ERR_STATE, "GENFIRE: not gridded");
+        GF_CHECK(j->prm.griddingMethod != 1, GENFIRE_ERR_STATE, "GENFIRE: FFT gridding forms no point list");
         use_device(j->ctx);
         const int64_t n = j->npts;
         if (n == 0) return;
@@ -190,7 +197,7 @@ int genfire_job_get_hits(genfire_job* j, int32_t* out)
     return guarded(j->ctx, [&] {
         GF_CHECK(j->gridded && j->hits.p, GENFIRE_ERR_STATE, "GENFIRE: not gridded");
         use_device(j->ctx);
-        const int64_t n = j->hg.total();
+        const int64_t n = j->prm.griddingMethod == 1 ? full_size(j) : j->hg.total();
         std::vector<uint32_t> t(n);
         GF_HIP(hipMemcpy(t.data(), j->hits.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; ++i) out[i] = (int32_t)(t[i] & ~kExactBit);

@@ -54,6 +54,25 @@ void grid_finalize(const HalfGrid& hg, int n1, int n2, int centro, const double2
                    hipStream_t st);
 constexpr uint32_t kExactBit = 0x80000000u;
 
+// ---- FFT gridding (gf_grid_fft.hip), fillInFourierGrid.m ----
+// One projection's rotation as used by the scatter: rot = the first two columns
+// of the projector matrix R (rotkCoords = R'_fg * [kx; ky; 0] = kx R(:,1) +
+// ky R(:,2), single); nrm = R(:,3), the slab normal; axis = its largest
+// component.
+struct FftPlane {
+    float rot[6];
+    float nrm[3];
+    int axis;
+};
+// Padded, ifftshifted projections as complex d x d x P (FFT input).
+void fftgrid_load(const float* d_proj, int dim1, int dim2, int P, int d, float2* d_F, hipStream_t st);
+// Accumulates one projection (d_Fp: its d x d spectrum in FFT order) into the
+// centred d^3 sums: acc += w v, sw += w, hits += 1 per matched lattice point.
+void fftgrid_scatter(const FftPlane& pl, int d, int shift_max, float rho, double cutoff, const float2* d_Fp,
+                     double2* d_acc, double* d_sw, uint32_t* d_hits, hipStream_t st);
+// Weighted average, hermitianSymmetrize, single cast; measuredK in FFT order.
+void fftgrid_finalize(int d, const double2* d_acc, const double* d_sw, float2* d_M, hipStream_t st);
+
 // ---- layout helpers (gf_grid_values.hip) ----
 // Centred <-> FFT-order conversion of an n1 x n2 x n3 complex grid
 // (to_centred = 1: out = fftshift(in), 0: out = ifftshift(in)).

@@ -170,6 +170,9 @@ template <class F> int guarded(genfire_ctx* ctx, F&& f)
 inline int64_t full_size(const genfire_job* j) { return (int64_t)j->n1 * j->n2 * j->n1; }
 inline int64_t box_size(const genfire_job* j) { return (int64_t)j->dim1 * j->dim2 * j->dim1; }
 
+// Fourier grid edges: (n1_oversampled, n2_oversampled) for DFT gridding, (d, d)
+// for FFT gridding; raises when the FFT-gridding edge is not a whole number.
+void grid_edges(const genfire_params& p, int dim1, int dim2, int* n1, int* n2);
 // The checks of CheckPrepareData (GENFIRE_Reconstructor.m:115-171) and of this
 // build's size limits; raises with the reference's message where it has one.
 void validate_job(const genfire_params* p, int dim1, int dim2, int P);
@@ -188,7 +191,8 @@ int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P);
 // 9 doubles) the quaternion product of ..._general.m:3-7.
 std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const double* vectors = nullptr);
 
-// fillInFourierGrid_DFT + CentroSymmetricity + runGridding post-processing.
+// runGridding: fillInFourierGrid_DFT + CentroSymmetricity (griddingMethod 2) or
+// fillInFourierGrid + hermitianSymmetrize (griddingMethod 1), and recIFFT.
 void do_grid(genfire_job* j);
 
 // reconstruct.m in phases.  recon_begin sets up (support, initial object,

@@ -167,15 +167,13 @@ std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const
 // ---------------------------------------------------------------------------
 // fillInFourierGrid_DFT + CentroSymmetricity + runGridding post-processing
 // ---------------------------------------------------------------------------
-void do_grid(genfire_job* j)
+namespace {
+
+// fillInFourierGrid_DFT.m:85-291 + CentroSymmetricity: measuredK into j->M.
+void grid_dft_body(genfire_job* j, hipStream_t st)
 {
-    GF_CHECK(j->have_inputs, GENFIRE_ERR_STATE, "GENFIRE: inputs not set (call genfire_job_set_inputs)");
     const int P = j->P, n1 = j->n1, n2 = j->n2;
     const int centro = j->prm.DFT_CentroSymmetricity ? 1 : 0;
-    hipStream_t st = j->st;
-    EventTimer tm;
-    tm.init();
-    GF_HIP(hipEventRecord(tm.a, st));
 
     // rotation matrices and plane constants (fillInFourierGrid_DFT.m:100-118)
     std::vector<PlaneConst> pcs(P);
@@ -278,6 +276,65 @@ void do_grid(genfire_job* j)
     j->M.alloc(sizeof(float2) * full_size(j));
     grid_finalize(j->hg, n1, n2, centro, j->acc.as<double2>(), j->sw.as<double>(), j->hits.as<uint32_t>(),
                   j->exact.as<float2>(), j->M.as<float2>(), st);
+}
+
+// fillInFourierGrid.m:27-163 (FFT gridding) + hermitianSymmetrize: measuredK into
+// j->M, hits over the centred d^3 grid.
+void grid_fft_body(genfire_job* j, hipStream_t st)
+{
+    const int P = j->P, d = j->n1;
+    const int64_t plane = (int64_t)d * d, vox = plane * d;
+    DevBuf spec;
+    spec.alloc(sizeof(float2) * plane * P);
+    fftgrid_load(j->proj.as<float>(), j->dim1, j->dim2, P, d, spec.as<float2>(), st);   // :59-61
+    fft_axis(spec.as<float2>(), d, d, P, 0, 0, st);
+    fft_axis(spec.as<float2>(), d, d, P, 1, 0, st);
+
+    j->acc.alloc(sizeof(double2) * vox);
+    j->sw.alloc(sizeof(double) * vox);
+    j->hits.alloc(sizeof(uint32_t) * vox);
+    GF_HIP(hipMemsetAsync(j->acc.p, 0, j->acc.bytes, st));
+    GF_HIP(hipMemsetAsync(j->sw.p, 0, j->sw.bytes, st));
+    GF_HIP(hipMemsetAsync(j->hits.p, 0, j->hits.bytes, st));
+
+    // shiftMax (:117-121); a matched point lies within the cutoff and, for
+    // shiftMax = 0, inside the voxel's unit cube (radius sqrt(3)/2)
+    const double cutoff = j->prm.interpolationCutoffDistance;
+    const int shift_max = j->prm.allowMultipleGridMatches ? (int)mround(cutoff) : 0;
+    const double radius = std::min(cutoff, (shift_max + 0.5) * std::sqrt(3.0));
+    const float rho = (float)(radius + 0.02 + 2e-5 * d);
+    for (int p = 0; p < P; ++p) {
+        double R[9];
+        zyz_matrix(j->angles[p], j->angles[P + p], j->angles[2 * P + p], R);   // :80-84 (R' of the projector's)
+        FftPlane pl{};
+        for (int i = 0; i < 6; ++i) pl.rot[i] = (float)R[i];
+        for (int i = 0; i < 3; ++i) pl.nrm[i] = (float)R[6 + i];
+        pl.axis = 0;
+        for (int i = 1; i < 3; ++i)
+            if (std::fabs(pl.nrm[i]) > std::fabs(pl.nrm[pl.axis])) pl.axis = i;
+        fftgrid_scatter(pl, d, shift_max, rho, cutoff, spec.as<float2>() + plane * p, j->acc.as<double2>(),
+                        j->sw.as<double>(), j->hits.as<uint32_t>(), st);
+    }
+    j->M.alloc(sizeof(float2) * full_size(j));
+    fftgrid_finalize(d, j->acc.as<double2>(), j->sw.as<double>(), j->M.as<float2>(), st);   // :157-163
+    j->npts = 0;
+    GF_HIP(hipStreamSynchronize(st));   // spec is freed on return
+}
+
+}  // namespace
+
+void do_grid(genfire_job* j)
+{
+    GF_CHECK(j->have_inputs, GENFIRE_ERR_STATE, "GENFIRE: inputs not set (call genfire_job_set_inputs)");
+    const int n1 = j->n1, n2 = j->n2;
+    hipStream_t st = j->st;
+    EventTimer tm;
+    tm.init();
+    GF_HIP(hipEventRecord(tm.a, st));
+    if (j->prm.griddingMethod == 1)
+        grid_fft_body(j, st);
+    else
+        grid_dft_body(j, st);
     j->have_mask = false;
     j->mask.release();
 
@@ -850,6 +907,22 @@ void release_grid_extras(genfire_job* j)
 // ---------------------------------------------------------------------------
 // Job set-up (CheckPrepareData, GENFIRE_Reconstructor.m:115-171) and inputs
 // ---------------------------------------------------------------------------
+void grid_edges(const genfire_params& p, int dim1, int dim2, int* n1, int* n2)
+{
+    if (p.griddingMethod == 1) {
+        // fillInFourierGrid.m:38-42: kMeasured is d x d with d from Dim2
+        // (particleWindowSize(2), GENFIRE_Reconstructor.m:121)
+        const double e = dim2 % 2 == 0 ? dim2 * p.oversamplingRatio : 1.0 + (dim2 - 1) * p.oversamplingRatio;
+        GF_CHECK(e == std::floor(e) && e >= 1.0 && e < 65536.0, GENFIRE_ERR_ARG,
+                 "GENFIRE: FFT gridding needs a whole-number grid edge (Dim2*oversamplingRatio for even Dim2, "
+                 "1+(Dim2-1)*oversamplingRatio for odd), got " + std::to_string(e));
+        *n1 = *n2 = (int)e;
+        return;
+    }
+    *n1 = (int)mround(dim1 * p.oversamplingRatio);   // GENFIRE_Reconstructor.m:168
+    *n2 = (int)mround(dim2 * p.oversamplingRatio);   // :169
+}
+
 void validate_job(const genfire_params* p, int dim1, int dim2, int P)
 {
     GF_CHECK(dim1 >= 2 && dim2 >= 2 && P >= 1, GENFIRE_ERR_ARG,
@@ -861,7 +934,13 @@ void validate_job(const genfire_params* p, int dim1, int dim2, int P)
     GF_CHECK(p->precision == GENFIRE_PREC_FP32 || p->precision == GENFIRE_PREC_SPLIT_BF16 ||
                  p->precision == GENFIRE_PREC_FP16,
              GENFIRE_ERR_ARG, "GENFIRE: unknown gridding precision");
-    const int n1 = (int)mround(dim1 * p->oversamplingRatio), n2 = (int)mround(dim2 * p->oversamplingRatio);
+    GF_CHECK(p->griddingMethod == 1 || p->griddingMethod == 2, GENFIRE_ERR_ARG,
+             "GENFIRE: Unrecognized gridding method.");   // GENFIRE_Reconstructor.m:151-153
+    int n1 = 0, n2 = 0;
+    grid_edges(*p, dim1, dim2, &n1, &n2);
+    if (p->griddingMethod == 1)
+        GF_CHECK(n1 >= dim1 && n1 >= dim2, GENFIRE_ERR_ARG,   // My_paddzero.m:27-28
+                 "GENFIRE: paddedsize should be equal to or larger than original volume in all dimensions!");
     GF_CHECK(n1 >= dim1 && n2 >= dim2, GENFIRE_ERR_ARG, "GENFIRE: oversampled size smaller than projection");
     GF_CHECK(fft_supported(n1) && fft_supported(n2), GENFIRE_ERR_UNSUPPORTED,
              "GENFIRE: oversampled grid " + std::to_string(n1) + "x" + std::to_string(n2) +
@@ -877,8 +956,7 @@ void job_setup(genfire_job* j, genfire_ctx* ctx, const genfire_params* p, int di
     j->dim1 = dim1;
     j->dim2 = dim2;
     j->P = P;
-    j->n1 = (int)mround(dim1 * p->oversamplingRatio);   // :168
-    j->n2 = (int)mround(dim2 * p->oversamplingRatio);   // :169
+    grid_edges(*p, dim1, dim2, &j->n1, &j->n2);
     j->hg = half_grid(j->n1, j->n2, p->DFT_CentroSymmetricity ? 1 : 0);
     j->delays = delay_indicators(p->constraintEnforcementMode);
     GF_HIP(hipStreamCreateWithFlags(&j->st, hipStreamNonBlocking));
@@ -910,6 +988,15 @@ void job_set_inputs(genfire_job* j, const float* projections, const double* angl
         GF_HIP(hipMemcpyAsync(j->support_box.p, support, box_size(j), hipMemcpyHostToDevice, j->st));
     }
     j->have_init = initial_object != nullptr;
+    if (initial_object && j->prm.griddingMethod == 1) {
+        // reconstruct.m:79 pads the initial object to n1_oversampled x
+        // n2_oversampled x n1_oversampled, which must be the grid's size
+        const int o1 = (int)mround(j->dim1 * j->prm.oversamplingRatio);
+        const int o2 = (int)mround(j->dim2 * j->prm.oversamplingRatio);
+        GF_CHECK(o1 == j->n1 && o2 == j->n2, GENFIRE_ERR_ARG,
+                 "GENFIRE: initial object padded to " + std::to_string(o1) + "x" + std::to_string(o2) + "x" +
+                     std::to_string(o1) + " does not match the FFT-gridding grid of edge " + std::to_string(j->n1));
+    }
     if (initial_object) {
         j->init_box.alloc(sizeof(float) * box_size(j));
         GF_HIP(hipMemcpyAsync(j->init_box.p, initial_object, sizeof(float) * box_size(j), hipMemcpyHostToDevice,
@@ -928,14 +1015,18 @@ void job_set_inputs(genfire_job* j, const float* projections, const double* angl
 // support, best, monitor model, measured-voxel lists), plus the inputs.
 int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P)
 {
-    const int64_t n1 = (int64_t)mround(dim1 * p.oversamplingRatio), n2 = (int64_t)mround(dim2 * p.oversamplingRatio);
+    int e1 = 0, e2 = 0;
+    grid_edges(p, dim1, dim2, &e1, &e2);
+    const int64_t n1 = e1, n2 = e2;
     const int64_t n3 = n1, nf = n1 * n2 * n3, box = (int64_t)dim1 * dim2 * dim1;
     const int64_t half_grid_vox = (n1 / 2 + 1) * (n2 + 1) * (n3 + 1);
     const double cut = std::max(0.05, p.interpolationCutoffDistance);
     const int64_t pts = (int64_t)((1.3 * cut + 0.05) * (double)(n1 * n2) * P) + 4096;
     const int64_t kp = (dim1 + 63) / 64 * 64, np = (dim2 + 15) / 16 * 16;
     const int64_t inputs = (int64_t)4 * dim1 * dim2 * P + box * 5;
-    const int64_t grid = pts * 28 + kp * np * P * 4 + half_grid_vox * 36 + nf * (8 + 1 + 8) + box * 4;
+    const int64_t grid = p.griddingMethod == 1
+                             ? n1 * n1 * P * 8 + nf * (16 + 8 + 4 + 8 + 8) + box * 4   // spectra, d^3 sums, M, K
+                             : pts * 28 + kp * np * P * 4 + half_grid_vox * 36 + nf * (8 + 1 + 8) + box * 4;
     const bool half = half_supported((int)n1, (int)n2, (int)n3);
     const int64_t hn = (n1 / 2 + 1) * n2 * n3;
     const int64_t spectrum = 8 * (half ? std::max<int64_t>(2 * hn, nf) : nf);

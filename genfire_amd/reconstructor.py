@@ -9,7 +9,7 @@ Same property names, defaults and call order as
     G.InputAngles = angles              # P x 3, degrees
     G.griddingMethod = 2
     G = G.CheckPrepareData()
-    G = G.runGridding()                 # fillInFourierGrid_DFT on the GPU
+    G = G.runGridding()                 # fillInFourierGrid_DFT (2) / fillInFourierGrid (1) on the GPU
     G = G.reconstruct()                 # reconstruct.m on the GPU
     R, Rarr, simu = Tian_calc_Rfactor_realspace(G.reconstruction, projs, angles)
 
@@ -211,6 +211,17 @@ class GENFIRE_Reconstructor:
             raise GenfireError(-1, f"GENFIRE: ERROR! constraintEnforcementMode value {mode} not understood")
         self.n1_oversampled = _matlab_round(self.Dim1 * self.oversamplingRatio)
         self.n2_oversampled = _matlab_round(self.Dim2 * self.oversamplingRatio)
+        if self.griddingMethod == 1:
+            # fillInFourierGrid.m:38-42 sizes kMeasured from Dim2; zeros() needs a whole number
+            e = (self.Dim2 * self.oversamplingRatio if self.Dim2 % 2 == 0
+                 else 1 + (self.Dim2 - 1) * self.oversamplingRatio)
+            if e != int(e):
+                raise GenfireError(-1, "GENFIRE: FFT gridding needs a whole-number grid edge "
+                                       f"(Dim2*oversamplingRatio for even Dim2, 1+(Dim2-1)*oversamplingRatio "
+                                       f"for odd), got {e}")
+            if e < self.Dim1:
+                raise GenfireError(-1, "GENFIRE: paddedsize should be equal to or larger than original "
+                                       "volume in all dimensions!")
         return self
 
     def _params(self) -> _lib.GenfireParams:
@@ -236,6 +247,8 @@ class GENFIRE_Reconstructor:
         p.numBinsRfree = int(self.numBinsRfree)
         p.percentValuesForRfree = float(self.percentValuesForRfree)
         p.RandomSeed = int(self.RandomSeed) if self.RandomSeed is not None else 0
+        p.griddingMethod = int(self.griddingMethod)
+        p.allowMultipleGridMatches = int(bool(self.allowMultipleGridMatches))
         return p
 
     def _ensure_job(self):
@@ -247,12 +260,21 @@ class GENFIRE_Reconstructor:
     def runGridding(self):
         """GENFIRE_Reconstructor.m:172-186."""
         if self.griddingMethod == 1:
-            raise GenfireError(-5, "GENFIRE: FFT gridding (griddingMethod = 1) is not part of this "
-                                   "build; use griddingMethod = 2 (DFT gridding)")
+            return self.fillInFourierGrid()
         return self.fillInFourierGrid_DFT()
+
+    def fillInFourierGrid(self):
+        """fillInFourierGrid.m:27-168 (FFT gridding + hermitianSymmetrize,
+        runGridding :182-185) on the device; the grid is a cube of edge
+        Dim2*oversamplingRatio (odd Dim2: 1 + (Dim2-1)*oversamplingRatio)."""
+        return self._grid(1)
 
     def fillInFourierGrid_DFT(self):
         """fillInFourierGrid_DFT.m:85-310 (+ CentroSymmetricity, runGridding :182-185)."""
+        return self._grid(2)
+
+    def _grid(self, method: int):
+        self.griddingMethod = method
         if self._job is not None:
             self._job.close()
             self._job = None

@@ -76,6 +76,8 @@ typedef struct genfire_params {
     int32_t numBinsRfree;               /* 10    resolution shells (reconstruct.m:47) */
     double percentValuesForRfree;       /* 0.05  fraction withheld per shell (:58) */
     int64_t RandomSeed;                 /* 0     seed of the withheld-point permutation (:52-55) */
+    int32_t griddingMethod;             /* 1     (:45)  1 FFT gridding, 2 DFT gridding */
+    int32_t allowMultipleGridMatches;   /* 1     (:46)  FFT gridding: match every voxel within the cutoff */
 } genfire_params;
 
 /* Library version, major*10000 + minor*100 + patch. */
@@ -91,12 +93,16 @@ const char* genfire_last_error(const genfire_ctx* ctx);
 
 /* CheckPrepareData (GENFIRE_Reconstructor.m:115-171): validates the parameters,
  * sets n1_oversampled = round(Dim1*ratio), n2_oversampled = round(Dim2*ratio)
- * (:168-169) and the enforcement schedule (:156-165), and creates a job. */
+ * (:168-169) and the enforcement schedule (:156-165), and creates a job.  The
+ * Fourier grid is n1 x n2 x n1 for DFT gridding; FFT gridding builds a cube of
+ * edge d = Dim2*ratio (Dim2 even) or 1 + (Dim2-1)*ratio (Dim2 odd), which must
+ * be a whole number no smaller than Dim1 (fillInFourierGrid.m:38-42, 60). */
 int genfire_job_create(genfire_ctx* ctx, const genfire_params* p, int32_t dim1, int32_t dim2,
                        int32_t num_projs, genfire_job** out);
 int genfire_job_destroy(genfire_job* job);
 
-/* n1_oversampled, n2_oversampled and the half-grid extent (l1, l2, l3) of
+/* The grid edges (n1_oversampled, n2_oversampled for DFT gridding; d, d for
+ * FFT gridding) and the half-grid extent (l1, l2, l3) of
  * fillInFourierGrid_DFT.m:134-136. */
 int genfire_job_sizes(const genfire_job* job, int32_t* n1, int32_t* n2, int64_t half_len[3]);
 
@@ -115,20 +121,27 @@ int genfire_job_set_inputs(genfire_job* job, const float* projections, const dou
 int genfire_job_set_inputs_f16(genfire_job* job, const uint16_t* projections_f16, const float* proj_scale,
                                const double* angles, const uint8_t* support, const float* initial_object);
 
-/* runGridding with griddingMethod = 2 (GENFIRE_Reconstructor.m:172-186):
- * fillInFourierGrid_DFT + CentroSymmetricity + single cast + measuredK_mask. */
+/* runGridding (GENFIRE_Reconstructor.m:172-186).  griddingMethod = 2:
+ * fillInFourierGrid_DFT + CentroSymmetricity; griddingMethod = 1:
+ * fillInFourierGrid (FFT of each padded projection, rotated nearest-voxel
+ * inverse-distance average, hermitianSymmetrize); then the single cast and
+ * measuredK_mask. */
 int genfire_job_grid(genfire_job* job);
 
 /* Gridded point list in projection order (the master lists of
  * fillInFourierGrid_DFT.m:257-261): 0-based half-grid linear index, projection
  * (0-based), distance D, u/n1, v/n2 and the DFT value (interleaved complex).
- * Any output pointer may be NULL. */
+ * Any output pointer may be NULL.  DFT gridding only (GENFIRE_ERR_STATE after
+ * FFT gridding, which forms no point list). */
 int genfire_job_grid_count(const genfire_job* job, int64_t* num_points);
 int genfire_job_get_points(genfire_job* job, int64_t* ind, int32_t* proj, float* dist, float* un,
                            float* vn, float* values);
 
 /* measuredK (n1 x n2 x n1 interleaved complex, centred as in the reference) and
- * the number of gridded points that landed on each half-grid voxel. */
+ * the number of gridded points that landed on each half-grid voxel (DFT
+ * gridding), or on each voxel of the centred d x d x d grid before
+ * hermitianSymmetrize (FFT gridding: the entries of masterInd per voxel,
+ * fillInFourierGrid.m:139). */
 int genfire_job_get_measuredK(genfire_job* job, float* measuredK);
 int genfire_job_get_hits(genfire_job* job, int32_t* hits);
 /* recIFFT (Dim1 x Dim2 x Dim1), fillInFourierGrid_DFT.m:304. */

@@ -11,6 +11,9 @@ reference repository):
   (index pass in the FP32 reference order via ``genfire_oracle.c``; values as an
   FP64 direct DFT; FP64 weighted average, :267-291)
 * ``@GENFIRE_Reconstructor/CentroSymmetricity.m:2-72``       -> :func:`centro_symmetricity`
+* ``@GENFIRE_Reconstructor/fillInFourierGrid.m:27-163``       -> :func:`grid_fft`
+  (FFT gridding; ``src/hermitianSymmetrize.m`` -> :func:`hermitian_symmetrize`,
+  ``src/my_fft.m`` -> :func:`my_fft`)
 * ``@GENFIRE_Reconstructor/GENFIRE_Reconstructor.m:115-186`` -> :func:`delay_indicators`,
   :func:`oversampled_size`, :func:`volumn_index`
 * ``@GENFIRE_Reconstructor/reconstruct.m:28-194``            -> :func:`reconstruct`
@@ -403,6 +406,140 @@ def grid_dft(projections: np.ndarray, angles: np.ndarray, n1: int, n2: int, cuto
 def rec_ifft(measuredK: np.ndarray, dim1: int, dim2: int) -> np.ndarray:
     """recIFFT = My_stripzero(real(my_ifft(measuredK)), [Dim1 Dim2 Dim1]), :304."""
     return stripzero(np.real(my_ifft(measuredK)), (dim1, dim2, dim1))
+
+
+# ---------------------------------------------------------------------------
+# FFT gridding (griddingMethod = 1)
+# ---------------------------------------------------------------------------
+
+def sincosd(x: float):
+    """MATLAB sind/cosd: the angle reduced in degrees (nearest multiple of 90),
+    so multiples of 90 give exact 0 and +-1."""
+    r = math.fmod(float(x), 360.0)
+    q = round(r / 90.0)
+    rem = r - 90.0 * q
+    sr, cr = math.sin(rem * math.pi / 180.0), math.cos(rem * math.pi / 180.0)
+    return [(sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr)][int(q) % 4]
+
+
+def fft_grid_rotation(phi: float, theta: float, psi: float) -> np.ndarray:
+    """fillInFourierGrid.m:80-84 (the RELION/XMIPP Euler matrix), FP64, R[i, j] = row i.
+    Evaluated left to right as written (no fused multiply-add)."""
+    sp, cp = sincosd(phi)
+    st, ct = sincosd(theta)
+    ss, cs = sincosd(psi)
+    return np.array([
+        [cs * ct * cp - ss * sp, cs * ct * sp + ss * cp, -cs * st],
+        [-ss * ct * cp - cs * sp, -ss * ct * sp + cs * cp, ss * st],
+        [st * cp, st * sp, ct]])
+
+
+def fft_grid_edge(dim2: int, ratio: float) -> int:
+    """fillInFourierGrid.m:38-42: kMeasured is d x d with d = Dim2*ratio (even Dim2)
+    or 1 + (Dim2-1)*ratio (odd), particleWindowSize(2) = Dim2 (GENFIRE_Reconstructor.m:121)."""
+    e = dim2 * ratio if dim2 % 2 == 0 else 1 + (dim2 - 1) * ratio
+    if e != math.floor(e):
+        raise ValueError(f"grid edge {e} is not a whole number")
+    return int(e)
+
+
+def my_fft(img: np.ndarray) -> np.ndarray:
+    """src/my_fft.m:10-12: fftshift(fftn(ifftshift(img))) (FP64)."""
+    return np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(img)))
+
+
+def hermitian_symmetrize(a: np.ndarray) -> np.ndarray:
+    """src/hermitianSymmetrize.m:22-65, literally: zero-pad each even edge by one
+    (at the end), average with the conjugate of the all-axes flip, dividing by the
+    number of nonzero entries of the pair, crop back.  FP64."""
+    shp = a.shape
+    ext = tuple(n + 1 if n % 2 == 0 else n for n in shp)
+    inp = np.zeros(ext, np.complex128)
+    inp[: shp[0], : shp[1], : shp[2]] = a
+    flip = inp[::-1, ::-1, ::-1]
+    out = inp + np.conj(flip)
+    cnt = (inp != 0).astype(np.float64)
+    cnt = cnt + cnt[::-1, ::-1, ::-1]
+    nz = cnt != 0
+    out[nz] = out[nz] / cnt[nz]
+    return out[: shp[0], : shp[1], : shp[2]]
+
+
+@dataclass
+class FftGridResult:
+    d: int
+    master_ind: np.ndarray      # 0-based linear index into the centred d^3 grid, list order
+    master_dist: np.ndarray     # distances (single)
+    master_val: np.ndarray      # kMeasured values (complex128)
+    hits: np.ndarray            # entries of masterInd per voxel (d^3, int32, Fortran order)
+    measuredK: np.ndarray       # centred d^3, complex64 (after hermitianSymmetrize and the single cast)
+
+
+def grid_fft(projections: np.ndarray, angles: np.ndarray, ratio: float, cutoff: float,
+             allow_multiple: bool = True) -> FftGridResult:
+    """fillInFourierGrid.m:27-163 + runGridding's single cast (GENFIRE_Reconstructor.m:182).
+
+    Point-centric and literal: the rotated lattice of every projection
+    (rotkCoords = R'*[kx;ky;kz] in single, products rounded and summed left to
+    right), the shift loops Y, X, Z (:125-143) appending (index, value,
+    distance) in list order, weights 1/(D + 1e-5) in single, FP64 accumarray
+    (:153-160), then :func:`hermitian_symmetrize`.  The spectra are FP64 FFTs of
+    the single padded projections (:59-61).  Relational tests of single against
+    double compare exactly."""
+    dim1, dim2, P = projections.shape
+    d = fft_grid_edge(dim2, ratio)
+    nc = int(matlab_round((d + 1) / 2))                                  # :48
+    r = (np.arange(1, d + 1) - nc).astype(np.float32)
+    kx = np.tile(r, d)                                                   # :52-53 (rows fastest)
+    ky = np.repeat(r, d)
+    kz = np.zeros(d * d, np.float32)
+    mX, mY, mZ, vals = [], [], [], []
+    for p in range(P):
+        spec = my_fft(paddzero(np.asarray(projections[:, :, p], np.float32), (d, d)))   # :59-61
+        Rt = fft_grid_rotation(*angles[p]).T.astype(np.float32)         # R' in single (:87)
+        m = [(Rt[i, 0] * kx + Rt[i, 1] * ky) + Rt[i, 2] * kz for i in range(3)]
+        mX.append(m[0]); mY.append(m[1]); mZ.append(m[2])
+        vals.append(spec.ravel(order="F"))
+    mX, mY, mZ = np.concatenate(mX), np.concatenate(mY), np.concatenate(mZ)
+    kM = np.concatenate(vals)
+    good = ~(kM == -999)                                                 # :108-113
+    mX, mY, mZ, kM = mX[good], mY[good], mZ[good], kM[good]
+    S = int(matlab_round(cutoff)) if allow_multiple else 0               # :117-121
+    rX = matlab_round(mX).astype(np.float32)
+    rY = matlab_round(mY).astype(np.float32)
+    rZ = matlab_round(mZ).astype(np.float32)
+    inds, dists, mv = [], [], []
+    for ys in range(-S, S + 1):
+        for xs in range(-S, S + 1):
+            for zs in range(-S, S + 1):
+                tX = rX + np.float32(xs)
+                tY = rY + np.float32(ys)
+                tZ = rZ + np.float32(zs)
+                dx, dy, dz = np.abs(mX - tX), np.abs(mY - tY), np.abs(mZ - tZ)
+                dist = np.sqrt((dx * dx + dy * dy) + dz * dz)           # :132 (single)
+                iX, iY, iZ = tX + nc, tY + nc, tZ + nc
+                ok = ~((iX > d) | (iX < 1) | (iY > d) | (iY < 1) | (iZ > d) | (iZ < 1))
+                ok &= dist.astype(np.float64) <= cutoff                 # :136
+                ind = ((iX[ok] - 1).astype(np.int64) + d * ((iY[ok] - 1).astype(np.int64)
+                                                            + d * (iZ[ok] - 1).astype(np.int64)))
+                inds.append(ind); dists.append(dist[ok]); mv.append(kM[ok])
+    master_ind = np.concatenate(inds)
+    master_dist = np.concatenate(dists)
+    master_val = np.concatenate(mv)
+    w = (master_dist + np.float32(1e-5)).astype(np.float32)              # :153
+    w = (np.float32(1.0) / w).astype(np.float32)                         # :154
+    wd = w.astype(np.float64)
+    tot = d ** 3
+    acc = (np.bincount(master_ind, weights=master_val.real * wd, minlength=tot)
+           + 1j * np.bincount(master_ind, weights=master_val.imag * wd, minlength=tot))   # :157
+    sw = np.bincount(master_ind, weights=wd, minlength=tot)             # :158
+    hits = np.bincount(master_ind, minlength=tot).astype(np.int32)
+    K = np.zeros(tot, np.complex128)
+    pos = sw > 0
+    K[pos] = acc[pos] / sw[pos]                                          # :159
+    K = hermitian_symmetrize(K.reshape(d, d, d, order="F"))              # :163
+    return FftGridResult(d=d, master_ind=master_ind, master_dist=master_dist, master_val=master_val,
+                         hits=hits.reshape(d, d, d, order="F"), measuredK=K.astype(np.complex64))
 
 
 # ---------------------------------------------------------------------------

@@ -32,6 +32,7 @@ def test_defaults_mirror_reference_properties():
     assert p.numIterations == 50 and p.oversamplingRatio == 3 and p.interpolationCutoffDistance == 0.7
     assert p.constraintEnforcementMode == 1 and p.constraintPositivity == 1 and p.constraintSupport == 1
     assert p.DFT_CentroSymmetricity == 1 and p.monitor_R == 0 and p.monitorR_loopLength == 10
+    assert p.griddingMethod == 1 and p.allowMultipleGridMatches == 1   # :45-46
     assert list(p.vector1) == [0, 0, 1] and list(p.vector2) == [0, 1, 0] and list(p.vector3) == [1, 0, 0]
 
 

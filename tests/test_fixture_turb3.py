@@ -44,6 +44,7 @@ def test_gpu_grid_turb3_bitexact(ctx, O, turb3):
     print("[turb3] gridding 64x64x41 at oversampling 4", flush=True)
     pj, ang = turb3["projections"], turb3["angles"]
     p = _lib.default_params()
+    p.griddingMethod = 2   # DFT gridding
     p.oversamplingRatio = 4
     p.interpolationCutoffDistance = 0.125
     p.constraintEnforcementMode = 3

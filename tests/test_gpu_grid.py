@@ -15,6 +15,7 @@ VALUE_TOL = 1e-5
 
 def _params(iters=5, ratio=2.0, cutoff=0.3, centro=1, v3=(1, 0, 0)):
     p = _lib.default_params()
+    p.griddingMethod = 2   # DFT gridding
     p.numIterations = iters
     p.oversamplingRatio = ratio
     p.interpolationCutoffDistance = cutoff

@@ -33,6 +33,7 @@ def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None):
 def _job(ctx, mk, projs, ang, sup, iters, mode=3, alg="genfire", monitor=0, L=5):
     d1, d2, P = projs.shape
     p = _lib.default_params()
+    p.griddingMethod = 2   # DFT gridding
     p.numIterations = iters
     p.oversamplingRatio = mk.shape[0] / d1
     p.interpolationCutoffDistance = 0.3
@@ -136,6 +137,7 @@ def test_rfree_matches_oracle(ctx, O, alg, shape):
     print(f"[iter] Rfree alg={alg} dims={shape}", flush=True)
     vol, projs, ang, mk, sup = _setup(O, dim=d1, dim2=d2, P=13, seed=5)
     p = _lib.default_params()
+    p.griddingMethod = 2   # DFT gridding
     p.numIterations = iters
     p.oversamplingRatio = mk.shape[0] / d1
     p.interpolationCutoffDistance = 0.3

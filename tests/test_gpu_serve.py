@@ -31,6 +31,7 @@ def _phantom(dim, P, seed, single_axis):
 
 def _params(iters, dr, monitor, prec=_lib.PREC_FP32):
     p = _lib.default_params()
+    p.griddingMethod = 2   # DFT gridding
     p.numIterations = iters
     p.oversamplingRatio = 2.0
     p.interpolationCutoffDistance = 0.3

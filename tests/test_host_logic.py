@@ -61,11 +61,23 @@ def test_set_parameters():
         G.set_parameters("noSuchProperty", 1)
 
 
-def test_fft_gridding_is_refused():
+def test_fft_gridding_edge_rules():
+    """fillInFourierGrid.m:38-42: the FFT-gridding grid edge comes from Dim2 and
+    must be a whole number no smaller than Dim1 (My_paddzero.m:27-28)."""
     G = _obj()
-    G.CheckPrepareData()
-    with pytest.raises(GenfireError, match="griddingMethod"):
-        G.runGridding()
+    G.griddingMethod = 1
+    G.oversamplingRatio = 2.5
+    G.InputProjections = np.zeros((16, 15, 3), np.float32)
+    G.InputAngles = np.zeros((3, 3))
+    G.CheckPrepareData()                         # odd Dim2: 1 + 14*2.5 = 36
+    G.InputProjections = np.zeros((16, 16, 3), np.float32)
+    G.oversamplingRatio = 2.1
+    with pytest.raises(GenfireError, match="whole-number grid edge"):
+        G.CheckPrepareData()                     # 16*2.1 = 33.6
+    G.oversamplingRatio = 1
+    G.InputProjections = np.zeros((20, 16, 3), np.float32)
+    with pytest.raises(GenfireError, match="paddedsize"):
+        G.CheckPrepareData()
 
 
 def test_volumn_index_matches_oracle(O):
