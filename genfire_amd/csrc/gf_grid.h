@@ -162,6 +162,16 @@ struct HalfIterArgs {
     float* rf_terms;   // |k - value| per entry, written by the z pass
     int n1, n2, n3;
     int positivity, support, dr;
+    // Real-space box (x, y, z extents, centred in FFT order) outside which the
+    // iterate's transform input is zero; the passes skip rows that lie outside
+    // it.  Equal to (n1, n2, n3) when nothing is pruned.
+    int bx, by, bz;
+    // DR on the box (dmode): the replacement delta D of each list entry (then
+    // its Hermitian part on the kx = 0 and n1/2 planes), and for the entries of
+    // those two planes the entry index of their in-plane mate
+    int dmode;
+    float2* csr_d;
+    const int64_t* csr_mate;
 };
 // Rfree[s + nbins * (*ctr)] = sum of terms of shell s (entries order[shell_off[s]
 // .. shell_off[s+1]), fixed order) / den[s], or -1 for an empty shell; *ctr is
@@ -172,6 +182,7 @@ constexpr uint32_t kHalfTermA = 1u << 28;     // voxel measured: errK term |k| v
 constexpr uint32_t kHalfTermB = 1u << 29;     // mate measured: errK term |k| vs mag.y
 constexpr uint32_t kHalfAverage = 1u << 30;   // only one of the pair measured: k = (V + k)/2
 constexpr uint32_t kHalfEnforced = 1u << 31;  // replaced at the current schedule stage
+constexpr uint32_t kHalfPassive = 1u << 27;   // kx = 0, n1/2 planes: unmeasured voxel whose mate is measured
 bool half_supported(int n1, int n2, int n3);
 int64_t half_partials(int n1, int n2, int n3);
 // Up to kHalfBatchMax jobs of one grid shape in one launch (launch row
@@ -186,6 +197,12 @@ void half_y(const HalfBatch& b, int inverse, hipStream_t st);
 void half_z(const HalfBatch& b, hipStream_t st);
 void half_last(const HalfBatch& b, hipStream_t st);
 void half_last_first(const HalfBatch& b, hipStream_t st);
+// dmode jobs: D -> its Hermitian part on the kx = 0 and n1/2 planes (after half_z)
+void half_dherm(const HalfBatch& b, hipStream_t st);
+void half_dherm(const HalfIterArgs& a, hipStream_t st);
+// dmode set-up: in-plane mate entry of every kx = 0 and n1/2 plane entry
+void hcsr_mates(const int64_t* d_off, const uint32_t* d_meta, int n1, int n2, int n3, int64_t n_plane0,
+                int64_t plane_m_begin, int64_t n_plane, int64_t* d_mate, hipStream_t st);
 void half_first(const HalfIterArgs& a, hipStream_t st);
 void half_y(const HalfIterArgs& a, int inverse, hipStream_t st);
 void half_z(const HalfIterArgs& a, hipStream_t st);

@@ -25,6 +25,24 @@
 //   P4 y: B (strided) -> IFFT -> A
 //   P5 x: A (strided) -> Re of kx = 0, n1/2 -> C2R -> x (GENFIRE) or DR update
 // Between two iterations P5 and the next P1 run as one pass (k_half_last_first).
+//
+// Box pruning.  With the support constraint on, P_R(x) vanishes outside the
+// centred Dim1 x Dim2 x Dim1 box, and only the box part of P_F's output is ever
+// used again (the next P_R, the monitor model, the returned P_R(x)).  So every
+// pass skips real-space rows outside the box: P1/P5 only touch rows (y, z) in
+// the box, P2 reads only y in the box and only z in the box, P3 reads and writes
+// only z in the box, P4 writes only y in the box.  Frequencies are never pruned.
+// GENFIRE's transform input P_R(x) lies in the box, so its passes compute the
+// same numbers as without pruning.  DR's input 2 P_R(x) - x does not: there the
+// iterate is carried in the form x_{k+1} = P_R(x_k) + d_k, d_k = real(ifftn(D_k))
+// with D_k = k' - k the replacement delta (nonzero only on the measured-voxel
+// lists), which follows from x_{k+1} = x_k + P_F(y_k) - P_R(x_k) and P_F(y_k) =
+// y_k + d_k.  Then fftn(2 P_R(x_{k+1}) - x_{k+1}) = fftn(2 P_R(x_{k+1}) -
+// P_R(x_k)) - fftn(d_k): a box-supported transform minus the Hermitian part D'_k
+// of D_k (D_k itself off the kx = 0, n1/2 planes, where C2R then R2C is the
+// identity; (D(s) + conj D(-s))/2 on them).  The z pass subtracts D' at the list
+// entries, forms the new D there, and transforms D, not k', back.
+#include <algorithm>
 #include <cmath>
 
 #include "gf_fft_core.h"
@@ -50,6 +68,55 @@ __device__ __forceinline__ float prh(float v, uint8_t s, int positivity, int sup
 }
 
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// FFT-order index s of an axis of length n lies in the centred box of extent b
+// (signed offsets -b/2 .. b-b/2-1): s < b - b/2 or s >= n - b/2.
+__host__ __device__ __forceinline__ bool inb(int s, int n, int b) { return s < b - b / 2 || s >= n - b / 2; }
+
+// The tiles of ts consecutive indices along an axis of length n that meet the
+// box: [0, lo_end) and [hi_begin, n/ts); all n/ts when the two runs meet.
+struct BoxTiles {
+    int lo_end, hi_begin, count;
+};
+__host__ __device__ __forceinline__ BoxTiles box_tiles(int n, int b, int ts)
+{
+    const int nt = n / ts, lo_end = (b - b / 2 + ts - 1) / ts, hi_begin = (n - b / 2) / ts;
+    if (lo_end >= hi_begin) return {nt, nt, nt};
+    return {lo_end, hi_begin, lo_end + (nt - hi_begin)};
+}
+__host__ __device__ __forceinline__ int box_tile(const BoxTiles& t, int i)
+{
+    return i < t.lo_end ? i : t.hi_begin + (i - t.lo_end);
+}
+
+// Tile of an x pass (P1/P5): one z in the box, 16 consecutive y rows that meet
+// it; cy = the launch's y-tile count (the largest of its jobs).
+__device__ __forceinline__ bool x_tile(const HalfIterArgs& a, int cy, int& i3, int& i2_0)
+{
+    const BoxTiles ty = box_tiles(a.n2, a.by, kHP), tz = box_tiles(a.n3, a.bz, 1);
+    const int zi = (int)blockIdx.x / cy, yt = (int)blockIdx.x % cy;
+    if (zi >= tz.count || yt >= ty.count) return false;
+    i3 = box_tile(tz, zi);
+    i2_0 = box_tile(ty, yt) * kHP;
+    return true;
+}
+
+// Tile of a y pass (P2/P4): one kx, 16 consecutive z that meet the box.
+__device__ __forceinline__ bool y_tile(const HalfIterArgs& a, int cz, int& kx, int& i3_0)
+{
+    const BoxTiles tz = box_tiles(a.n3, a.bz, kHP);
+    kx = (int)blockIdx.x / cz;
+    const int zt = (int)blockIdx.x % cz;
+    if (zt >= tz.count) return false;
+    i3_0 = box_tile(tz, zt) * kHP;
+    return true;
+}
+
+// Which elements of the real pair (xc, xc+1) of a row lie in the box (bit 0, 1).
+__device__ __forceinline__ int pair_in(bool row_in, int xc, int n, int b)
+{
+    return row_in ? ((inb(xc, n, b) ? 1 : 0) | (inb(xc + 1, n, b) ? 2 : 0)) : 0;
+}
 
 // ---------------------------------------------------------------------------
 // P1: P_R and the real-to-complex x transform.  N = n1, complex length M = N/2.
@@ -102,6 +169,7 @@ __device__ __forceinline__ void r2c_store(const HalfIterArgs& a, float2* lds, fl
     const int pp = t % kHP, c = t / kHP;
     const float2* rp = lds + pp * M;
     float2* out = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
+    if (!inb(i2_0 + pp, n2, a.by)) return;   // a row outside the box is zero and is never read
     auto split = [&](int kx) {
         const float2 zk = rp[swz(kx == M ? 0 : kx, pp)];          // Z[k mod M]
         const float2 zm = conjf2(rp[swz(kx == 0 ? 0 : M - kx, pp)]);  // conj Z[(M - k) mod M]
@@ -119,16 +187,33 @@ __device__ __forceinline__ void r2c_store(const HalfIterArgs& a, float2* lds, fl
 
 // The value P1 transforms for the real pair xv: P_R(x) (GENFIRE) or the
 // reflection 2 P_R(x) - x (DR); P_R(x) also goes to best when that is set.
-__device__ __forceinline__ float2 p1_value(const HalfIterArgs& a, float2 xv, uint16_t s2, int64_t ad)
+__device__ __forceinline__ float2 p1_value(const HalfIterArgs& a, float2 xv, uint16_t s2, int64_t ad, bool wr = true)
 {
     const float ra = prh(xv.x, (uint8_t)(s2 & 0xFF), a.positivity, a.support);
     const float rb = prh(xv.y, (uint8_t)(s2 >> 8), a.positivity, a.support);
-    if (a.best) *reinterpret_cast<float2*>(a.best + ad) = make_float2(ra, rb);
+    if (a.best && wr) *reinterpret_cast<float2*>(a.best + ad) = make_float2(ra, rb);
     return a.dr ? make_float2(2.0f * ra - xv.x, 2.0f * rb - xv.y) : make_float2(ra, rb);
 }
 
+// Loads of the real pair at ad restricted to the box: x (when needed) and the
+// support bytes; elements outside the box read as zero.
+__device__ __forceinline__ void load_pair(const HalfIterArgs& a, int64_t ad, int in, bool need_x, float2& xv,
+                                          uint16_t& s2)
+{
+    xv = make_float2(0.0f, 0.0f);
+    s2 = 0;
+    if (!in) return;
+    if (need_x) {
+        xv = *reinterpret_cast<const float2*>(a.x + ad);
+        if (!(in & 1)) xv.x = 0.0f;
+        if (!(in & 2)) xv.y = 0.0f;
+    }
+    s2 = *reinterpret_cast<const uint16_t*>(a.S + ad);
+}
+
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch hb, const float2* __restrict__ twM,
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch hb, int cy,
+                                                                      const float2* __restrict__ twM,
                                                                       const float2* __restrict__ twN)
 {
     const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
@@ -138,23 +223,24 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch 
     extern __shared__ float2 lds[];
     const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = a.n2;
-    const int tpr = n2 / kHP;
-    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
+    int i3, i2_0;
+    if (!x_tile(a, cy, i3, i2_0)) return;
     const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
+    const bool row_in = inb(i2_0 + p, n2, a.by);
 
     // All loads are issued before any store (best may be written), so they are
     // in flight together.
     float2 xv[E];
     uint16_t s2[E];
+    int in[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        const int64_t ad = row + 2 * (j + T * s);
-        xv[s] = *reinterpret_cast<const float2*>(a.x + ad);
-        s2[s] = *reinterpret_cast<const uint16_t*>(a.S + ad);
+        in[s] = pair_in(row_in, 2 * (j + T * s), N, a.bx);
+        load_pair(a, row + 2 * (j + T * s), in[s], true, xv[s], s2[s]);
     }
     float2 v[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s) v[in_slot<M>(s)] = p1_value(a, xv[s], s2[s], row + 2 * (j + T * s));
+    for (int s = 0; s < E; ++s) v[in_slot<M>(s)] = p1_value(a, xv[s], s2[s], row + 2 * (j + T * s), in[s] != 0);
     r2c_store<N>(a, lds, v, i2_0, i3, twM, twN);
 }
 
@@ -162,7 +248,7 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch 
 // P2: y forward, A -> B.  Tile: one kx, 16 consecutive z.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, const float2* __restrict__ tw)
+__global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, int cz, const float2* __restrict__ tw)
 {
     const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     using PL = RegPlan<N>;
@@ -170,14 +256,18 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
     extern __shared__ float2 lds[];
     const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = N, n3 = a.n3, h = a.n1 / 2 + 1;
-    const int tpr = n3 / kHP;
-    const int kx = blockIdx.x / tpr, i3_0 = (blockIdx.x % tpr) * kHP;
+    int kx, i3_0;
+    if (!y_tile(a, cz, kx, i3_0)) return;
     const float2* src = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p));
+    const bool z_in = inb(i3_0 + p, n3, a.bz);
     float2 v[E];
 #pragma unroll
     for (int i = 0; i < E / R0; ++i)
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v[i * R0 + r] = src[j + T * i + r * (N / R0)];
+        for (int r = 0; r < R0; ++r) {
+            const int y = j + T * i + r * (N / R0);
+            v[i * R0 + r] = (z_in && inb(y, N, a.by)) ? src[y] : make_float2(0.0f, 0.0f);
+        }
     float2* rowl = lds + p * N;
     reg_fft<N, false>(v, rowl, j, p, tw);
 #pragma unroll
@@ -218,7 +308,10 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
 #pragma unroll
     for (int i = 0; i < E / R0; ++i)
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v[i * R0 + r] = pen[j + T * i + r * (N / R0)];
+        for (int r = 0; r < R0; ++r) {
+            const int z = j + T * i + r * (N / R0);
+            v[i * R0 + r] = inb(z, N, a.bz) ? pen[z] : make_float2(0.0f, 0.0f);
+        }
     // The tile's list range and each lane's first entry are loaded before the
     // forward transform, which hides their latency (a tile has fewer entries
     // than lanes in most of the grid).
@@ -239,6 +332,19 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
 #pragma unroll
         for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
     __syncthreads();
+    // DR on the box: the transform of the box part minus D' of the last
+    // iteration is the transform of the iterate's reflection
+    if (a.dmode) {
+        for (int e = e0; e < end; e += NT) {
+            const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
+            const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
+            float2* slot = lds + pl * N + swz(z, pl);
+            const float2 dv = a.csr_d[e];
+            const float2 kv = *slot;
+            *slot = make_float2(kv.x - dv.x, kv.y - dv.y);
+        }
+        __syncthreads();
+    }
     // Rfree terms (reconstruct.m:137-147) on the transform before replacement
     if (a.rf_off) {
         const int rb = (int)a.rf_off[r0], re = (int)a.rf_off[r0 + kZP];
@@ -271,8 +377,11 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             const float ak = sqrtf(kv.x * kv.x + kv.y * kv.y);
             if (m & kHalfTermA) err += fabsf(ak - mg.x);
             if (m & kHalfTermB) err += fabsf(ak - mg.y);
+            float2 kn = kv;
             if (m & kHalfEnforced)
-                *slot = (m & kHalfAverage) ? make_float2(0.5f * (V.x + kv.x), 0.5f * (V.y + kv.y)) : V;
+                kn = (m & kHalfAverage) ? make_float2(0.5f * (V.x + kv.x), 0.5f * (V.y + kv.y)) : V;
+            if (a.dmode) a.csr_d[e] = make_float2(kn.x - kv.x, kn.y - kv.y);   // D = k' - k
+            else if (m & kHalfEnforced) *slot = kn;
         }
         double de = (double)err;
         for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off, 64);
@@ -284,6 +393,17 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             for (int w = 0; w < NT / 64; ++w) sum += red[w];
             a.err_part[blockIdx.x] = sum;
         }
+    }
+    // DR on the box: the inverse transform takes the sparse D
+    if (a.dmode) {
+        for (int i = t; i < kZP * N; i += NT) lds[i] = make_float2(0.0f, 0.0f);
+        __syncthreads();
+        for (int e = e0; e < end; e += NT) {
+            const uint32_t m = a.csr_meta[e];
+            const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
+            lds[pl * N + swz(z, pl)] = a.csr_d[e];
+        }
+        __syncthreads();
     }
     // the inverse transform takes fresh lane indices and its own twiddle
     // pointer (the same table) so that nothing of the forward one stays live
@@ -297,14 +417,17 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
-        for (int q = 0; q < RL; ++q) pen2[j2 + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
+        for (int q = 0; q < RL; ++q) {
+            const int z = j2 + T * i + q * (N / RL);
+            if (inb(z, N, a.bz)) pen2[z] = cscale(v[i * RL + q], sc);
+        }
 }
 
 // ---------------------------------------------------------------------------
 // P4: y inverse, B -> A.  Tile: one kx, 16 consecutive z (strided read).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, const float2* __restrict__ tw)
+__global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, int cz, const float2* __restrict__ tw)
 {
     const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     using PL = RegPlan<N>;
@@ -312,15 +435,16 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     extern __shared__ float2 lds[];
     const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = N, n3 = a.n3, h = a.n1 / 2 + 1;
-    const int tpr = n3 / kHP;
-    const int kx = blockIdx.x / tpr, i3_0 = (blockIdx.x % tpr) * kHP;
+    int kx, i3_0;
+    if (!y_tile(a, cz, kx, i3_0)) return;
     {
         const int pp = t % kHP, c = t / kHP;
         float2* rp = lds + pp * N;
         const float2* src = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
+        const bool z_in = inb(i3_0 + pp, n3, a.bz);
         float2 w[N / T];
 #pragma unroll
-        for (int q = 0; q < N / T; ++q) w[q] = src[(int64_t)n3 * (c + q * T)];
+        for (int q = 0; q < N / T; ++q) w[q] = z_in ? src[(int64_t)n3 * (c + q * T)] : make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int q = 0; q < N / T; ++q) rp[swz(c + q * T, pp)] = w[q];
     }
@@ -333,10 +457,14 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     reg_fft<N, true>(v, rowl, j2, p2, tw);
     float2* dst = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p2));
     const float sc = 1.0f / (float)N;
+    if (!inb(i3_0 + p2, n3, a.bz)) return;
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
-        for (int q = 0; q < RL; ++q) dst[j2 + T * i + q * (N / RL)] = cscale(v[i * RL + q], sc);
+        for (int q = 0; q < RL; ++q) {
+            const int y = j2 + T * i + q * (N / RL);
+            if (inb(y, N, a.by)) dst[y] = cscale(v[i * RL + q], sc);
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -360,13 +488,14 @@ __device__ __forceinline__ void load_c2r(const HalfIterArgs& a, float2* lds, flo
         const int pp = t % kHP, c = t / kHP;
         float2* rp = lds + pp * M;
         const float2* src = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
+        const bool y_in = inb(i2_0 + pp, n2, a.by);   // rows outside the box are not formed
         float2 w[M / T];
 #pragma unroll
-        for (int q = 0; q < M / T; ++q) w[q] = src[(int64_t)n2 * (c + q * T)];
+        for (int q = 0; q < M / T; ++q) w[q] = y_in ? src[(int64_t)n2 * (c + q * T)] : make_float2(0.0f, 0.0f);
         if (c == 0) w[0].y = 0.0f;                 // Re of the kx = 0 bin
 #pragma unroll
         for (int q = 0; q < M / T; ++q) rp[swz(c + q * T, pp)] = w[q];
-        if (c == 0) nyq[pp] = make_float2(src[(int64_t)n2 * M].x, 0.0f);   // Re of kx = M
+        if (c == 0) nyq[pp] = make_float2(y_in ? src[(int64_t)n2 * M].x : 0.0f, 0.0f);   // Re of kx = M
     }
     __syncthreads();
     float2* rowl = lds + p * M;
@@ -402,8 +531,25 @@ __device__ __forceinline__ float2 p5_update(const HalfIterArgs& a, float2 pf, fl
     return xn;
 }
 
+// DR on the box, one real pair: d = real(ifftn(D)) of the last replacement,
+// r = P_R(x_k), x_{k+1} = r + d, pf = P_F(y_k) = (2 r - x_k) + d.  Returns
+// x_{k+1}; r goes to *rk.
+__device__ __forceinline__ float2 p5_dmode(const HalfIterArgs& a, float2 d, float2 xv, uint16_t s2, int64_t ad,
+                                          float2* rk)
+{
+    const float ra = prh(xv.x, (uint8_t)(s2 & 0xFF), a.positivity, a.support);
+    const float rb = prh(xv.y, (uint8_t)(s2 >> 8), a.positivity, a.support);
+    if (a.pf_out)
+        *reinterpret_cast<float2*>(a.pf_out + ad) = make_float2((2.0f * ra - xv.x) + d.x, (2.0f * rb - xv.y) + d.y);
+    const float2 xn = make_float2(ra + d.x, rb + d.y);
+    *reinterpret_cast<float2*>(a.x + ad) = xn;
+    *rk = make_float2(ra, rb);
+    return xn;
+}
+
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch hb, const float2* __restrict__ twM,
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch hb, int cy,
+                                                                     const float2* __restrict__ twM,
                                                                      const float2* __restrict__ twN)
 {
     const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
@@ -411,25 +557,34 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch h
     using PL = RegPlan<M>;
     constexpr int E = PL::E, T = M / E;
     extern __shared__ float2 lds[];
-    const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = a.n2;
-    const int tpr = n2 / kHP;
-    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
+    int i3, i2_0;
+    if (!x_tile(a, cy, i3, i2_0)) return;
     float2 v[E];
     load_c2r<N>(a, lds, v, i2_0, i3, twM, twN);
+    const int t = phase_tid(), j = t % T, p = t / T;
     const float sc = 1.0f / (float)M;
     const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
+    const bool row_in = inb(i2_0 + p, n2, a.by);
     float2 xv[E];
     uint16_t s2[E];
+    int in[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        const int64_t ad = row + 2 * (j + T * s);
-        xv[s] = a.dr ? *reinterpret_cast<const float2*>(a.x + ad) : make_float2(0.0f, 0.0f);
-        s2[s] = a.dr ? *reinterpret_cast<const uint16_t*>(a.S + ad) : (uint16_t)0;
+        in[s] = pair_in(row_in, 2 * (j + T * s), N, a.bx);
+        load_pair(a, row + 2 * (j + T * s), a.dr ? in[s] : 0, true, xv[s], s2[s]);
     }
 #pragma unroll
-    for (int s = 0; s < E; ++s)
-        p5_update(a, cscale(v[out_slot<M>(s)], sc), xv[s], s2[s], row + 2 * (j + T * s), true);
+    for (int s = 0; s < E; ++s) {
+        if (!in[s]) continue;
+        const float2 d = cscale(v[out_slot<M>(s)], sc);
+        if (a.dmode) {
+            float2 rk;
+            p5_dmode(a, d, xv[s], s2[s], row + 2 * (j + T * s), &rk);
+        } else {
+            p5_update(a, d, xv[s], s2[s], row + 2 * (j + T * s), true);
+        }
+    }
 }
 
 // P5 of one iteration fused with P1 of the next: the C2R output updates the
@@ -442,7 +597,7 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch h
 // transform's twiddle loads live for reuse by the forward one, which doubles
 // the register count and halves occupancy.
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfBatch hb,
+__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfBatch hb, int cy,
                                                                            const float2* __restrict__ twM,
                                                                            const float2* __restrict__ twN,
                                                                            const float2* __restrict__ twM2,
@@ -453,29 +608,44 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfB
     using PL = RegPlan<M>;
     constexpr int E = PL::E, T = M / E;
     extern __shared__ float2 lds[];
-    const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = a.n2;
-    const int tpr = n2 / kHP;
-    const int i3 = blockIdx.x / tpr, i2_0 = (blockIdx.x % tpr) * kHP;
+    int i3, i2_0;
+    if (!x_tile(a, cy, i3, i2_0)) return;
     float2 v[E];
     load_c2r<N>(a, lds, v, i2_0, i3, twM, twN);
     const float sc = 1.0f / (float)M;
     const int tu = phase_tid(), ju = tu % T, pu = tu / T;
     const int64_t row = (int64_t)N * ((i2_0 + pu) + (int64_t)n2 * i3) + 2 * ju;
+    const bool row_in = inb(i2_0 + pu, n2, a.by);
     float2 xv[E];
     uint16_t s2[E];
+    int in[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        const int64_t ad = row + 2 * T * s;
-        xv[s] = a.dr ? *reinterpret_cast<const float2*>(a.x + ad) : make_float2(0.0f, 0.0f);
-        s2[s] = *reinterpret_cast<const uint16_t*>(a.S + ad);
+        in[s] = pair_in(row_in, 2 * (ju + T * s), N, a.bx);
+        load_pair(a, row + 2 * T * s, in[s], a.dr != 0, xv[s], s2[s]);
     }
     float2 w[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int64_t ad = row + 2 * T * s;
-        const float2 xn = p5_update(a, cscale(v[out_slot<M>(s)], sc), xv[s], s2[s], ad, a.dr != 0);
-        w[in_slot<M>(s)] = p1_value(a, xn, s2[s], ad);
+        float2 u = make_float2(0.0f, 0.0f);
+        if (in[s]) {
+            const float2 d = cscale(v[out_slot<M>(s)], sc);
+            if (a.dmode) {
+                // next transform input: 2 P_R(x_{k+1}) - P_R(x_k)
+                float2 rk;
+                const float2 xn = p5_dmode(a, d, xv[s], s2[s], ad, &rk);
+                const float2 rn = make_float2(prh(xn.x, (uint8_t)(s2[s] & 0xFF), a.positivity, a.support),
+                                              prh(xn.y, (uint8_t)(s2[s] >> 8), a.positivity, a.support));
+                if (a.best) *reinterpret_cast<float2*>(a.best + ad) = rn;
+                u = make_float2(2.0f * rn.x - rk.x, 2.0f * rn.y - rk.y);
+            } else {
+                const float2 xn = p5_update(a, d, xv[s], s2[s], ad, a.dr != 0);
+                u = p1_value(a, xn, s2[s], ad);
+            }
+        }
+        w[in_slot<M>(s)] = u;
     }
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = w[s];
@@ -517,8 +687,8 @@ __global__ void k_hcsr_count(const float2* __restrict__ M, const uint8_t* __rest
         for (int z = 0; z < n3; ++z) {
             const HalfVox hv = half_vox(kx, ky, z, n1, n2, n3);
             const bool A = hmeas(M, mask, hv.s);
-            const bool B = !hv.self_paired && hmeas(M, mask, hv.mate);
-            c += (A || B) ? 1u : 0u;
+            const bool Bm = hmeas(M, mask, hv.mate);   // on the kx = 0, n1/2 planes: the in-plane mate
+            c += (A || Bm) ? 1u : 0u;
         }
         cnt[q] = c;
     }
@@ -536,8 +706,18 @@ __global__ void k_hcsr_fill(const float2* __restrict__ M, const uint8_t* __restr
         for (int z = 0; z < n3; ++z) {
             const HalfVox hv = half_vox(kx, ky, z, n1, n2, n3);
             const bool A = hmeas(M, mask, hv.s);
-            const bool B = !hv.self_paired && hmeas(M, mask, hv.mate);
-            if (!A && !B) continue;
+            const bool Bm = hmeas(M, mask, hv.mate);
+            const bool B = !hv.self_paired && Bm;
+            if (!A && !Bm) continue;
+            if (!A && hv.self_paired) {
+                // listed without terms or replacement: it carries the mate's
+                // share of the replacement delta on these planes
+                meta[o] = (uint32_t)z | ((uint32_t)ky << 11) | kHalfPassive;
+                val[o] = make_float2(0.0f, 0.0f);
+                mag[o] = make_float2(0.0f, 0.0f);
+                ++o;
+                continue;
+            }
             const float2 ms = M[hv.s];
             const float2 mm = hv.self_paired ? make_float2(0.0f, 0.0f) : conjf2(M[hv.mate]);
             uint32_t f = (uint32_t)z | ((uint32_t)ky << 11);
@@ -576,6 +756,7 @@ __global__ void k_hcsr_enforce(int n1, int n2, int n3, const int64_t* __restrict
         const double v2 = kcoord((ky + n2 / 2) % n2, n2);
         for (int64_t e = off[q]; e < off[q + 1]; ++e) {
             uint32_t m = meta[e] & ~kHalfEnforced;
+            if (m & kHalfPassive) continue;
             const int z = (int)(m & 0x7FFu);
             const double v3 = kcoord((z + n3 / 2) % n3, n3);
             const double Q = sqrt((v2 * v2 + v1 * v1) + v3 * v3);
@@ -624,6 +805,57 @@ __global__ __launch_bounds__(256) void k_rfree_reduce(const float* __restrict__ 
     if (threadIdx.x == 0) out[sh + (int64_t)nbins * (*ctr)] = den[sh] > 0.0 ? (float)(red[0] / den[sh]) : -1.0f;
 }
 
+// D -> D' on the kx = 0 and n1/2 planes: (D(s) + conj D(-s))/2 per mate pair,
+// Re D(s) for a voxel that is its own mate.  Plane entries are the list's
+// first run (pencils kx = 0) and its last (kx = n1/2); mate[i] is the entry of
+// plane entry i's in-plane mate.
+__global__ void k_dherm(HalfBatch hb)
+{
+    const HalfIterArgs& a = hb.job[blockIdx.y];
+    if (!a.dmode) return;
+    const int64_t n0 = a.csr_off[a.n2];
+    const int64_t mb = a.csr_off[(int64_t)(a.n1 / 2) * a.n2], me = a.csr_off[(int64_t)(a.n1 / 2 + 1) * a.n2];
+    const int64_t cnt = n0 + (me - mb);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i < n0 ? i : mb + (i - n0);
+        const int64_t m = a.csr_mate[i];
+        if (m < e) continue;   // the pair is handled from its lower entry (or has no mate)
+        const float2 de = a.csr_d[e];
+        if (m == e) {
+            a.csr_d[e] = make_float2(de.x, 0.0f);
+            continue;
+        }
+        const float2 dm = a.csr_d[m];
+        a.csr_d[e] = make_float2(0.5f * (de.x + dm.x), 0.5f * (de.y - dm.y));
+        a.csr_d[m] = make_float2(0.5f * (dm.x + de.x), 0.5f * (dm.y - de.y));
+    }
+}
+
+__global__ void k_hcsr_mates(const int64_t* __restrict__ off, const uint32_t* __restrict__ meta, int n1, int n2,
+                             int n3, int64_t n0, int64_t mb, int64_t cnt, int64_t* __restrict__ mate)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i < n0 ? i : mb + (i - n0);
+        const int kx = i < n0 ? 0 : n1 / 2;
+        const uint32_t m = meta[e];
+        const int z = (int)(m & 0x7FFu), ky = (int)((m >> 11) & 0x7FFu);
+        const int mky = (n2 - ky) % n2, mz = (n3 - z) % n3;
+        const int64_t q = (int64_t)mky + (int64_t)n2 * kx;
+        int64_t lo = off[q], hi = off[q + 1], found = -1;
+        while (lo < hi) {   // entries of a pencil are z-ascending
+            const int64_t mid = (lo + hi) / 2;
+            const int zm = (int)(meta[mid] & 0x7FFu);
+            if (zm == mz) {
+                found = mid;
+                break;
+            }
+            if (zm < mz) lo = mid + 1;
+            else hi = mid;
+        }
+        mate[i] = found;
+    }
+}
+
 __global__ void k_hsum(const double* part, int n, double* out)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -642,46 +874,68 @@ template <class K> void set_lds(K kern, size_t bytes)
         GF_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+// Launch extents of the pruned passes: the largest tile counts of the batch.
+struct XTiles {
+    int cy, cz;
+};
+XTiles x_tiles(const HalfBatch& b)
+{
+    XTiles r{0, 0};
+    for (int i = 0; i < b.count; ++i) {
+        const HalfIterArgs& a = b.job[i];
+        r.cy = std::max(r.cy, box_tiles(a.n2, a.by, kHP).count);
+        r.cz = std::max(r.cz, box_tiles(a.n3, a.bz, 1).count);
+    }
+    return r;
+}
+int y_tiles(const HalfBatch& b)
+{
+    int cz = 0;
+    for (int i = 0; i < b.count; ++i) cz = std::max(cz, box_tiles(b.job[i].n3, b.job[i].bz, kHP).count);
+    return cz;
+}
+
 template <int N> void launch_first(const HalfBatch& b, hipStream_t st)
 {
-    const HalfIterArgs& a = b.job[0];
     const size_t lds = (size_t)kHP * (N / 2) * sizeof(float2);
     set_lds(k_half_first<N>, lds);
-    k_half_first<N><<<dim3((a.n2 / kHP) * a.n3, b.count), half_threads<N / 2>(), lds, st>>>(b, fft_twiddles(N / 2),
-                                                                                          fft_twiddles(N));
+    const XTiles xt = x_tiles(b);
+    k_half_first<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, fft_twiddles(N / 2),
+                                                                                   fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
 template <int N> void launch_last(const HalfBatch& b, hipStream_t st)
 {
-    const HalfIterArgs& a = b.job[0];
     const size_t lds = (size_t)kHP * (N / 2 + 1) * sizeof(float2);
     set_lds(k_half_last<N>, lds);
-    k_half_last<N><<<dim3((a.n2 / kHP) * a.n3, b.count), half_threads<N / 2>(), lds, st>>>(b, fft_twiddles(N / 2),
-                                                                                         fft_twiddles(N));
+    const XTiles xt = x_tiles(b);
+    k_half_last<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, fft_twiddles(N / 2),
+                                                                                  fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
 template <int N> void launch_last_first(const HalfBatch& b, hipStream_t st)
 {
-    const HalfIterArgs& a = b.job[0];
     const size_t lds = (size_t)kHP * (N / 2 + 1) * sizeof(float2);
     set_lds(k_half_last_first<N>, lds);
     const float2* twM = fft_twiddles(N / 2);
     const float2* twN = fft_twiddles(N);
-    k_half_last_first<N><<<dim3((a.n2 / kHP) * a.n3, b.count), half_threads<N / 2>(), lds, st>>>(b, twM, twN, twM,
-                                                                                               twN);
+    const XTiles xt = x_tiles(b);
+    k_half_last_first<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, twM, twN, twM,
+                                                                                        twN);
     GF_LAUNCH_CHECK();
 }
 template <int N> void launch_y(const HalfBatch& b, int inverse, hipStream_t st)
 {
     const HalfIterArgs& a = b.job[0];
     const size_t lds = (size_t)kHP * N * sizeof(float2);
-    const dim3 blocks((a.n1 / 2 + 1) * (a.n3 / kHP), b.count);
+    const int cz = y_tiles(b);
+    const dim3 blocks((a.n1 / 2 + 1) * cz, b.count);
     if (inverse) {
         set_lds(k_half_y_inv<N>, lds);
-        k_half_y_inv<N><<<blocks, half_threads<N>(), lds, st>>>(b, fft_twiddles(N));
+        k_half_y_inv<N><<<blocks, half_threads<N>(), lds, st>>>(b, cz, fft_twiddles(N));
     } else {
         set_lds(k_half_y_fwd<N>, lds);
-        k_half_y_fwd<N><<<blocks, half_threads<N>(), lds, st>>>(b, fft_twiddles(N));
+        k_half_y_fwd<N><<<blocks, half_threads<N>(), lds, st>>>(b, cz, fft_twiddles(N));
     }
     GF_LAUNCH_CHECK();
 }
@@ -801,7 +1055,27 @@ void half_last_first(const HalfBatch& b, hipStream_t st)
 #undef C
 }
 
+void half_dherm(const HalfBatch& b, hipStream_t st)
+{
+    check_batch(b);
+    bool any = false;
+    for (int i = 0; i < b.count; ++i) any = any || b.job[i].dmode;
+    if (!any) return;
+    k_dherm<<<dim3(64, b.count), 256, 0, st>>>(b);
+    GF_LAUNCH_CHECK();
+}
+
+void hcsr_mates(const int64_t* d_off, const uint32_t* d_meta, int n1, int n2, int n3, int64_t n_plane0,
+                int64_t plane_m_begin, int64_t n_plane, int64_t* d_mate, hipStream_t st)
+{
+    if (n_plane <= 0) return;
+    k_hcsr_mates<<<pblocks(n_plane), 256, 0, st>>>(d_off, d_meta, n1, n2, n3, n_plane0, plane_m_begin, n_plane,
+                                                   d_mate);
+    GF_LAUNCH_CHECK();
+}
+
 void half_first(const HalfIterArgs& a, hipStream_t st) { half_first(one(a), st); }
+void half_dherm(const HalfIterArgs& a, hipStream_t st) { half_dherm(one(a), st); }
 void half_y(const HalfIterArgs& a, int inverse, hipStream_t st) { half_y(one(a), inverse, st); }
 void half_z(const HalfIterArgs& a, hipStream_t st) { half_z(one(a), st); }
 void half_last(const HalfIterArgs& a, hipStream_t st) { half_last(one(a), st); }

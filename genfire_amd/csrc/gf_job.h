@@ -114,7 +114,7 @@ struct genfire_job {
 
     // iteration
     gf::DevBuf x, K, S, best, pfbuf, err_part, errK, errctr, den_tmp, den, recon;
-    gf::DevBuf csr_cnt, csr_off, csr_meta, csr_val, csr_mag;
+    gf::DevBuf csr_cnt, csr_off, csr_meta, csr_val, csr_mag, csr_d, csr_mate;
     int64_t nmeas = 0;
     gf::DevBuf vol, vol_y, Rmat, rsums, rout;
     // Rfree: withheld voxels per half pencil, their terms, shell order and denominators

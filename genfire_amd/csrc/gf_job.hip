@@ -683,6 +683,35 @@ void recon_begin(genfire_job* j)
     ha.positivity = ia.positivity;
     ha.support = ia.support;
     ha.dr = ia.dr;
+    // Box pruning needs P_R(x) to vanish outside the Dim box (support on);
+    // GENFIRE_PRUNE=0 turns it off (every row transformed, as for support off).
+    const char* pe = std::getenv("GENFIRE_PRUNE");
+    const bool box = half && ia.support && !(pe && std::string(pe) == "0");
+    ha.bx = box ? j->dim1 : n1;
+    ha.by = box ? j->dim2 : n2;
+    ha.bz = box ? j->dim1 : n3;
+    ha.dmode = (box && dr) ? 1 : 0;
+    if (ha.dmode) {
+        // replacement deltas start at zero (D'_0 = 0: the first transform is of
+        // 2 P_R(x_1) - x_1 with x_1 in the box), and the in-plane mates of the
+        // kx = 0 and n1/2 plane entries
+        j->csr_d.alloc(sizeof(float2) * std::max<int64_t>(j->nmeas, 1));
+        GF_HIP(hipMemsetAsync(j->csr_d.p, 0, j->csr_d.bytes, st));
+        int64_t o[3];
+        const int64_t* off = j->csr_off.as<int64_t>();
+        GF_HIP(hipMemcpyAsync(&o[0], off + n2, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipMemcpyAsync(&o[1], off + (int64_t)(n1 / 2) * n2, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipMemcpyAsync(&o[2], off + (int64_t)(n1 / 2 + 1) * n2, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
+        const int64_t npl = o[0] + (o[2] - o[1]);
+        j->csr_mate.alloc(sizeof(int64_t) * std::max<int64_t>(npl, 1));
+        hcsr_mates(off, j->csr_meta.as<uint32_t>(), n1, n2, n3, o[0], o[1], npl, j->csr_mate.as<int64_t>(), st);
+        ha.csr_d = j->csr_d.as<float2>();
+        ha.csr_mate = j->csr_mate.as<int64_t>();
+    } else {
+        j->csr_d.release();
+        j->csr_mate.release();
+    }
     if (j->rf_bins > 0) {
         ha.rf_off = j->rf_off.as<int64_t>();
         ha.rf_meta = j->rf_meta.as<uint32_t>();
@@ -741,6 +770,7 @@ void recon_body(genfire_job* j, int it)
         kstat_begin(j, GENFIRE_KSTAT_FFT_FUSED, &e_fused);
         half_z(r.ha, st);
         kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
+        half_dherm(r.ha, st);
         rfree_step(j, st);
         half_y(r.ha, 1, st);
         if (it < r.N) half_last_first(p5_args(j, it), st);
@@ -776,6 +806,7 @@ void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStr
         }
         half_y(all, 0, st);
         half_z(all, st);
+        half_dherm(all, st);
         for (int i = c0; i < c1; ++i) rfree_step(jobs[i], st);
         half_y(all, 1, st);
         if (cont.count) half_last_first(cont, st);
@@ -1031,7 +1062,7 @@ int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P)
     const int64_t hn = (n1 / 2 + 1) * n2 * n3;
     const int64_t spectrum = 8 * (half ? std::max<int64_t>(2 * hn, nf) : nf);
     const int64_t meas = std::min<int64_t>(nf, 2 * pts);
-    const int64_t iter = nf * (4 + 1 + 4 + 4 + 8) + spectrum + meas * 20 + n1 * n2 * 12 + box * 8 +
+    const int64_t iter = nf * (4 + 1 + 4 + 4 + 8) + spectrum + meas * 28 + n1 * n2 * 12 + box * 8 +
                          (int64_t)P * 4 * 8 * (1 + p.numIterations / std::max(1, p.monitorR_loopLength));
     return inputs + std::max(grid, iter) + (64ll << 20);
 }
