@@ -27,11 +27,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "recon/hour + gridded k-pts/s at 256³×91 proj; p50/p99 latency; R-factor Δ vs FP64"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the y-forward pass, keyed by grid edge; None where not
-# measured. rocprofv3 --pmc FETCH_SIZE (x2.00: the counter reads half of a known
-# 1 GiB read on MI355X) + WRITE_SIZE (x1.00, exact on a known 1 GiB fill), in
-# profiles/tilt256_hbm_traffic.md: 263218 KiB fetched, 526336 KiB written.
-TRAFFIC = {512: 2 * 263218 * 1024 + 526336 * 1024}
+# HBM bytes per launch of the (box-pruned) y-forward pass of a 256^3 box in a
+# 512^3 grid; None where not measured. rocprofv3 --pmc FETCH_SIZE (x2.00: the
+# counter reads half of a known 1 GiB read on MI355X) + WRITE_SIZE (x1.00, exact
+# on a known 1 GiB fill), in profiles/tilt256_hbm_traffic.md: 65853 KiB
+# fetched, 263168 KiB written.
+TRAFFIC = {512: 2 * 65853 * 1024 + 263168 * 1024}
 
 
 def log(msg):
@@ -105,6 +106,13 @@ def rfactor_fp64_delta(ctx, model, projs, ang, idx):
     r_gpu, _, _ = ctx.rfactor(model, sub_p, sub_a)
     r_64, _, _ = O.rfactor_posthoc(model, sub_p, sub_a)
     return abs(r_gpu - r_64), r_gpu
+
+
+def box_rows(n: int, b: int, ts: int) -> int:
+    """Rows of the ts-row tiles along an axis of length n that meet the centred
+    box of extent b (FFT order: [0, b - b//2) and [n - b//2, n))."""
+    nt, lo_end, hi_begin = n // ts, (b - b // 2 + ts - 1) // ts, (n - b // 2) // ts
+    return n if lo_end >= hi_begin else ts * (lo_end + nt - hi_begin)
 
 
 def grid_value_error(job, projs, idx, npts=2000, crop=0):
@@ -338,9 +346,11 @@ def main():
     ms_step = float(np.mean(lat_ms))
     n = job.n1
     h = n // 2 + 1
-    # P2 (y-forward pass of the half-spectrum iteration): one read and one write
-    # of the complex64 half spectrum
-    pass_bytes = 16.0 * h * n * n
+    # P2 (y-forward pass of the half-spectrum iteration) with the support box
+    # pruning: it reads the rows with y and z in the Dim box and writes every ky
+    # of the z rows its tiles cover (complex64)
+    zrows = box_rows(n, w.dim, 16)
+    pass_bytes = 8.0 * h * (w.dim * w.dim + n * zrows)
     nfp, msfp = stats[_lib.KSTAT_FFT_PASS]
     nit, msit = stats[_lib.KSTAT_ITERATION]
     pass_ms = msfp / max(nfp, 1)
@@ -378,7 +388,7 @@ def main():
         "iteration_roofline_frac": (b_it / (it_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(n),
-                     "kernel": f"k_half_y_fwd<{n}> (y-forward pass), {pass_bytes / 1e9:.3f} GB algorithmic "
+                     "kernel": f"k_half_y_fwd<{n}> (y-forward pass, box-pruned), {pass_bytes / 1e9:.3f} GB algorithmic "
                                f"per launch, {pass_ms:.3f} ms mean over {nfp} launches"},
     }
     if args.served_jobs > 0:
