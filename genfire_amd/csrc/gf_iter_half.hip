@@ -136,6 +136,24 @@ template <int M> __device__ __forceinline__ constexpr int out_slot(int s)
     return (s % (RegPlan<M>::E / RL)) * RL + s / (RegPlan<M>::E / RL);
 }
 
+// A zero the compiler cannot see through.  An LDS table address built with it
+// keeps one transform's twiddle loads from being reused, and so kept live, by
+// the next transform of the same kernel.
+__device__ __forceinline__ int opaque_zero()
+{
+    int z = 0;
+    asm volatile("" : "+v"(z));
+    return z;
+}
+
+// Copies a twiddle table into LDS.  The transforms read their twiddles from
+// there: as global loads they kept the texture address unit busy for most of
+// a pass.  The first barrier inside the transform orders the copy.
+__device__ __forceinline__ void tw_to_lds(float2* dst, const float2* __restrict__ src, int n, int t, int nt)
+{
+    for (int i = t; i < n; i += nt) dst[i] = src[i];
+}
+
 // threadIdx.x behind an empty asm statement.  Each phase of a fused pass takes
 // its lane indices from this, so the compiler recomputes the phase's addresses
 // instead of keeping those of an earlier phase live across the whole kernel.
@@ -225,6 +243,10 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch 
     const int n2 = a.n2;
     int i3, i2_0;
     if (!x_tile(a, cy, i3, i2_0)) return;
+    float2* twlM = lds + kHP * (M + 1);
+    float2* twlN = twlM + M;
+    tw_to_lds(twlM, twM, M, t, half_threads<M>());
+    tw_to_lds(twlN, twN, M + 1, t, half_threads<M>());
     const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
     const bool row_in = inb(i2_0 + p, n2, a.by);
 
@@ -241,7 +263,7 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch 
     float2 v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) v[in_slot<M>(s)] = p1_value(a, xv[s], s2[s], row + 2 * (j + T * s), in[s] != 0);
-    r2c_store<N>(a, lds, v, i2_0, i3, twM, twN);
+    r2c_store<N>(a, lds, v, i2_0, i3, twlM, twlN);
 }
 
 // ---------------------------------------------------------------------------
@@ -258,6 +280,8 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
     const int n2 = N, n3 = a.n3, h = a.n1 / 2 + 1;
     int kx, i3_0;
     if (!y_tile(a, cz, kx, i3_0)) return;
+    float2* twl = lds + kHP * N;
+    tw_to_lds(twl, tw, N, t, half_threads<N>());
     const float2* src = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p));
     const bool z_in = inb(i3_0 + p, n3, a.bz);
     float2 v[E];
@@ -269,7 +293,7 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
             v[i * R0 + r] = (z_in && inb(y, N, a.by)) ? src[y] : make_float2(0.0f, 0.0f);
         }
     float2* rowl = lds + p * N;
-    reg_fft<N, false>(v, rowl, j, p, tw);
+    reg_fft<N, false>(v, rowl, j, p, twl);
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
@@ -303,6 +327,9 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     const int n2 = a.n2, n3 = N;
     const int tpr = n2 / kZP;
     const int kx = blockIdx.x / tpr, ky_0 = (blockIdx.x % tpr) * kZP;
+    // LDS: the tile, one errK slot per wave, the twiddle table
+    float2* twl = lds + kZP * N + NT / 64;
+    tw_to_lds(twl, tw, N, t, NT);
     float2* pen = a.B + (int64_t)n3 * ((ky_0 + p) + (int64_t)n2 * kx);
     float2 v[E];
 #pragma unroll
@@ -326,7 +353,7 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
         V0 = a.csr_val[e0];
     }
     float2* rowl = lds + p * N;
-    reg_fft<N, false>(v, rowl, j, p, tw);
+    reg_fft<N, false>(v, rowl, j, p, twl);
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
@@ -411,7 +438,7 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     float2* rowl2 = lds + p2 * N;
     reg_read<N, E, R0>(rowl2, v, j2, p2);
     __syncthreads();
-    reg_fft<N, true>(v, rowl2, j2, p2, tw2);
+    reg_fft<N, true>(v, rowl2, j2, p2, lds + (kZP * N + NT / 64 + opaque_zero()));
     const float sc = 1.0f / (float)N;
     float2* pen2 = a.B + (int64_t)n3 * ((ky_0 + p2) + (int64_t)n2 * kx);
 #pragma unroll
@@ -437,6 +464,8 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     const int n2 = N, n3 = a.n3, h = a.n1 / 2 + 1;
     int kx, i3_0;
     if (!y_tile(a, cz, kx, i3_0)) return;
+    float2* twl = lds + kHP * N;
+    tw_to_lds(twl, tw, N, t, half_threads<N>());
     {
         const int pp = t % kHP, c = t / kHP;
         float2* rp = lds + pp * N;
@@ -454,7 +483,7 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     float2 v[E];
     reg_read<N, E, R0>(rowl, v, j2, p2);
     __syncthreads();
-    reg_fft<N, true>(v, rowl, j2, p2, tw);
+    reg_fft<N, true>(v, rowl, j2, p2, twl);
     float2* dst = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p2));
     const float sc = 1.0f / (float)N;
     if (!inb(i3_0 + p2, n3, a.bz)) return;
@@ -560,8 +589,12 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch h
     const int n2 = a.n2;
     int i3, i2_0;
     if (!x_tile(a, cy, i3, i2_0)) return;
+    float2* twlM = lds + kHP * (M + 1);
+    float2* twlN = twlM + M;
+    tw_to_lds(twlM, twM, M, (int)threadIdx.x, half_threads<M>());
+    tw_to_lds(twlN, twN, M + 1, (int)threadIdx.x, half_threads<M>());
     float2 v[E];
-    load_c2r<N>(a, lds, v, i2_0, i3, twM, twN);
+    load_c2r<N>(a, lds, v, i2_0, i3, twlM, twlN);
     const int t = phase_tid(), j = t % T, p = t / T;
     const float sc = 1.0f / (float)M;
     const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
@@ -611,8 +644,12 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfB
     const int n2 = a.n2;
     int i3, i2_0;
     if (!x_tile(a, cy, i3, i2_0)) return;
+    float2* twlM = lds + kHP * (M + 1);
+    float2* twlN = twlM + M;
+    tw_to_lds(twlM, twM, M, (int)threadIdx.x, half_threads<M>());
+    tw_to_lds(twlN, twN, M + 1, (int)threadIdx.x, half_threads<M>());
     float2 v[E];
-    load_c2r<N>(a, lds, v, i2_0, i3, twM, twN);
+    load_c2r<N>(a, lds, v, i2_0, i3, twlM, twlN);
     const float sc = 1.0f / (float)M;
     const int tu = phase_tid(), ju = tu % T, pu = tu / T;
     const int64_t row = (int64_t)N * ((i2_0 + pu) + (int64_t)n2 * i3) + 2 * ju;
@@ -649,7 +686,8 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfB
     }
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = w[s];
-    r2c_store<N>(a, lds, v, i2_0, i3, twM2, twN2);
+    const int z0 = opaque_zero();
+    r2c_store<N>(a, lds, v, i2_0, i3, lds + (kHP * (M + 1) + z0), lds + (kHP * (M + 1) + M + z0));
 }
 
 // ---------------------------------------------------------------------------
@@ -895,9 +933,13 @@ int y_tiles(const HalfBatch& b)
     return cz;
 }
 
+// LDS of an x pass: the tile and the Nyquist bins, then the twiddle tables of
+// the N/2-point transform and of the real split (N/2 + 1 entries).
+template <int N> constexpr size_t x_pass_lds() { return ((size_t)kHP * (N / 2 + 1) + N + 1) * sizeof(float2); }
+
 template <int N> void launch_first(const HalfBatch& b, hipStream_t st)
 {
-    const size_t lds = (size_t)kHP * (N / 2) * sizeof(float2);
+    const size_t lds = x_pass_lds<N>();
     set_lds(k_half_first<N>, lds);
     const XTiles xt = x_tiles(b);
     k_half_first<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, fft_twiddles(N / 2),
@@ -906,7 +948,7 @@ template <int N> void launch_first(const HalfBatch& b, hipStream_t st)
 }
 template <int N> void launch_last(const HalfBatch& b, hipStream_t st)
 {
-    const size_t lds = (size_t)kHP * (N / 2 + 1) * sizeof(float2);
+    const size_t lds = x_pass_lds<N>();
     set_lds(k_half_last<N>, lds);
     const XTiles xt = x_tiles(b);
     k_half_last<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, fft_twiddles(N / 2),
@@ -915,7 +957,7 @@ template <int N> void launch_last(const HalfBatch& b, hipStream_t st)
 }
 template <int N> void launch_last_first(const HalfBatch& b, hipStream_t st)
 {
-    const size_t lds = (size_t)kHP * (N / 2 + 1) * sizeof(float2);
+    const size_t lds = x_pass_lds<N>();
     set_lds(k_half_last_first<N>, lds);
     const float2* twM = fft_twiddles(N / 2);
     const float2* twN = fft_twiddles(N);
@@ -927,7 +969,7 @@ template <int N> void launch_last_first(const HalfBatch& b, hipStream_t st)
 template <int N> void launch_y(const HalfBatch& b, int inverse, hipStream_t st)
 {
     const HalfIterArgs& a = b.job[0];
-    const size_t lds = (size_t)kHP * N * sizeof(float2);
+    const size_t lds = ((size_t)kHP * N + N) * sizeof(float2);   // tile + twiddle table
     const int cz = y_tiles(b);
     const dim3 blocks((a.n1 / 2 + 1) * cz, b.count);
     if (inverse) {
@@ -942,8 +984,9 @@ template <int N> void launch_y(const HalfBatch& b, int inverse, hipStream_t st)
 template <int N> void launch_z(const HalfBatch& b, hipStream_t st)
 {
     const HalfIterArgs& a = b.job[0];
-    // tile + one double per wave for the errK partial
-    const size_t lds = (size_t)kZP * N * sizeof(float2) + sizeof(double) * (zfused_threads<N>() / 64);
+    // tile + one double per wave for the errK partial + twiddle table
+    const size_t lds = (size_t)kZP * N * sizeof(float2) + sizeof(double) * (zfused_threads<N>() / 64) +
+                       (size_t)N * sizeof(float2);
     set_lds(k_half_z_fused<N>, lds);
     k_half_z_fused<N><<<dim3((a.n1 / 2 + 1) * (a.n2 / kZP), b.count), zfused_threads<N>(), lds, st>>>(
         b, fft_twiddles(N), fft_twiddles(N));
