@@ -346,11 +346,12 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     const int beg = (int)a.csr_off[r0], end = (int)a.csr_off[r0 + kZP];   // < 2^31 entries
     const int e0 = beg + t;
     uint32_t m0 = 0;
-    float2 mg0 = make_float2(0.0f, 0.0f), V0 = mg0;
+    float2 mg0 = make_float2(0.0f, 0.0f), V0 = mg0, d0 = mg0;
     if (e0 < end) {
         m0 = a.csr_meta[e0];
         mg0 = a.csr_mag[e0];
         V0 = a.csr_val[e0];
+        if (a.dmode) d0 = a.csr_d[e0];
     }
     float2* rowl = lds + p * N;
     reg_fft<N, false>(v, rowl, j, p, twl);
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
             const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
             float2* slot = lds + pl * N + swz(z, pl);
-            const float2 dv = a.csr_d[e];
+            const float2 dv = (e == e0) ? d0 : a.csr_d[e];
             const float2 kv = *slot;
             *slot = make_float2(kv.x - dv.x, kv.y - dv.y);
         }
@@ -388,6 +389,7 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     // errK numerator (:135) and replacement (:163) from the half-spectrum lists;
     // the numerator is reduced here (LDS past the tile) so that it is not live
     // through the inverse transform
+    float2 dn0 = make_float2(0.0f, 0.0f);   // the new delta of the lane's first entry
     {
         float err = 0.0f;   // at most a few entries per lane
         for (int e = e0; e < end; e += NT) {
@@ -407,8 +409,13 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             float2 kn = kv;
             if (m & kHalfEnforced)
                 kn = (m & kHalfAverage) ? make_float2(0.5f * (V.x + kv.x), 0.5f * (V.y + kv.y)) : V;
-            if (a.dmode) a.csr_d[e] = make_float2(kn.x - kv.x, kn.y - kv.y);   // D = k' - k
-            else if (m & kHalfEnforced) *slot = kn;
+            if (a.dmode) {
+                const float2 dn = make_float2(kn.x - kv.x, kn.y - kv.y);   // D = k' - k
+                a.csr_d[e] = dn;
+                if (e == e0) dn0 = dn;
+            } else if (m & kHalfEnforced) {
+                *slot = kn;
+            }
         }
         double de = (double)err;
         for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off, 64);
@@ -426,9 +433,9 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
         for (int i = t; i < kZP * N; i += NT) lds[i] = make_float2(0.0f, 0.0f);
         __syncthreads();
         for (int e = e0; e < end; e += NT) {
-            const uint32_t m = a.csr_meta[e];
+            const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
             const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
-            lds[pl * N + swz(z, pl)] = a.csr_d[e];
+            lds[pl * N + swz(z, pl)] = (e == e0) ? dn0 : a.csr_d[e];
         }
         __syncthreads();
     }
