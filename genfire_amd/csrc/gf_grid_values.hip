@@ -314,7 +314,7 @@ namespace {
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kMP = 64;    // points per workgroup (4 waves x 16)
-constexpr int kKC = 32;    // projection rows staged per step
+constexpr int kKC = 16;    // projection rows staged per step (two buffers)
 
 __global__ void k_transpose_proj(const float* __restrict__ proj, int dim1, int dim2, int P, int Kp, int Np,
                                  float* __restrict__ Pt)
@@ -335,8 +335,8 @@ __global__ __launch_bounds__(256) void k_dft_mfma(const float* __restrict__ Pt, 
     constexpr int NTT = NCH / 16;
     extern __shared__ float smem[];
     const int H1 = Kp / 16, H2 = Np / 16;
-    float* Pl = smem;                                         // kKC x STR
-    float2* t1hi = reinterpret_cast<float2*>(Pl + kKC * STR); // [H1][64]
+    float* Pl = smem;                                             // 2 x kKC x STR
+    float2* t1hi = reinterpret_cast<float2*>(Pl + 2 * kKC * STR); // [H1][64]
     float2* t1lo = t1hi + H1 * kMP;                           // [16][64]
     float2* t2hi = t1lo + 16 * kMP;                           // [H2][64]
     float2* t2lo = t2hi + H2 * kMP;                           // [16][64]
@@ -356,22 +356,40 @@ __global__ __launch_bounds__(256) void k_dft_mfma(const float* __restrict__ Pt, 
     const int w = tid / 64, l = tid % 64, li = l % 16, lk = l / 16;
     const int pa = 16 * w + li;
     float Fr[4] = {0.0f, 0.0f, 0.0f, 0.0f}, Fi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // Projection rows are staged through two LDS buffers: the next chunk's
+    // global loads are in flight while the MFMAs consume the current one.
+    constexpr int SPT = kKC * NCH / 256;   // staged values per thread and chunk
+    static_assert(kKC * NCH % 256 == 0, "staging split");
+    const int nkc = Kp / kKC;              // Kp is a multiple of kKC
     for (int nc = 0; nc < Np; nc += NCH) {
         v4f acr[NTT], aci[NTT];
 #pragma unroll
         for (int nt = 0; nt < NTT; ++nt) { acr[nt] = v4f{0, 0, 0, 0}; aci[nt] = v4f{0, 0, 0, 0}; }
-        for (int kc = 0; kc < Kp; kc += kKC) {
-            __syncthreads();
-            for (int idx = tid; idx < kKC * NCH; idx += 256) {
-                const int k = idx / NCH, c = idx % NCH;
-                Pl[k * STR + c] = (kc + k < Kp && nc + c < Np) ? Pp[(int64_t)(kc + k) * Np + nc + c] : 0.0f;
+        float pre[SPT];
+        auto stage_load = [&](int kc) {
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                const int idx = tid + 256 * q, k = idx / NCH, c = idx % NCH;
+                pre[q] = (nc + c < Np) ? Pp[(int64_t)(kc + k) * Np + nc + c] : 0.0f;
+            }
+        };
+        stage_load(0);
+        for (int ic = 0; ic < nkc; ++ic) {
+            float* Pb = Pl + (ic & 1) * (kKC * STR);
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                const int idx = tid + 256 * q, k = idx / NCH, c = idx % NCH;
+                Pb[k * STR + c] = pre[q];
             }
             __syncthreads();
+            if (ic + 1 < nkc) stage_load((ic + 1) * kKC);
+            const int kc = ic * kKC;
+#pragma unroll
             for (int kk = 0; kk < kKC / 4; ++kk) {
                 const int a = kc + 4 * kk + lk;
                 const float2 e1 = a < Kp ? cmul(t1hi[(a >> 4) * kMP + pa], t1lo[(a & 15) * kMP + pa])
                                          : make_float2(0.0f, 0.0f);
-                const float* prow = Pl + (4 * kk + lk) * STR + li;
+                const float* prow = Pb + (4 * kk + lk) * STR + li;
 #pragma unroll
                 for (int nt = 0; nt < NTT; ++nt) {
                     const float b = prow[16 * nt];
@@ -380,6 +398,7 @@ __global__ __launch_bounds__(256) void k_dft_mfma(const float* __restrict__ Pt, 
                 }
             }
         }
+        __syncthreads();   // the buffers are refilled by the next column chunk
         // F += sum_b E2(pt, b) G(pt, b): lane holds columns nc + 16nt + li of points 16w + 4lk + r
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -438,7 +457,7 @@ void grid_values_mfma_fp32(const float* d_Pt, int dim1, int dim2, const float* d
     const int Kp = (dim1 + kKC - 1) / kKC * kKC, Np = (dim2 + 15) / 16 * 16;
     const int H1 = Kp / 16, H2 = Np / 16;
     auto lds = [&](int nch) {
-        return sizeof(float) * kKC * (nch + 16) + sizeof(float2) * kMP * (H1 + H2 + 32);
+        return sizeof(float) * 2 * kKC * (nch + 16) + sizeof(float2) * kMP * (H1 + H2 + 32);
     };
     if (Np >= 128) {
         const size_t b = lds(128);
