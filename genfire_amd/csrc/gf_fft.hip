@@ -251,6 +251,8 @@ __global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
     float2* __restrict__ K = (MODE == kPlain) ? a.data : it.K;
     float2* row = lds + p * N;
     const int64_t pbase = g.base + p * g.ps;
+    float2* twt = lds + TP * N;   // stage twiddle table past the tile
+    reg_tw_fill<N>(twt, a.tw, t, NT);
 
     float2 v[E];
 #pragma unroll
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
             v[i * R0 + r] = valid ? pass_load<MODE>(a, K, pbase + e * g.es) : make_float2(0.0f, 0.0f);
         }
 
-    reg_fft<N, INV>(v, row, j, p, a.tw);
+    reg_fft<N, INV>(v, row, j, p, twt);
 
     if (MODE == kFused) {
         // natural order into LDS, replace, back into the first-stage layout
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
         __syncthreads();
         reg_read<N, E, R0>(row, v, j, p);
         __syncthreads();
-        reg_fft<N, true>(v, row, j, p, a.tw);
+        reg_fft<N, true>(v, row, j, p, twt);
     }
 
     if (!valid) return;
@@ -305,7 +307,7 @@ template <int N> LaunchShape shape_n(int axis, int n1, int n2, int n3)
 {
     if constexpr (RegPlan<N>::ok) {
         return {ntiles(axis, n1, n2, n3, RegPlan<N>::TP), reg_threads<N>(),
-                (size_t)RegPlan<N>::TP * N * sizeof(float2)};
+                ((size_t)RegPlan<N>::TP * N + reg_tw_size<N>()) * sizeof(float2)};
     } else {
         return {ntiles(axis, n1, n2, n3, tile_pencils(N)), kFT, (size_t)tile_pencils(N) * N * sizeof(float2)};
     }
@@ -326,8 +328,12 @@ template <int N, int MODE, bool INV>
 void launch_n(const PassArgs& a, hipStream_t st)
 {
     const LaunchShape sh = shape_n<N>(a.axis, a.it.n1, a.it.n2, a.it.n3);
-    if constexpr (RegPlan<N>::ok)
+    if constexpr (RegPlan<N>::ok) {
+        if (sh.lds > 65536)
+            GF_HIP(hipFuncSetAttribute((const void*)k_fft_reg<N, MODE, INV>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
         k_fft_reg<N, MODE, INV><<<(unsigned)sh.blocks, sh.threads, sh.lds, st>>>(a);
+    }
     else
         k_fft<N, MODE, INV><<<(unsigned)sh.blocks, sh.threads, sh.lds, st>>>(a);
     GF_LAUNCH_CHECK();

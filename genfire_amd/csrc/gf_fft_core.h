@@ -234,8 +234,42 @@ template <int N> constexpr int reg_last_radix()
     return RegPlan<N>::R3 > 1 ? RegPlan<N>::R3 : RegPlan<N>::R2 > 1 ? RegPlan<N>::R2 : RegPlan<N>::R1;
 }
 
+// Twiddles of the register FFT, stage by stage: the stage of span LS and radix
+// R (every stage after the first) holds (R-1) x LS entries, entry (r-1)*LS + k
+// = exp(-2 pi i r k / (LS R)), so the lanes of a wave read consecutive entries
+// (an LDS copy of it has no bank conflicts, unlike the strided r*k reads of the
+// plain N-entry table).
+template <int N> constexpr int reg_tw_size()
+{
+    using PL = RegPlan<N>;
+    int sz = (PL::R1 - 1) * PL::R0;
+    if (PL::R2 > 1) sz += (PL::R2 - 1) * PL::R0 * PL::R1;
+    if (PL::R3 > 1) sz += (PL::R3 - 1) * PL::R0 * PL::R1 * PL::R2;
+    return sz;
+}
+
+// Fills the stage table from the plain table tw[m] = exp(-2 pi i m / N); lanes
+// t of nt share the work.
+template <int N>
+__device__ __forceinline__ void reg_tw_fill(float2* dst, const float2* __restrict__ tw, int t, int nt)
+{
+    using PL = RegPlan<N>;
+    int off = 0;
+    auto fill = [&](int LS, int R) {
+        const int n = (R - 1) * LS;
+        for (int i = t; i < n; i += nt) {
+            const int r = 1 + i / LS, k = i % LS;
+            dst[off + i] = tw[r * k * (N / (LS * R))];
+        }
+        off += n;
+    };
+    fill(PL::R0, PL::R1);
+    if (PL::R2 > 1) fill(PL::R0 * PL::R1, PL::R2);
+    if (PL::R3 > 1) fill(PL::R0 * PL::R1 * PL::R2, PL::R3);
+}
+
 template <int N, int E, int R, int LS, bool INV>
-__device__ __forceinline__ void reg_stage(float2 (&v)[E], int j, const float2* __restrict__ tw)
+__device__ __forceinline__ void reg_stage(float2 (&v)[E], int j, const float2* __restrict__ tws)
 {
     constexpr int T = N / E;
 #pragma unroll
@@ -244,7 +278,7 @@ __device__ __forceinline__ void reg_stage(float2 (&v)[E], int j, const float2* _
             const int k = (j + T * i) & (LS - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 w = tw[r * k * (N / (LS * R))];
+                float2 w = tws[(r - 1) * LS + k];
                 if (INV) w.y = -w.y;
                 v[i * R + r] = cmul(v[i * R + r], w);
             }
@@ -277,8 +311,10 @@ __device__ __forceinline__ void reg_read(const float2* row, float2 (&v)[E], int 
         for (int r = 0; r < R; ++r) v[i * R + r] = row[swz(j + T * i + r * (N / R), p)];
 }
 
+// In-register Stockham FFT of the pencils of a tile; twt is the stage table of
+// reg_tw_fill (normally in LDS).
 template <int N, bool INV>
-__device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row, int j, int p, const float2* tw)
+__device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row, int j, int p, const float2* twt)
 {
     using PL = RegPlan<N>;
     constexpr int E = PL::E, R0 = PL::R0, R1 = PL::R1, R2 = PL::R2, R3 = PL::R3;
@@ -286,25 +322,25 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row,
                   "register FFT plan");
     static_assert(((R0 * R1) & (R0 * R1 - 1)) == 0 && (R3 == 1 || ((R0 * R1 * R2) & (R0 * R1 * R2 - 1)) == 0),
                   "the span of every stage but the last must be a power of two");
-    reg_stage<N, E, R0, 1, INV>(v, j, tw);
+    reg_stage<N, E, R0, 1, INV>(v, j, twt);
     reg_write<N, E, R0, 1>(row, v, j, p);
     __syncthreads();
     reg_read<N, E, R1>(row, v, j, p);
     __syncthreads();
-    reg_stage<N, E, R1, R0, INV>(v, j, tw);
+    reg_stage<N, E, R1, R0, INV>(v, j, twt);
     if constexpr (R2 > 1) {
         reg_write<N, E, R1, R0>(row, v, j, p);
         __syncthreads();
         reg_read<N, E, R2>(row, v, j, p);
         __syncthreads();
-        reg_stage<N, E, R2, R0 * R1, INV>(v, j, tw);
+        reg_stage<N, E, R2, R0 * R1, INV>(v, j, twt + (R1 - 1) * R0);
     }
     if constexpr (R3 > 1) {
         reg_write<N, E, R2, R0 * R1>(row, v, j, p);
         __syncthreads();
         reg_read<N, E, R3>(row, v, j, p);
         __syncthreads();
-        reg_stage<N, E, R3, R0 * R1 * R2, INV>(v, j, tw);
+        reg_stage<N, E, R3, R0 * R1 * R2, INV>(v, j, twt + (R1 - 1) * R0 + (R2 - 1) * R0 * R1);
     }
 }
 

@@ -146,9 +146,10 @@ __device__ __forceinline__ int opaque_zero()
     return z;
 }
 
-// Copies a twiddle table into LDS.  The transforms read their twiddles from
-// there: as global loads they kept the texture address unit busy for most of
-// a pass.  The first barrier inside the transform orders the copy.
+// Copies a table into LDS (the real split's twiddles).  The transforms read
+// their twiddles from LDS stage tables (reg_tw_fill): as global loads they kept
+// the texture address unit busy for most of a pass.  The first barrier inside
+// the transform orders the copy.
 __device__ __forceinline__ void tw_to_lds(float2* dst, const float2* __restrict__ src, int n, int t, int nt)
 {
     for (int i = t; i < n; i += nt) dst[i] = src[i];
@@ -244,8 +245,8 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch 
     int i3, i2_0;
     if (!x_tile(a, cy, i3, i2_0)) return;
     float2* twlM = lds + kHP * (M + 1);
-    float2* twlN = twlM + M;
-    tw_to_lds(twlM, twM, M, t, half_threads<M>());
+    float2* twlN = twlM + reg_tw_size<M>();
+    reg_tw_fill<M>(twlM, twM, t, half_threads<M>());
     tw_to_lds(twlN, twN, M + 1, t, half_threads<M>());
     const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
     const bool row_in = inb(i2_0 + p, n2, a.by);
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
     int kx, i3_0;
     if (!y_tile(a, cz, kx, i3_0)) return;
     float2* twl = lds + kHP * N;
-    tw_to_lds(twl, tw, N, t, half_threads<N>());
+    reg_tw_fill<N>(twl, tw, t, half_threads<N>());
     const float2* src = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p));
     const bool z_in = inb(i3_0 + p, n3, a.bz);
     float2 v[E];
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     const int kx = blockIdx.x / tpr, ky_0 = (blockIdx.x % tpr) * kZP;
     // LDS: the tile, one errK slot per wave, the twiddle table
     float2* twl = lds + kZP * N + NT / 64;
-    tw_to_lds(twl, tw, N, t, NT);
+    reg_tw_fill<N>(twl, tw, t, NT);
     float2* pen = a.B + (int64_t)n3 * ((ky_0 + p) + (int64_t)n2 * kx);
     float2 v[E];
 #pragma unroll
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     int kx, i3_0;
     if (!y_tile(a, cz, kx, i3_0)) return;
     float2* twl = lds + kHP * N;
-    tw_to_lds(twl, tw, N, t, half_threads<N>());
+    reg_tw_fill<N>(twl, tw, t, half_threads<N>());
     {
         const int pp = t % kHP, c = t / kHP;
         float2* rp = lds + pp * N;
@@ -597,8 +598,8 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch h
     int i3, i2_0;
     if (!x_tile(a, cy, i3, i2_0)) return;
     float2* twlM = lds + kHP * (M + 1);
-    float2* twlN = twlM + M;
-    tw_to_lds(twlM, twM, M, (int)threadIdx.x, half_threads<M>());
+    float2* twlN = twlM + reg_tw_size<M>();
+    reg_tw_fill<M>(twlM, twM, (int)threadIdx.x, half_threads<M>());
     tw_to_lds(twlN, twN, M + 1, (int)threadIdx.x, half_threads<M>());
     float2 v[E];
     load_c2r<N>(a, lds, v, i2_0, i3, twlM, twlN);
@@ -652,8 +653,8 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfB
     int i3, i2_0;
     if (!x_tile(a, cy, i3, i2_0)) return;
     float2* twlM = lds + kHP * (M + 1);
-    float2* twlN = twlM + M;
-    tw_to_lds(twlM, twM, M, (int)threadIdx.x, half_threads<M>());
+    float2* twlN = twlM + reg_tw_size<M>();
+    reg_tw_fill<M>(twlM, twM, (int)threadIdx.x, half_threads<M>());
     tw_to_lds(twlN, twN, M + 1, (int)threadIdx.x, half_threads<M>());
     float2 v[E];
     load_c2r<N>(a, lds, v, i2_0, i3, twlM, twlN);
@@ -694,7 +695,8 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfB
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = w[s];
     const int z0 = opaque_zero();
-    r2c_store<N>(a, lds, v, i2_0, i3, lds + (kHP * (M + 1) + z0), lds + (kHP * (M + 1) + M + z0));
+    r2c_store<N>(a, lds, v, i2_0, i3, lds + (kHP * (M + 1) + z0),
+                 lds + (kHP * (M + 1) + reg_tw_size<M>() + z0));
 }
 
 // ---------------------------------------------------------------------------
@@ -942,7 +944,10 @@ int y_tiles(const HalfBatch& b)
 
 // LDS of an x pass: the tile and the Nyquist bins, then the twiddle tables of
 // the N/2-point transform and of the real split (N/2 + 1 entries).
-template <int N> constexpr size_t x_pass_lds() { return ((size_t)kHP * (N / 2 + 1) + N + 1) * sizeof(float2); }
+template <int N> constexpr size_t x_pass_lds()
+{
+    return ((size_t)kHP * (N / 2 + 1) + reg_tw_size<N / 2>() + N / 2 + 1) * sizeof(float2);
+}
 
 template <int N> void launch_first(const HalfBatch& b, hipStream_t st)
 {
@@ -976,7 +981,7 @@ template <int N> void launch_last_first(const HalfBatch& b, hipStream_t st)
 template <int N> void launch_y(const HalfBatch& b, int inverse, hipStream_t st)
 {
     const HalfIterArgs& a = b.job[0];
-    const size_t lds = ((size_t)kHP * N + N) * sizeof(float2);   // tile + twiddle table
+    const size_t lds = ((size_t)kHP * N + reg_tw_size<N>()) * sizeof(float2);   // tile + twiddle table
     const int cz = y_tiles(b);
     const dim3 blocks((a.n1 / 2 + 1) * cz, b.count);
     if (inverse) {
@@ -993,7 +998,7 @@ template <int N> void launch_z(const HalfBatch& b, hipStream_t st)
     const HalfIterArgs& a = b.job[0];
     // tile + one double per wave for the errK partial + twiddle table
     const size_t lds = (size_t)kZP * N * sizeof(float2) + sizeof(double) * (zfused_threads<N>() / 64) +
-                       (size_t)N * sizeof(float2);
+                       (size_t)reg_tw_size<N>() * sizeof(float2);
     set_lds(k_half_z_fused<N>, lds);
     k_half_z_fused<N><<<dim3((a.n1 / 2 + 1) * (a.n2 / kZP), b.count), zfused_threads<N>(), lds, st>>>(
         b, fft_twiddles(N), fft_twiddles(N));
