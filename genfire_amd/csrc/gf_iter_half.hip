@@ -728,8 +728,10 @@ __global__ void k_hcsr_count(const float2* __restrict__ M, const uint8_t* __rest
 {
     const int h = n1 / 2 + 1;
     const int64_t np = (int64_t)h * n2;
+    // lanes take consecutive kx (contiguous voxels of M and of the mates); the
+    // count goes to pencil ky + n2*kx
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-        const int kx = (int)(q / n2), ky = (int)(q % n2);
+        const int kx = (int)(q % h), ky = (int)(q / h);
         uint32_t c = 0;
         for (int z = 0; z < n3; ++z) {
             const HalfVox hv = half_vox(kx, ky, z, n1, n2, n3);
@@ -737,7 +739,7 @@ __global__ void k_hcsr_count(const float2* __restrict__ M, const uint8_t* __rest
             const bool Bm = hmeas(M, mask, hv.mate);   // on the kx = 0, n1/2 planes: the in-plane mate
             c += (A || Bm) ? 1u : 0u;
         }
-        cnt[q] = c;
+        cnt[(int64_t)ky + (int64_t)n2 * kx] = c;
     }
 }
 
@@ -748,8 +750,8 @@ __global__ void k_hcsr_fill(const float2* __restrict__ M, const uint8_t* __restr
     const int h = n1 / 2 + 1;
     const int64_t np = (int64_t)h * n2;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-        const int kx = (int)(q / n2), ky = (int)(q % n2);
-        int64_t o = off[q];
+        const int kx = (int)(q % h), ky = (int)(q / h);   // as in k_hcsr_count
+        int64_t o = off[(int64_t)ky + (int64_t)n2 * kx];
         for (int z = 0; z < n3; ++z) {
             const HalfVox hv = half_vox(kx, ky, z, n1, n2, n3);
             const bool A = hmeas(M, mask, hv.s);
