@@ -30,12 +30,13 @@ def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None):
     return vol, projs, ang, g.measuredK, sup
 
 
-def _job(ctx, mk, projs, ang, sup, iters, mode=3, alg="genfire", monitor=0, L=5):
+def _job(ctx, mk, projs, ang, sup, iters, mode=3, alg="genfire", monitor=0, L=5, support=1):
     d1, d2, P = projs.shape
     p = _lib.default_params()
     p.griddingMethod = 2   # DFT gridding
     p.numIterations = iters
     p.oversamplingRatio = mk.shape[0] / d1
+    p.constraintSupport = support
     p.interpolationCutoffDistance = 0.3
     p.constraintEnforcementMode = mode
     p.algorithm = _lib.ALG_DR if alg == "dr" else _lib.ALG_GENFIRE
@@ -149,6 +150,31 @@ def test_box_pruning_matches_full_passes(ctx, O, alg, shape, monkeypatch):
         assert np.linalg.norm(rec1 - rec0) / np.linalg.norm(rec0) < 1e-5
         np.testing.assert_allclose(out["1"][1], out["0"][1], atol=2e-6, rtol=0)
         np.testing.assert_allclose(out["1"][2], out["0"][2], atol=1e-5, rtol=0)
+
+
+@pytest.mark.parametrize("alg", ["genfire", "dr"])
+@pytest.mark.parametrize("case", ["odd_box_63_in_128", "support_off_64"])
+def test_box_edges_match_oracle(ctx, O, alg, case):
+    """An odd box (63 of 128: real pairs straddle the box edge, y and z tiles are
+    cut by it) and the support constraint off (nothing pruned, every row
+    transformed) against the oracle's full-grid restatement."""
+    print(f"[iter] box edges alg={alg} case={case}", flush=True)
+    if case == "odd_box_63_in_128":
+        dim, ratio, support = 63, 128 / 63, 1
+    else:
+        dim, ratio, support = 64, 2.0, 0
+    vol, projs, ang, mk, sup = _setup(O, dim=dim, ratio=ratio, P=11, seed=8)
+    assert mk.shape == (128, 128, 128)
+    iters = 20
+    job = _job(ctx, mk, projs, ang, sup, iters, mode=1, alg=alg, monitor=1, L=10, support=support)
+    ref = O.reconstruct(mk, mk != 0, sup, dim, dim, iters, mode=1, algorithm=alg, precision="fp64",
+                        support_on=bool(support), monitor_R=True, loop_length=10, projections=projs,
+                        angles=ang)
+    rec = job.reconstruction()
+    err = np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction)
+    assert err < 1e-4, err
+    np.testing.assert_allclose(job.errK(), ref.errK, atol=2e-5, rtol=0)
+    np.testing.assert_allclose(job.errR()[0], ref.errR, atol=1e-4, rtol=0)
 
 
 @pytest.mark.parametrize("alg", ["genfire", "dr"])
