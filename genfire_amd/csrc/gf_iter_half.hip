@@ -362,8 +362,9 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
         for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
     __syncthreads();
     // DR on the box: the transform of the box part minus D' of the last
-    // iteration is the transform of the iterate's reflection
-    if (a.dmode) {
+    // iteration is the transform of the iterate's reflection.  The Rfree terms
+    // read it from LDS; otherwise the errK loop subtracts in registers.
+    if (a.dmode && a.rf_off) {
         for (int e = e0; e < end; e += NT) {
             const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
             const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
@@ -403,7 +404,11 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             }
             const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
             float2* slot = lds + pl * N + swz(z, pl);
-            const float2 kv = *slot;
+            float2 kv = *slot;
+            if (a.dmode && !a.rf_off) {
+                const float2 dv = (e == e0) ? d0 : a.csr_d[e];
+                kv = make_float2(kv.x - dv.x, kv.y - dv.y);
+            }
             const float ak = sqrtf(kv.x * kv.x + kv.y * kv.y);
             if (m & kHalfTermA) err += fabsf(ak - mg.x);
             if (m & kHalfTermB) err += fabsf(ak - mg.y);
