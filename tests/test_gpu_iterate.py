@@ -30,7 +30,7 @@ def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None):
     return vol, projs, ang, g.measuredK, sup
 
 
-def _job(ctx, mk, projs, ang, sup, iters, mode=3, alg="genfire", monitor=0, L=5, support=1):
+def _job(ctx, mk, projs, ang, sup, iters, mode=3, alg="genfire", monitor=0, L=5, support=1, init=None):
     d1, d2, P = projs.shape
     p = _lib.default_params()
     p.griddingMethod = 2   # DFT gridding
@@ -43,7 +43,7 @@ def _job(ctx, mk, projs, ang, sup, iters, mode=3, alg="genfire", monitor=0, L=5,
     p.monitor_R = monitor
     p.monitorR_loopLength = L
     job = _lib.Job(ctx, p, d1, d2, P)
-    job.set_inputs(projs, ang, sup)
+    job.set_inputs(projs, ang, sup, init)
     job.set_measuredK(mk)
     job.reconstruct()
     return job
@@ -170,6 +170,24 @@ def test_box_edges_match_oracle(ctx, O, alg, case):
     ref = O.reconstruct(mk, mk != 0, sup, dim, dim, iters, mode=1, algorithm=alg, precision="fp64",
                         support_on=bool(support), monitor_R=True, loop_length=10, projections=projs,
                         angles=ang)
+    rec = job.reconstruction()
+    err = np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction)
+    assert err < 1e-4, err
+    np.testing.assert_allclose(job.errK(), ref.errK, atol=2e-5, rtol=0)
+    np.testing.assert_allclose(job.errR()[0], ref.errR, atol=1e-4, rtol=0)
+
+
+@pytest.mark.parametrize("alg", ["genfire", "dr"])
+def test_initial_object_matches_oracle(ctx, O, alg):
+    """initialObject (reconstruct.m:76-80) on the box-pruned half-spectrum path:
+    the first transform starts from the padded object, not from zero."""
+    print(f"[iter] initial object alg={alg}", flush=True)
+    vol, projs, ang, mk, sup = _setup(O, dim=64, P=11, seed=9)
+    init = (0.5 * vol + np.random.default_rng(3).uniform(0, 0.1, vol.shape)).astype(np.float32)
+    iters = 15
+    job = _job(ctx, mk, projs, ang, sup, iters, mode=1, alg=alg, monitor=1, L=5, init=init)
+    ref = O.reconstruct(mk, mk != 0, sup, 64, 64, iters, mode=1, algorithm=alg, precision="fp64",
+                        initial_object=init, monitor_R=True, loop_length=5, projections=projs, angles=ang)
     rec = job.reconstruction()
     err = np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction)
     assert err < 1e-4, err

@@ -104,6 +104,34 @@ def test_results_same_alone_and_in_mix(ctx, mix_inputs):
     srv.close()
 
 
+def test_batched_jobs_with_different_boxes(ctx):
+    """Two 128^3 jobs whose support boxes differ (Dim 64 at ratio 2, Dim 48 at
+    ratio 8/3) share the batched launches; each result equals its job alone."""
+    print("[serve] one grid shape, two box sizes", flush=True)
+    reqs = []
+    for dim, ratio, P, seed in [(64, 2.0, 15, 11), (48, 128 / 48, 13, 12)]:
+        vol, ang, sup = _phantom(dim, P, seed, single_axis=False)
+        projs = np.stack([ctx.project(vol, *a) for a in ang], 2).astype(np.float32)
+        p = _params(25, True, True)
+        p.oversamplingRatio = ratio
+        reqs.append((projs, ang, sup, p))
+    alone = [_alone(ctx, *r) for r in reqs]
+    cfg = _lib.default_server_config()
+    cfg.max_active = 2
+    srv = _lib.Server(ctx, cfg)
+    ids = [srv.submit(p, projs, ang, sup) for projs, ang, sup, p in reqs]
+    srv.drain()
+    for rid, (rec0, ek0, er0) in zip(ids, alone):
+        assert srv.status(rid) == _lib.REQ_DONE
+        rec, ek, er, _ = srv.result(rid)
+        np.testing.assert_array_equal(rec, rec0)
+        np.testing.assert_array_equal(ek, ek0)
+        np.testing.assert_array_equal(er, er0)
+    st = srv.stats()
+    assert st["batched_jobs"] > st["batched_launches"], st
+    srv.close()
+
+
 def test_budget_admits_one_at_a_time(ctx, mix_inputs):
     print("[serve] budget for one job", flush=True)
     cfg = _lib.default_server_config()
