@@ -316,8 +316,7 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
 // of registers, so smaller workgroups keep three of them resident per CU).
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch hb, const float2* __restrict__ tw,
-                                                                     const float2* __restrict__ tw2)
+__global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch hb, const float2* __restrict__ tw)
 {
     const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     using PL = RegPlan<N>;
@@ -445,8 +444,8 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
         }
         __syncthreads();
     }
-    // the inverse transform takes fresh lane indices and its own twiddle
-    // pointer (the same table) so that nothing of the forward one stays live
+    // the inverse transform takes fresh lane indices and its own address of the
+    // twiddle table (opaque_zero) so that nothing of the forward one stays live
     const int t2 = phase_tid(), j2 = t2 % T, p2 = t2 / T;
     float2* rowl2 = lds + p2 * N;
     reg_read<N, E, R0>(rowl2, v, j2, p2);
@@ -638,16 +637,14 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch h
 // through memory.  GENFIRE's x is the C2R output itself and is not stored
 // (pf_out carries it where the R-factor needs it); DR stores x.
 //
-// The inverse and the forward transform take their twiddles through separate
-// arguments (the same tables): with one pointer the compiler keeps the inverse
-// transform's twiddle loads live for reuse by the forward one, which doubles
-// the register count and halves occupancy.
+// The forward transform reads the LDS twiddle tables through addresses built
+// with opaque_zero(): with the same addresses the compiler keeps the inverse
+// transform's twiddle loads live for reuse by the forward one, which doubles the
+// register count and halves occupancy.
 template <int N>
 __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfBatch hb, int cy,
                                                                            const float2* __restrict__ twM,
-                                                                           const float2* __restrict__ twN,
-                                                                           const float2* __restrict__ twM2,
-                                                                           const float2* __restrict__ twN2)
+                                                                           const float2* __restrict__ twN)
 {
     const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     constexpr int M = N / 2;
@@ -981,8 +978,7 @@ template <int N> void launch_last_first(const HalfBatch& b, hipStream_t st)
     const float2* twM = fft_twiddles(N / 2);
     const float2* twN = fft_twiddles(N);
     const XTiles xt = x_tiles(b);
-    k_half_last_first<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, twM, twN, twM,
-                                                                                        twN);
+    k_half_last_first<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, twM, twN);
     GF_LAUNCH_CHECK();
 }
 template <int N> void launch_y(const HalfBatch& b, int inverse, hipStream_t st)
@@ -1008,7 +1004,7 @@ template <int N> void launch_z(const HalfBatch& b, hipStream_t st)
                        (size_t)reg_tw_size<N>() * sizeof(float2);
     set_lds(k_half_z_fused<N>, lds);
     k_half_z_fused<N><<<dim3((a.n1 / 2 + 1) * (a.n2 / kZP), b.count), zfused_threads<N>(), lds, st>>>(
-        b, fft_twiddles(N), fft_twiddles(N));
+        b, fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
 
