@@ -58,6 +58,10 @@ constexpr int kHP = 16;  // pencils per workgroup: 16 x 8 bytes = one 128-byte r
 constexpr int kZP = 8;   // pencils per workgroup of the fused z pass (contiguous pencils)
 
 template <int N> constexpr int half_threads() { return kHP * (N / RegPlan<N>::E); }
+// Rows per workgroup of the x passes (P1/P5: real rows of the M-point complex
+// transform) and their workgroup size.
+template <int M> constexpr int xrows() { return kHP; }
+template <int M> constexpr int x_threads() { return xrows<M>() * (M / RegPlan<M>::E); }
 template <int N> constexpr int zfused_threads() { return kZP * (N / RegPlan<N>::E); }
 
 __device__ __forceinline__ float prh(float v, uint8_t s, int positivity, int support)
@@ -91,13 +95,15 @@ __host__ __device__ __forceinline__ int box_tile(const BoxTiles& t, int i)
 
 // Tile of an x pass (P1/P5): one z in the box, 16 consecutive y rows that meet
 // it; cy = the launch's y-tile count (the largest of its jobs).
+template <int M>
 __device__ __forceinline__ bool x_tile(const HalfIterArgs& a, int cy, int& i3, int& i2_0)
 {
-    const BoxTiles ty = box_tiles(a.n2, a.by, kHP), tz = box_tiles(a.n3, a.bz, 1);
+    constexpr int XR = xrows<M>();
+    const BoxTiles ty = box_tiles(a.n2, a.by, XR), tz = box_tiles(a.n3, a.bz, 1);
     const int zi = (int)blockIdx.x / cy, yt = (int)blockIdx.x % cy;
     if (zi >= tz.count || yt >= ty.count) return false;
     i3 = box_tile(tz, zi);
-    i2_0 = box_tile(ty, yt) * kHP;
+    i2_0 = box_tile(ty, yt) * XR;
     return true;
 }
 
@@ -185,7 +191,8 @@ __device__ __forceinline__ void r2c_store(const HalfIterArgs& a, float2* lds, fl
 #pragma unroll
         for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (M / RL), p)] = v[i * RL + q];
     __syncthreads();
-    const int pp = t % kHP, c = t / kHP;
+    constexpr int XR = xrows<M>();
+    const int pp = t % XR, c = t / XR;
     const float2* rp = lds + pp * M;
     float2* out = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
     if (!inb(i2_0 + pp, n2, a.by)) return;   // a row outside the box is zero and is never read
@@ -231,7 +238,7 @@ __device__ __forceinline__ void load_pair(const HalfIterArgs& a, int64_t ad, int
 }
 
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch hb, int cy,
+__global__ __launch_bounds__(x_threads<N / 2>()) void k_half_first(HalfBatch hb, int cy,
                                                                       const float2* __restrict__ twM,
                                                                       const float2* __restrict__ twN)
 {
@@ -243,11 +250,11 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_first(HalfBatch 
     const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = a.n2;
     int i3, i2_0;
-    if (!x_tile(a, cy, i3, i2_0)) return;
-    float2* twlM = lds + kHP * (M + 1);
+    if (!x_tile<M>(a, cy, i3, i2_0)) return;
+    float2* twlM = lds + xrows<M>() * (M + 1);
     float2* twlN = twlM + reg_tw_size<M>();
-    reg_tw_fill<M>(twlM, twM, t, half_threads<M>());
-    tw_to_lds(twlN, twN, M + 1, t, half_threads<M>());
+    reg_tw_fill<M>(twlM, twM, t, x_threads<M>());
+    tw_to_lds(twlN, twN, M + 1, t, x_threads<M>());
     const int64_t row = (int64_t)N * ((i2_0 + p) + (int64_t)n2 * i3);
     const bool row_in = inb(i2_0 + p, n2, a.by);
 
@@ -522,11 +529,12 @@ __device__ __forceinline__ void load_c2r(const HalfIterArgs& a, float2* lds, flo
     constexpr int M = N / 2;
     using PL = RegPlan<M>;
     constexpr int E = PL::E, T = M / E, R0 = PL::R0;
-    float2* nyq = lds + kHP * M;   // X[M] of each pencil
+    constexpr int XR = xrows<M>();
+    float2* nyq = lds + XR * M;    // X[M] of each pencil
     const int t = phase_tid(), j = t % T, p = t / T;
     const int n2 = a.n2, h = M + 1;
     {
-        const int pp = t % kHP, c = t / kHP;
+        const int pp = t % XR, c = t / XR;
         float2* rp = lds + pp * M;
         const float2* src = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
         const bool y_in = inb(i2_0 + pp, n2, a.by);   // rows outside the box are not formed
@@ -589,7 +597,7 @@ __device__ __forceinline__ float2 p5_dmode(const HalfIterArgs& a, float2 d, floa
 }
 
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch hb, int cy,
+__global__ __launch_bounds__(x_threads<N / 2>()) void k_half_last(HalfBatch hb, int cy,
                                                                      const float2* __restrict__ twM,
                                                                      const float2* __restrict__ twN)
 {
@@ -600,11 +608,11 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch h
     extern __shared__ float2 lds[];
     const int n2 = a.n2;
     int i3, i2_0;
-    if (!x_tile(a, cy, i3, i2_0)) return;
-    float2* twlM = lds + kHP * (M + 1);
+    if (!x_tile<M>(a, cy, i3, i2_0)) return;
+    float2* twlM = lds + xrows<M>() * (M + 1);
     float2* twlN = twlM + reg_tw_size<M>();
-    reg_tw_fill<M>(twlM, twM, (int)threadIdx.x, half_threads<M>());
-    tw_to_lds(twlN, twN, M + 1, (int)threadIdx.x, half_threads<M>());
+    reg_tw_fill<M>(twlM, twM, (int)threadIdx.x, x_threads<M>());
+    tw_to_lds(twlN, twN, M + 1, (int)threadIdx.x, x_threads<M>());
     float2 v[E];
     load_c2r<N>(a, lds, v, i2_0, i3, twlM, twlN);
     const int t = phase_tid(), j = t % T, p = t / T;
@@ -642,7 +650,7 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last(HalfBatch h
 // transform's twiddle loads live for reuse by the forward one, which doubles the
 // register count and halves occupancy.
 template <int N>
-__global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfBatch hb, int cy,
+__global__ __launch_bounds__(x_threads<N / 2>()) void k_half_last_first(HalfBatch hb, int cy,
                                                                            const float2* __restrict__ twM,
                                                                            const float2* __restrict__ twN)
 {
@@ -653,11 +661,11 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfB
     extern __shared__ float2 lds[];
     const int n2 = a.n2;
     int i3, i2_0;
-    if (!x_tile(a, cy, i3, i2_0)) return;
-    float2* twlM = lds + kHP * (M + 1);
+    if (!x_tile<M>(a, cy, i3, i2_0)) return;
+    float2* twlM = lds + xrows<M>() * (M + 1);
     float2* twlN = twlM + reg_tw_size<M>();
-    reg_tw_fill<M>(twlM, twM, (int)threadIdx.x, half_threads<M>());
-    tw_to_lds(twlN, twN, M + 1, (int)threadIdx.x, half_threads<M>());
+    reg_tw_fill<M>(twlM, twM, (int)threadIdx.x, x_threads<M>());
+    tw_to_lds(twlN, twN, M + 1, (int)threadIdx.x, x_threads<M>());
     float2 v[E];
     load_c2r<N>(a, lds, v, i2_0, i3, twlM, twlN);
     const float sc = 1.0f / (float)M;
@@ -697,8 +705,8 @@ __global__ __launch_bounds__(half_threads<N / 2>()) void k_half_last_first(HalfB
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = w[s];
     const int z0 = opaque_zero();
-    r2c_store<N>(a, lds, v, i2_0, i3, lds + (kHP * (M + 1) + z0),
-                 lds + (kHP * (M + 1) + reg_tw_size<M>() + z0));
+    r2c_store<N>(a, lds, v, i2_0, i3, lds + (xrows<M>() * (M + 1) + z0),
+                 lds + (xrows<M>() * (M + 1) + reg_tw_size<M>() + z0));
 }
 
 // ---------------------------------------------------------------------------
@@ -929,12 +937,12 @@ template <class K> void set_lds(K kern, size_t bytes)
 struct XTiles {
     int cy, cz;
 };
-XTiles x_tiles(const HalfBatch& b)
+template <int M> XTiles x_tiles(const HalfBatch& b)
 {
     XTiles r{0, 0};
     for (int i = 0; i < b.count; ++i) {
         const HalfIterArgs& a = b.job[i];
-        r.cy = std::max(r.cy, box_tiles(a.n2, a.by, kHP).count);
+        r.cy = std::max(r.cy, box_tiles(a.n2, a.by, xrows<M>()).count);
         r.cz = std::max(r.cz, box_tiles(a.n3, a.bz, 1).count);
     }
     return r;
@@ -950,15 +958,15 @@ int y_tiles(const HalfBatch& b)
 // the N/2-point transform and of the real split (N/2 + 1 entries).
 template <int N> constexpr size_t x_pass_lds()
 {
-    return ((size_t)kHP * (N / 2 + 1) + reg_tw_size<N / 2>() + N / 2 + 1) * sizeof(float2);
+    return ((size_t)xrows<N / 2>() * (N / 2 + 1) + reg_tw_size<N / 2>() + N / 2 + 1) * sizeof(float2);
 }
 
 template <int N> void launch_first(const HalfBatch& b, hipStream_t st)
 {
     const size_t lds = x_pass_lds<N>();
     set_lds(k_half_first<N>, lds);
-    const XTiles xt = x_tiles(b);
-    k_half_first<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, fft_twiddles(N / 2),
+    const XTiles xt = x_tiles<N / 2>(b);
+    k_half_first<N><<<dim3(xt.cy * xt.cz, b.count), x_threads<N / 2>(), lds, st>>>(b, xt.cy, fft_twiddles(N / 2),
                                                                                    fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
@@ -966,8 +974,8 @@ template <int N> void launch_last(const HalfBatch& b, hipStream_t st)
 {
     const size_t lds = x_pass_lds<N>();
     set_lds(k_half_last<N>, lds);
-    const XTiles xt = x_tiles(b);
-    k_half_last<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, fft_twiddles(N / 2),
+    const XTiles xt = x_tiles<N / 2>(b);
+    k_half_last<N><<<dim3(xt.cy * xt.cz, b.count), x_threads<N / 2>(), lds, st>>>(b, xt.cy, fft_twiddles(N / 2),
                                                                                   fft_twiddles(N));
     GF_LAUNCH_CHECK();
 }
@@ -977,8 +985,8 @@ template <int N> void launch_last_first(const HalfBatch& b, hipStream_t st)
     set_lds(k_half_last_first<N>, lds);
     const float2* twM = fft_twiddles(N / 2);
     const float2* twN = fft_twiddles(N);
-    const XTiles xt = x_tiles(b);
-    k_half_last_first<N><<<dim3(xt.cy * xt.cz, b.count), half_threads<N / 2>(), lds, st>>>(b, xt.cy, twM, twN);
+    const XTiles xt = x_tiles<N / 2>(b);
+    k_half_last_first<N><<<dim3(xt.cy * xt.cz, b.count), x_threads<N / 2>(), lds, st>>>(b, xt.cy, twM, twN);
     GF_LAUNCH_CHECK();
 }
 template <int N> void launch_y(const HalfBatch& b, int inverse, hipStream_t st)
