@@ -178,6 +178,24 @@ def test_box_edges_match_oracle(ctx, O, alg, case):
 
 
 @pytest.mark.parametrize("alg", ["genfire", "dr"])
+def test_mode2_schedule_matches_oracle(ctx, O, alg):
+    """constraintEnforcementMode 2 (GENFIRE_Reconstructor.m:160-161: the
+    resolution cutoff falls from 0.95 and then holds all voxels) on the pruned
+    half-spectrum path."""
+    print(f"[iter] mode 2 alg={alg}", flush=True)
+    vol, projs, ang, mk, sup = _setup(O, dim=64, P=11, seed=10)
+    iters = 28
+    job = _job(ctx, mk, projs, ang, sup, iters, mode=2, alg=alg, monitor=1, L=7)
+    ref = O.reconstruct(mk, mk != 0, sup, 64, 64, iters, mode=2, algorithm=alg, precision="fp64",
+                        monitor_R=True, loop_length=7, projections=projs, angles=ang)
+    rec = job.reconstruction()
+    err = np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction)
+    assert err < 1e-4, err
+    np.testing.assert_allclose(job.errK(), ref.errK, atol=2e-5, rtol=0)
+    np.testing.assert_allclose(job.errR()[0], ref.errR, atol=1e-4, rtol=0)
+
+
+@pytest.mark.parametrize("alg", ["genfire", "dr"])
 def test_initial_object_matches_oracle(ctx, O, alg):
     """initialObject (reconstruct.m:76-80) on the box-pruned half-spectrum path:
     the first transform starts from the padded object, not from zero."""
