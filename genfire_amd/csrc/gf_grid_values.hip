@@ -303,11 +303,15 @@ void pad_centre_u8(const uint8_t* d_src, int d1, int d2, int d3, uint8_t* d_dst,
 // G[j, b] = sum_a E1[j, a] P[a, b] is a real GEMM with the E1 real and imaginary
 // parts as two A operands sharing the B operand P; F_j = sum_b E2[j, b] G[j, b].
 // A wave owns 16 points (the 16 rows of v_mfma_f32_16x16x4f32); lane l supplies
-// A[l%16][l/16] = E1(point l%16, a) built from two LDS tables of FP64-reduced
-// phases (E1 = E1hi(a/16) * E1lo(a%16)), B[l/16][l%16] = P[a][b] from LDS, and
+// A[l%16][l/16] = E1(point l%16, a) built from three LDS tables of FP64-reduced
+// phases, a = 128i + 16j + k: E1 = E1hh(i) * E1hm(j) * E1lo(k), the first two
+// multiplied once per 16-row step; B[l/16][l%16] = P[a][b] from LDS; and it
 // holds G[4(l/16)+r][16nt + l%16].  The E2 contraction runs on the accumulators
-// and ends in a 16-lane reduction.  P is pre-transposed to row-major
-// [a][b] (zero-padded) once per job.
+// (E2 split the same way, b = 128i + 16j + k) and ends in a 16-lane reduction.
+// P is pre-transposed to row-major [a][b] (zero-padded) once per job.
+// The three-level tables keep a workgroup's LDS small enough for three
+// workgroups per CU up to 512 x 512 projections, and the register budget is
+// held to three waves per SIMD.
 // ===========================================================================
 namespace {
 
@@ -326,32 +330,51 @@ __global__ void k_transpose_proj(const float* __restrict__ proj, int dim1, int d
     }
 }
 
-template <int NCH>
-__global__ __launch_bounds__(256) void k_dft_mfma(const float* __restrict__ Pt, int Kp, int Np, int dim1, int dim2,
+// Rows of the three-level phase tables for H = Kp/16 (or Np/16) 16-row blocks.
+__host__ __device__ constexpr int hh_rows(int H) { return (H + 7) / 8; }
+
+// The six phase tables of a 64-point tile, contiguous from t1hh:
+// [R1] E1(128 i - c1), [8] E1(16 j), [16] E1(k), then the same three for E2,
+// each row 64 points wide.
+__device__ __forceinline__ void fill_phase_tables(float2* t1hh, int R1, int R2, int c1, int c2,
                                                   const float* __restrict__ un, const float* __restrict__ vn,
-                                                  const PointTile* __restrict__ tiles, float2* __restrict__ F)
+                                                  const PointTile& tl, int tid)
+{
+    const int rows1 = R1 + 24;
+    for (int idx = tid; idx < kMP * (rows1 + R2 + 24); idx += 256) {
+        const int pt = idx % kMP, row = idx / kMP;
+        const bool ok = pt < tl.count;
+        const bool first = row < rows1;
+        const float t = ok ? (first ? un[tl.begin + pt] : vn[tl.begin + pt]) : 0.0f;
+        const int r = first ? row : row - rows1, R = first ? R1 : R2, c = first ? c1 : c2;
+        const int k = r < R ? 128 * r - c : (r < R + 8 ? 16 * (r - R) : r - R - 8);
+        t1hh[(first ? 0 : rows1 * kMP) + r * kMP + pt] = cis_turns((double)k, t);
+    }
+}
+
+template <int NCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_dft_mfma(const float* __restrict__ Pt, int Kp, int Np, int dim1, int dim2,
+                const float* __restrict__ un, const float* __restrict__ vn,
+                const PointTile* __restrict__ tiles, float2* __restrict__ F)
 {
     constexpr int STR = NCH + 16;           // LDS row stride: 4 rows of 16 lanes on distinct banks
     constexpr int NTT = NCH / 16;
+    static_assert(NTT <= 8 && 8 % NTT == 0, "a column chunk stays inside one 128-column block");
     extern __shared__ float smem[];
     const int H1 = Kp / 16, H2 = Np / 16;
+    const int R1 = hh_rows(H1), R2 = hh_rows(H2);
     float* Pl = smem;                                             // 2 x kKC x STR
-    float2* t1hi = reinterpret_cast<float2*>(Pl + 2 * kKC * STR); // [H1][64]
-    float2* t1lo = t1hi + H1 * kMP;                           // [16][64]
-    float2* t2hi = t1lo + 16 * kMP;                           // [H2][64]
-    float2* t2lo = t2hi + H2 * kMP;                           // [16][64]
+    float2* t1hh = reinterpret_cast<float2*>(Pl + 2 * kKC * STR); // [R1][64]: a = 128i
+    float2* t1hm = t1hh + R1 * kMP;                               // [8][64]:  16j
+    float2* t1lo = t1hm + 8 * kMP;                                // [16][64]: k
+    float2* t2hh = t1lo + 16 * kMP;                               // [R2][64]
+    float2* t2hm = t2hh + R2 * kMP;                               // [8][64]
+    float2* t2lo = t2hm + 8 * kMP;                                // [16][64]
     const PointTile tl = tiles[blockIdx.x];
     const int tid = threadIdx.x;
     const int c1 = dim1 / 2, c2 = dim2 / 2;
-    for (int idx = tid; idx < kMP * (H1 + H2 + 32); idx += 256) {
-        const int pt = idx % kMP, row = idx / kMP;
-        const bool ok = pt < tl.count;
-        const float t1 = ok ? un[tl.begin + pt] : 0.0f, t2 = ok ? vn[tl.begin + pt] : 0.0f;
-        if (row < H1) t1hi[row * kMP + pt] = cis_turns((double)(16 * row - c1), t1);
-        else if (row < H1 + 16) t1lo[(row - H1) * kMP + pt] = cis_turns((double)(row - H1), t1);
-        else if (row < H1 + 16 + H2) t2hi[(row - H1 - 16) * kMP + pt] = cis_turns((double)(16 * (row - H1 - 16) - c2), t2);
-        else t2lo[(row - H1 - 16 - H2) * kMP + pt] = cis_turns((double)(row - H1 - 16 - H2), t2);
-    }
+    fill_phase_tables(t1hh, R1, R2, c1, c2, un, vn, tl, tid);
     const float* Pp = Pt + (int64_t)tl.proj * Kp * Np;
     const int w = tid / 64, l = tid % 64, li = l % 16, lk = l / 16;
     const int pa = 16 * w + li;
@@ -374,7 +397,7 @@ __global__ __launch_bounds__(256) void k_dft_mfma(const float* __restrict__ Pt, 
             }
         };
         stage_load(0);
-        for (int ic = 0; ic < nkc; ++ic) {
+        for (int ic = 0; ic < nkc; ++ic) {   // rows 16 ic .. 16 ic + 15 (kKC = 16)
             float* Pb = Pl + (ic & 1) * (kKC * STR);
 #pragma unroll
             for (int q = 0; q < SPT; ++q) {
@@ -383,12 +406,10 @@ __global__ __launch_bounds__(256) void k_dft_mfma(const float* __restrict__ Pt, 
             }
             __syncthreads();
             if (ic + 1 < nkc) stage_load((ic + 1) * kKC);
-            const int kc = ic * kKC;
+            const float2 e1h = cmul(t1hh[(ic >> 3) * kMP + pa], t1hm[(ic & 7) * kMP + pa]);
 #pragma unroll
             for (int kk = 0; kk < kKC / 4; ++kk) {
-                const int a = kc + 4 * kk + lk;
-                const float2 e1 = a < Kp ? cmul(t1hi[(a >> 4) * kMP + pa], t1lo[(a & 15) * kMP + pa])
-                                         : make_float2(0.0f, 0.0f);
+                const float2 e1 = cmul(e1h, t1lo[(4 * kk + lk) * kMP + pa]);
                 const float* prow = Pb + (4 * kk + lk) * STR + li;
 #pragma unroll
                 for (int nt = 0; nt < NTT; ++nt) {
@@ -408,13 +429,13 @@ __global__ __launch_bounds__(256) void k_dft_mfma(const float* __restrict__ Pt, 
             for (int nt = 0; nt < NTT; ++nt) {
                 const int m = nc / 16 + nt;
                 if (m < H2) {
-                    const float2 e = t2hi[m * kMP + pt];
+                    const float2 e = t2hm[(m & 7) * kMP + pt];
                     const float gr = acr[nt][r], gi = aci[nt][r];
                     hr += gr * e.x - gi * e.y;
                     hi += gr * e.y + gi * e.x;
                 }
             }
-            const float2 el = t2lo[li * kMP + pt];
+            const float2 el = cmul(t2hh[(nc >> 7) * kMP + pt], t2lo[li * kMP + pt]);
             float fr = hr * el.x - hi * el.y, fi = hr * el.y + hi * el.x;
 #pragma unroll
             for (int off = 1; off < 16; off <<= 1) {
@@ -457,7 +478,7 @@ void grid_values_mfma_fp32(const float* d_Pt, int dim1, int dim2, const float* d
     const int Kp = (dim1 + kKC - 1) / kKC * kKC, Np = (dim2 + 15) / 16 * 16;
     const int H1 = Kp / 16, H2 = Np / 16;
     auto lds = [&](int nch) {
-        return sizeof(float) * 2 * kKC * (nch + 16) + sizeof(float2) * kMP * (H1 + H2 + 32);
+        return sizeof(float) * 2 * kKC * (nch + 16) + sizeof(float2) * kMP * (hh_rows(H1) + hh_rows(H2) + 48);
     };
     if (Np >= 128) {
         const size_t b = lds(128);
@@ -552,25 +573,21 @@ __global__ __launch_bounds__(256) void k_dft_mfma_lp(const unsigned short* __res
     constexpr bool SPLIT = MODE == GENFIRE_PREC_SPLIT_BF16;
     constexpr int NTT = NCH / 16;
     constexpr int NB = SPLIT ? 2 : 1;       // staged operand arrays (hi, lo)
+    static_assert(NTT <= 8 && 8 % NTT == 0, "a column chunk stays inside one 128-column block");
     extern __shared__ float4 smem4[];
     unsigned short* Bl = reinterpret_cast<unsigned short*>(smem4);   // [NB][NCH][kLpSK]
     const int H1 = Kp / 16, H2 = Np / 16;
-    float2* t1hi = reinterpret_cast<float2*>(Bl + NB * NCH * kLpSK);  // [H1][64]
-    float2* t1lo = t1hi + H1 * kMP;                                   // [16][64]
-    float2* t2hi = t1lo + 16 * kMP;                                   // [H2][64]
-    float2* t2lo = t2hi + H2 * kMP;                                   // [16][64]
+    const int R1 = hh_rows(H1), R2 = hh_rows(H2);
+    float2* t1hh = reinterpret_cast<float2*>(Bl + NB * NCH * kLpSK);  // [R1][64]: a = 128i
+    float2* t1hm = t1hh + R1 * kMP;                                   // [8][64]:  16j
+    float2* t1lo = t1hm + 8 * kMP;                                    // [16][64]: k
+    float2* t2hh = t1lo + 16 * kMP;                                   // [R2][64]
+    float2* t2hm = t2hh + R2 * kMP;                                   // [8][64]
+    float2* t2lo = t2hm + 8 * kMP;                                    // [16][64]
     const PointTile tl = tiles[blockIdx.x];
     const int tid = threadIdx.x;
     const int c1 = dim1 / 2, c2 = dim2 / 2;
-    for (int idx = tid; idx < kMP * (H1 + H2 + 32); idx += 256) {
-        const int pt = idx % kMP, row = idx / kMP;
-        const bool ok = pt < tl.count;
-        const float t1 = ok ? un[tl.begin + pt] : 0.0f, t2 = ok ? vn[tl.begin + pt] : 0.0f;
-        if (row < H1) t1hi[row * kMP + pt] = cis_turns((double)(16 * row - c1), t1);
-        else if (row < H1 + 16) t1lo[(row - H1) * kMP + pt] = cis_turns((double)(row - H1), t1);
-        else if (row < H1 + 16 + H2) t2hi[(row - H1 - 16) * kMP + pt] = cis_turns((double)(16 * (row - H1 - 16) - c2), t2);
-        else t2lo[(row - H1 - 16 - H2) * kMP + pt] = cis_turns((double)(row - H1 - 16 - H2), t2);
-    }
+    fill_phase_tables(t1hh, R1, R2, c1, c2, un, vn, tl, tid);
     __syncthreads();
     const int w = tid / 64, l = tid % 64, li = l % 16, lk = l / 16;
     const int pa = 16 * w + li;
@@ -597,7 +614,7 @@ __global__ __launch_bounds__(256) void k_dft_mfma_lp(const unsigned short* __res
 #pragma unroll
             for (int ks = 0; ks < kLpKC / 32; ++ks) {
                 const int a0 = kc + 32 * ks + 8 * lk;          // this lane's rows a0..a0+7
-                const float2 ehi = t1hi[(a0 >> 4) * kMP + pa];
+                const float2 ehi = cmul(t1hh[(a0 >> 7) * kMP + pa], t1hm[((a0 >> 4) & 7) * kMP + pa]);
                 v8u are, aim, are2, aim2;
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
@@ -647,13 +664,13 @@ __global__ __launch_bounds__(256) void k_dft_mfma_lp(const unsigned short* __res
             for (int nt = 0; nt < NTT; ++nt) {
                 const int m = nc / 16 + nt;
                 if (m < H2) {
-                    const float2 e = t2hi[m * kMP + pt];
+                    const float2 e = t2hm[(m & 7) * kMP + pt];
                     const float gr = acr[nt][r], gi = aci[nt][r];
                     hr += gr * e.x - gi * e.y;
                     hi += gr * e.y + gi * e.x;
                 }
             }
-            const float2 el = t2lo[li * kMP + pt];
+            const float2 el = cmul(t2hh[(nc >> 7) * kMP + pt], t2lo[li * kMP + pt]);
             float fr = hr * el.x - hi * el.y, fi = hr * el.y + hi * el.x;
 #pragma unroll
             for (int off = 1; off < 16; off <<= 1) {
@@ -679,7 +696,7 @@ int lp_cols(int dim2) { return (dim2 + 15) / 16 * 16; }
 template <int MODE, int NCH> size_t lp_lds(int Kp, int Np)
 {
     constexpr int NB = MODE == GENFIRE_PREC_SPLIT_BF16 ? 2 : 1;
-    return sizeof(unsigned short) * NB * NCH * kLpSK + sizeof(float2) * kMP * (Kp / 16 + Np / 16 + 32);
+    return sizeof(unsigned short) * NB * NCH * kLpSK + sizeof(float2) * kMP * (hh_rows(Kp / 16) + hh_rows(Np / 16) + 48);
 }
 
 template <int MODE, int NCH>

@@ -110,6 +110,26 @@ def _gpu_vals_sorted(pts, p, ref):
     return pts["val"][g][go]
 
 
+@pytest.mark.parametrize("d1,d2", [(256, 256), (320, 192)])
+def test_fp32_values_large_projections(ctx, O, d1, d2):
+    # Projections past 128 rows and columns use more than one entry of the
+    # coarsest phase tables (rows a = 128i + 16j + k); 320 x 192 also ends in a
+    # partial 128-column block.  3000 points per projection against the FP64 DFT.
+    print(f"[grid] FP32 values {d1}x{d2}", flush=True)
+    ang = np.random.default_rng(23).uniform([-5, -70, -5], [5, 70, 5], (2, 3))
+    projs = np.random.default_rng(13).uniform(0.0, 1.0, (d1, d2, len(ang))).astype(np.float32)
+    job = _lib.Job(ctx, _params(ratio=2.0, cutoff=0.3, v3=(0, 0, 1)), d1, d2, len(ang))
+    job.set_inputs(projs, ang)
+    job.grid()
+    pts = job.points()
+    for p in range(len(ang)):
+        g = np.flatnonzero(pts["proj"] == p)
+        g = np.random.default_rng(p).choice(g, size=min(3000, len(g)), replace=False)
+        F64 = O.dft64(projs[:, :, p], pts["un"][g], pts["vn"][g])
+        err = np.linalg.norm(pts["val"][g].astype(np.complex128) - F64) / np.linalg.norm(F64)
+        assert err < VALUE_TOL, (p, err)
+
+
 def test_recIFFT(ctx, O):
     print("[grid] recIFFT", flush=True)
     job, projs, ang, (d1, d2, *_rest) = _run(ctx, "tilt64_single_axis")
