@@ -259,6 +259,8 @@ def main():
                     help="store the projections as FP16 with one max-abs scale per projection")
     ap.add_argument("--kspace-crop", type=int, default=0,
                     help="report the FP64 gridded-value error on the central crop^3 of k-space only")
+    ap.add_argument("--single-axis", action="store_true",
+                    help="tilt series about y only (phi = psi = 0) instead of the workload's three Euler angles")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
@@ -284,9 +286,11 @@ def main():
         return
 
     w = phantom.WORKLOADS[args.workload]
+    import dataclasses
     if args.iterations:
-        import dataclasses
         w = dataclasses.replace(w, iterations=args.iterations)
+    if args.single_axis:
+        w = dataclasses.replace(w, three_euler=False)
     ctx = _lib.Context(0)
     t0 = time.perf_counter()
     vol, projs, ang, sup = phantom.make_inputs(w, phantom.gpu_project_many(ctx))
@@ -378,6 +382,7 @@ def main():
                    "grid": n, "iterations": w.iterations, "algorithm": w.algorithm,
                    "rfactor_every": w.monitor_every, "gridding": args.gridding, "gridding_precision": args.precision,
                    "cutoff": w.cutoff, "oversampling": w.oversampling,
+                   "tilt_series": "three Euler angles" if w.three_euler else "single axis (y)",
                    "projection_storage": "fp16 + per-projection scale" if args.fp16_inputs else "fp32"},
         "gridded_kpts_per_s": npts / (tm["grid_ms"] * 1e-3),
         "gridded_points": npts,

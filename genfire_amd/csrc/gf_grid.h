@@ -217,7 +217,8 @@ void hcsr_den(const float2* d_mag, const uint32_t* d_meta, int64_t n, double* d_
 
 // ---- projector (gf_project.hip) ----
 // Per projection the rotation is stored as kRStride floats: the column-major
-// 3x3 ZYZ matrix and, at [9], 1 when the y-fastest volume copy is to be used.
+// 3x3 ZYZ matrix; at [9], 1 when the y-fastest volume copy is to be used; at
+// [10], 1 when the matrix maps the y axis exactly onto itself (then [9] is 1).
 constexpr int kRStride = 12;
 struct ProjArgs {
     const float* vol;  // d1 x d2 x d3 column-major (x fastest)
@@ -233,6 +234,7 @@ struct ProjArgs {
 };
 int project_blocks(int m1, int m2);
 bool project_wants_yfast(const float* R);
+bool project_y_exact(const float* R);
 void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipStream_t st);
 void project(const ProjArgs& a, hipStream_t st);
 void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st);

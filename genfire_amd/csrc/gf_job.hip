@@ -146,7 +146,8 @@ void kstat_end(genfire_job* j, int k, std::pair<hipEvent_t, hipEvent_t>& ev)
 }
 
 // ZYZ matrices of the projector (calculate3Dprojection_RealSpaceinterp.m:13-15),
-// kRStride floats per projection with the y-fastest flag at [9].
+// kRStride floats per projection with the y-fastest flag at [9] and the
+// exact-y flag at [10].
 std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const double* vectors)
 {
     std::vector<float> Rm((size_t)kRStride * P, 0.0f);
@@ -159,7 +160,8 @@ std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const
             zyz_matrix(angles[p], angles[P + p], angles[2 * P + p], R);
         float* r = Rm.data() + (size_t)kRStride * p;
         for (int q = 0; q < 9; ++q) r[q] = (float)R[q];
-        if (project_wants_yfast(r)) { r[9] = 1.0f; *any_y = true; }
+        if (project_y_exact(r)) { r[9] = 1.0f; r[10] = 1.0f; *any_y = true; }
+        else if (project_wants_yfast(r)) { r[9] = 1.0f; *any_y = true; }
     }
     return Rm;
 }

@@ -43,6 +43,104 @@ constexpr int kRows = 4;     // waves per workgroup along the other pixel axis
 // (R[9] of the projection selects it).  For tilts about y the pixel-y direction
 // is the volume's y axis, so each gather of a wave reads a few contiguous runs
 // instead of one cache line per lane.
+//
+// Rotations that map the volume's y axis onto itself (R[10] set: tilts about y)
+// take a second form when d2 is a multiple of 4: every ray of a pixel column i
+// then samples the same (x, z) cell with the same weights at each step, its y
+// being exactly j + s2, so one lane marches the four rays of an aligned group
+// of four y and reads each corner of the cell for all four with one 16-byte
+// load -- a quarter of the gather instructions of the pair form.  Each ray's
+// interpolation is the pair form's: with ty = 0 its y-lerp returns the lower
+// value, and the ray at y = d2-1 (cell d2-2, ty = 1) takes the two values from
+// the same load.  A workgroup covers 16 pixel columns x 64 y; the tile slots
+// past the ones it needs only write zero partials.
+__device__ __forceinline__ void project_ytilt(const ProjArgs& a, const float* R, int p, int tile,
+                                              double& q0, double& q1, double& q2, double& q3)
+{
+    const int d1 = a.d1, d2 = a.d2, d3 = a.d3;
+    const int s1 = d1 / 2 - a.m1 / 2, s2 = d2 / 2 - a.m2 / 2;
+    const int g0 = s2 / 4, ng = (s2 + a.m2 + 3) / 4 - g0;   // groups of four y covering the window
+    const int yb = (ng + 15) / 16;
+    const int w = (int)threadIdx.x / kLanes, l = (int)threadIdx.x % kLanes;
+    const int i = (tile / yb) * 16 + 4 * w + l / 16;
+    const int y0 = 4 * (g0 + (tile % yb) * 16 + l % 16);
+    if (i >= a.m1 || y0 >= s2 + a.m2) return;
+    const float X = (float)(i + s1 - d1 / 2), Y = (float)(y0 - d2 / 2);
+    const float c1 = (float)(d1 / 2), c3 = (float)(d3 / 2);
+    const float bx = R[0] * X + R[3] * Y + c1;      // R[3] = R[5] = 0: the same for the four rays
+    const float bz = R[2] * X + R[5] * Y + c3;
+    const float rx = R[6], rz = R[8];
+    float klo = 0.0f, khi = (float)(d3 - 1);
+    const float bb[2] = {bx, bz}, rr[2] = {rx, rz};
+    const float hi[2] = {(float)(d1 - 1), (float)(d3 - 1)};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (rr[q] != 0.0f) {
+            const float t0 = (0.0f - bb[q]) / rr[q] + c3, t1 = (hi[q] - bb[q]) / rr[q] + c3;
+            klo = fmaxf(klo, fminf(t0, t1) - 1.0f);
+            khi = fminf(khi, fmaxf(t0, t1) + 1.0f);
+        } else if (!(bb[q] >= 0.0f && bb[q] <= hi[q])) {
+            khi = -1.0f;
+        }
+    }
+    const int k0 = (int)fminf((float)d3, fmaxf(0.0f, ceilf(klo)));
+    const int k1 = (int)fmaxf(-1.0f, fminf((float)(d3 - 1), floorf(khi)));
+    const bool top = y0 + 3 == d2 - 1;      // the fourth ray sits in cell d2-2 with ty = 1
+    const float* vol = a.vol_y + y0;        // index y + d2*(x + d1*z)
+    const int sx = d2, sz = d1 * d2;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = k0; k <= k1; ++k) {
+        const float Z = (float)(k - d3 / 2);
+        const float fx = bx + rx * Z, fz = bz + rz * Z;
+        int a0, e0;
+        float ta, te;
+        if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fz, d3, &e0, &te)) continue;
+        const float* v = vol + d2 * (a0 + d1 * e0);
+        const float4 A = *reinterpret_cast<const float4*>(v);             // (a0,   y0..y0+3, e0)
+        const float4 B = *reinterpret_cast<const float4*>(v + sx);        // (a0+1, ., e0)
+        const float4 C = *reinterpret_cast<const float4*>(v + sz);        // (a0,   ., e0+1)
+        const float4 D = *reinterpret_cast<const float4*>(v + sz + sx);   // (a0+1, ., e0+1)
+        const float a4[4] = {A.x, A.y, A.z, A.w}, b4[4] = {B.x, B.y, B.z, B.w};
+        const float c4[4] = {C.x, C.y, C.z, C.w}, d4[4] = {D.x, D.y, D.z, D.w};
+        float L[4], U[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            L[r] = a4[r] + ta * (b4[r] - a4[r]);
+            U[r] = c4[r] + ta * (d4[r] - c4[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) acc[r] += L[r] + te * (U[r] - L[r]);
+        const float ty = 1.0f;
+        const float l3 = top ? L[2] + ty * (L[3] - L[2]) : L[3];
+        const float u3 = top ? U[2] + ty * (U[3] - U[2]) : U[3];
+        acc[3] += l3 + te * (u3 - l3);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int j = y0 + r - s2;
+        if (j < 0 || j >= a.m2) continue;
+        const int64_t o = (int64_t)i + (int64_t)a.m1 * (j + (int64_t)a.m2 * p);
+        if (a.out) a.out[o] = acc[r];
+        if (a.meas) {
+            const double m = (double)a.meas[o];
+            const double b = (double)acc[r];
+            q0 += a.post_hoc ? fabs(fabs(b) - fabs(m)) : fabs(b - m);
+            q1 += (b - m) * (b - m);
+            q2 += fabs(m);
+            q3 += m * m;
+        }
+    }
+}
+
+__device__ __forceinline__ int ytilt_tiles(const ProjArgs& a)
+{
+    const int s2 = a.d2 / 2 - a.m2 / 2, g0 = s2 / 4, ng = (s2 + a.m2 + 3) / 4 - g0;
+    return ((a.m1 + 15) / 16) * ((ng + 15) / 16);
+}
+
+__device__ __forceinline__ void tile_sums(const ProjArgs& a, int p, int tile, int tiles, double q0, double q1,
+                                          double q2, double q3);
+
 __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles)
 {
     const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
@@ -50,6 +148,12 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
     const int tile = slot % tiles;
     if (p >= a.P) return;
     const float* R = a.R + kRStride * p;
+    if (R[10] != 0.0f && a.d2 % 4 == 0 && ytilt_tiles(a) <= tiles) {
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        if (tile < ytilt_tiles(a)) project_ytilt(a, R, p, tile, q0, q1, q2, q3);
+        tile_sums(a, p, tile, tiles, q0, q1, q2, q3);
+        return;
+    }
     const bool yfast = R[9] != 0.0f;
     int i, j;
     if (yfast) {
@@ -148,6 +252,14 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
             q3 = m * m;
         }
     }
+    tile_sums(a, p, tile, tiles, q0, q1, q2, q3);
+}
+
+// The R-factor partials of one workgroup: wave sums, then a fixed-order sum of
+// the four waves into slot (p, c, tile).
+__device__ __forceinline__ void tile_sums(const ProjArgs& a, int p, int tile, int tiles, double q0, double q1,
+                                          double q2, double q3)
+{
     if (!a.sums) return;
     __shared__ double red[4][kRows];
     for (int off = 32; off > 0; off >>= 1) {
@@ -206,6 +318,11 @@ int project_blocks(int m1, int m2)
 }
 
 bool project_wants_yfast(const float* R) { return fabsf(R[4]) > fabsf(R[0]); }
+
+bool project_y_exact(const float* R)
+{
+    return R[1] == 0.0f && R[3] == 0.0f && R[4] == 1.0f && R[5] == 0.0f && R[7] == 0.0f;
+}
 
 void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipStream_t st)
 {

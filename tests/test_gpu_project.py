@@ -22,6 +22,43 @@ def test_project_matches_oracle(ctx, O, shape, angles):
     assert err < 2e-5, err
 
 
+@pytest.mark.parametrize("shape,theta", [
+    ((33, 32, 31), 37.0),      # d2 % 4 == 0: tilts about y take the four-ray form
+    ((31, 64, 29), -70.0),
+    ((32, 30, 32), 37.0),      # d2 % 4 != 0: the pair form
+    ((24, 16, 20), 90.0),
+])
+def test_project_tilt_about_y_matches_oracle(ctx, O, shape, theta):
+    print(f"[proj] tilt about y {shape} {theta}", flush=True)
+    vol = np.random.default_rng(2).uniform(0, 1, shape).astype(np.float32)
+    got = ctx.project(vol, 0.0, theta, 0.0)
+    ref = O.project(vol, 0.0, theta, 0.0)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 2e-5, err
+    # the last y row (cell d2-2 with weight 1) and the first
+    assert np.abs(got[:, -1] - ref[:, -1]).max() <= 2e-5 * np.abs(ref).max()
+    assert np.abs(got[:, 0] - ref[:, 0]).max() <= 2e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("crop", [(30, 27), (40, 36), (7, 5)])
+def test_rfactor_tilt_series_about_y_matches_oracle(ctx, O, crop):
+    # windows with an odd offset along y (s2 = 18 - 27 // 2 = 5), the full
+    # volume, and a small one: the four-ray groups straddle the window's edges
+    print(f"[proj] rfactor tilt series {crop}", flush=True)
+    rng = np.random.default_rng(5)
+    vol = rng.uniform(0, 1, (40, 36, 44)).astype(np.float32)
+    ang = np.stack([np.zeros(7), np.linspace(-63, 63, 7), np.zeros(7)], 1)
+    m1, m2 = crop
+    s1, s2 = 20 - m1 // 2, 18 - m2 // 2
+    projs = np.stack([O.project(vol, *a) for a in ang], 2)[s1:s1 + m1, s2:s2 + m2]
+    projs = (projs * rng.uniform(0.9, 1.1, projs.shape)).astype(np.float32)
+    R, Rarr, sims = ctx.rfactor(vol, projs, ang)
+    Rr, Rarr_r, sims_r = O.rfactor_posthoc(vol, projs, ang)
+    assert abs(R - Rr) < 1e-5
+    np.testing.assert_allclose(Rarr, Rarr_r, atol=1e-5)
+    np.testing.assert_allclose(sims, sims_r, rtol=0, atol=2e-5 * np.abs(sims_r).max())
+
+
 def test_rfactor_matches_oracle(ctx, O):
     print("[proj] rfactor", flush=True)
     rng = np.random.default_rng(4)
