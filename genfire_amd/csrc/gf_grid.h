@@ -223,6 +223,8 @@ constexpr int kRStride = 12;
 struct ProjArgs {
     const float* vol;  // d1 x d2 x d3 column-major (x fastest)
     const float* vol_y; // the same volume y-fastest (index y + d2*(x + d1*z)), nullable when unused
+    const float* vol_z; // z-pair copies of vol_y (volume_zpairs), nullable: then vol_y alone serves
+    int64_t zodd;       // offset of the odd-pair copy in vol_z (floats)
     int d1, d2, d3;
     const float* R;    // P x kRStride, single
     int P;
@@ -236,6 +238,8 @@ int project_blocks(int m1, int m2);
 bool project_wants_yfast(const float* R);
 bool project_y_exact(const float* R);
 void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipStream_t st);
+int64_t zpair_floats(int d1, int d2, int d3);   // floats of one z-pair copy
+void volume_zpairs(const float* d_vol_y, float* d_dst, int d1, int d2, int d3, hipStream_t st);
 void project(const ProjArgs& a, hipStream_t st);
 void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st);
 

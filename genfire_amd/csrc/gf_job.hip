@@ -379,11 +379,16 @@ void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
     EventTimer& tm = *j->rs.rf[slot];
     GF_HIP(hipEventRecord(tm.a, st));
     crop_centre(d_model_fft, 0, j->n1, j->n2, j->n1, j->vol.as<float>(), j->dim1, j->dim2, j->dim1, st);
-    if (j->rmat_any_y) volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    if (j->rmat_any_y) {
+        volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
+        volume_zpairs(j->vol_y.as<float>(), j->vol_z.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    }
     const int nblk = project_blocks(j->dim1, j->dim2);
     ProjArgs pa{};
     pa.vol = j->vol.as<float>();
     pa.vol_y = j->rmat_any_y ? j->vol_y.as<float>() : nullptr;
+    pa.vol_z = j->rmat_any_y ? j->vol_z.as<float>() : nullptr;
+    pa.zodd = zpair_floats(j->dim1, j->dim2, j->dim1);
     pa.d1 = j->dim1; pa.d2 = j->dim2; pa.d3 = j->dim1;
     pa.R = j->Rmat.as<float>();
     pa.P = P;
@@ -654,7 +659,10 @@ void recon_begin(genfire_job* j)
         j->Rmat.alloc(sizeof(float) * Rm.size());
         GF_HIP(hipMemcpyAsync(j->Rmat.p, Rm.data(), sizeof(float) * Rm.size(), hipMemcpyHostToDevice, st));
         j->vol.alloc(sizeof(float) * box_size(j));
-        if (j->rmat_any_y) j->vol_y.alloc(sizeof(float) * box_size(j));
+        if (j->rmat_any_y) {
+            j->vol_y.alloc(sizeof(float) * box_size(j));
+            j->vol_z.alloc(sizeof(float) * 2 * zpair_floats(j->dim1, j->dim2, j->dim1));
+        }
         j->rsums.alloc(sizeof(double) * (size_t)j->P * 4 * project_blocks(j->dim1, j->dim2));
         j->rout.alloc(sizeof(double) * (size_t)r.nR * j->P * 4);
         GF_HIP(hipStreamSynchronize(st));
@@ -1065,6 +1073,7 @@ int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P)
     const int64_t spectrum = 8 * (half ? std::max<int64_t>(2 * hn, nf) : nf);
     const int64_t meas = std::min<int64_t>(nf, 2 * pts);
     const int64_t iter = nf * (4 + 1 + 4 + 4 + 8) + spectrum + meas * 28 + n1 * n2 * 12 + box * 8 +
+                         8 * zpair_floats(dim1, dim2, dim1) +   // the projector's two z-pair copies
                          (int64_t)P * 4 * 8 * (1 + p.numIterations / std::max(1, p.monitorR_loopLength));
     return inputs + std::max(grid, iter) + (64ll << 20);
 }

@@ -28,6 +28,8 @@ __device__ __forceinline__ bool lin_axis(float f, int n, int* i0, float* t)
 
 // Unaligned pair load: two adjacent floats of the volume in one 8-byte access.
 typedef float pair_t __attribute__((ext_vector_type(2), aligned(4)));
+// Two pairs of the z-pair copies in one 16-byte access (8-byte aligned).
+typedef float quad_t __attribute__((ext_vector_type(4), aligned(8)));
 
 // The workgroups of one projection run on one XCD: the hardware deals workgroups
 // to the 8 XCDs round-robin, so physical block b goes to XCD b % 8 and is mapped
@@ -195,7 +197,28 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
         }
         const int k0 = (int)fminf((float)d3, fmaxf(0.0f, ceilf(klo)));
         const int k1 = (int)fmaxf(-1.0f, fminf((float)(d3 - 1), floorf(khi)));
-        if (yfast) {
+        if (yfast && a.vol_z) {
+            // z-pair copies: the cell's four voxels at x = a0 are one 16-byte
+            // load, (b0, e0) (b0, e0+1) (b0+1, e0) (b0+1, e0+1), from the copy
+            // whose pairs start at e0's parity; the same at x = a0 + 1
+            const int sx = 2 * d2;
+            for (int k = k0; k <= k1; ++k) {
+                const float Z = (float)(k - d3 / 2);
+                const float fx = bx + rx * Z, fy = by + ry * Z, fz = bz + rz * Z;
+                int a0, b0, e0;
+                float ta, tb, te;
+                if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
+                    continue;
+                const int odd = e0 & 1;
+                const float* v = a.vol_z + (odd ? a.zodd : 0) + 2 * (b0 + d2 * (a0 + d1 * (e0 >> 1)));
+                const quad_t q0 = *reinterpret_cast<const quad_t*>(v);        // x = a0
+                const quad_t q1 = *reinterpret_cast<const quad_t*>(v + sx);   // x = a0 + 1
+                const float x00 = q0.x + ta * (q1.x - q0.x), x10 = q0.z + ta * (q1.z - q0.z);
+                const float x01 = q0.y + ta * (q1.y - q0.y), x11 = q0.w + ta * (q1.w - q0.w);
+                const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
+                acc += y0 + te * (y1 - y0);
+            }
+        } else if (yfast) {
             const float* vol = a.vol_y;           // index y + d2*(x + d1*z)
             const int sx = d2, sz = d1 * d2;
             for (int k = k0; k <= k1; ++k) {
@@ -298,6 +321,19 @@ __global__ __launch_bounds__(256) void k_yfast(const float* __restrict__ src, fl
     }
 }
 
+// z-pair copy of a y-fastest volume: pair m of copy `odd` holds z = 2m + odd + s,
+// s = 0, 1, interleaved per voxel: dst[s + 2*(y + d2*(x + d1*m))]; z >= d3 is 0.
+__global__ __launch_bounds__(256) void k_zpair(const float* __restrict__ src, float* __restrict__ dst, int d1, int d2,
+                                               int d3, int odd, int64_t total)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int s = (int)(t & 1);
+        const int64_t yx = (t >> 1) % ((int64_t)d1 * d2), m = (t >> 1) / ((int64_t)d1 * d2);
+        const int64_t z = 2 * m + odd + s;
+        dst[t] = z < d3 ? src[yx + (int64_t)d1 * d2 * z] : 0.0f;
+    }
+}
+
 __global__ void k_project_reduce(const double* sums, int P, int nblk, double* out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -329,6 +365,18 @@ void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipS
     dim3 grid((d1 + 31) / 32, (d2 + 31) / 32, d3);
     k_yfast<<<grid, 256, 0, st>>>(d_src, d_dst, d1, d2, d3);
     GF_LAUNCH_CHECK();
+}
+
+int64_t zpair_floats(int d1, int d2, int d3) { return 2 * (int64_t)d1 * d2 * (d3 / 2 + 1); }
+
+void volume_zpairs(const float* d_vol_y, float* d_dst, int d1, int d2, int d3, hipStream_t st)
+{
+    const int64_t total = zpair_floats(d1, d2, d3);
+    for (int odd = 0; odd < 2; ++odd) {
+        const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
+        k_zpair<<<(unsigned)blocks, 256, 0, st>>>(d_vol_y, d_dst + odd * total, d1, d2, d3, odd, total);
+        GF_LAUNCH_CHECK();
+    }
 }
 
 void project(const ProjArgs& a, hipStream_t st)
