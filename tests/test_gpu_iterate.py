@@ -8,9 +8,10 @@ from genfire_amd import _lib
 pytestmark = pytest.mark.gpu
 
 
-def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None):
+def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None, euler=False):
     """Blob phantom, trilinear tilt series and the oracle's measuredK.  dim2 != dim
-    makes the projections (and the grid) non-square: vol is dim x dim2 x dim."""
+    makes the projections (and the grid) non-square: vol is dim x dim2 x dim.
+    euler adds phi and psi in [-5, 5] degrees to the tilts about y."""
     dim2 = dim if dim2 is None else dim2
     rng = np.random.default_rng(seed)
     vol = np.zeros((dim, dim2, dim), np.float32)
@@ -22,6 +23,9 @@ def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None):
         g = [np.exp(-((kk - q) ** 2) / (2 * s * s)) for kk, q in zip((k1, k2, k1), x0)]
         vol += (rng.uniform(0.5, 1) * g[0][:, None, None] * g[1][None, :, None] * g[2][None, None, :]).astype(np.float32)
     ang = np.stack([np.zeros(P), np.linspace(-60, 60, P), np.zeros(P)], 1)
+    if euler:
+        ang[:, 0] = rng.uniform(-5, 5, P)
+        ang[:, 2] = rng.uniform(-5, 5, P)
     projs = np.stack([O.project(vol, *a) for a in ang], 2).astype(np.float32)
     n1, n2 = int(round(dim * ratio)), int(round(dim2 * ratio))
     g = O.grid_dft(projs, ang, n1, n2, 0.3)
@@ -150,6 +154,24 @@ def test_box_pruning_matches_full_passes(ctx, O, alg, shape, monkeypatch):
         assert np.linalg.norm(rec1 - rec0) / np.linalg.norm(rec0) < 1e-5
         np.testing.assert_allclose(out["1"][1], out["0"][1], atol=2e-6, rtol=0)
         np.testing.assert_allclose(out["1"][2], out["0"][2], atol=1e-5, rtol=0)
+
+
+@pytest.mark.parametrize("dim,ratio", [(32, 2.0), (33, 64 / 33)])
+def test_monitor_three_euler_matches_oracle(ctx, O, dim, ratio):
+    """The R-factor monitor on tilts with phi and psi off zero (one ray per lane
+    on the z-pair copies of the model): an even volume edge and an odd one,
+    whose last even z pair ends in the zero padding."""
+    print(f"[iter] monitor three Euler dim={dim}", flush=True)
+    vol, projs, ang, mk, sup = _setup(O, dim=dim, ratio=ratio, P=13, seed=9, euler=True)
+    iters = 20
+    job = _job(ctx, mk, projs, ang, sup, iters, mode=1, alg="dr", monitor=1, L=5)
+    ref = O.reconstruct(mk, mk != 0, sup, dim, dim, iters, mode=1, algorithm="dr", precision="fp64",
+                        monitor_R=True, loop_length=5, projections=projs, angles=ang)
+    rec = job.reconstruction()
+    assert np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction) < 1e-4
+    _, R1, R2 = job.errR()
+    np.testing.assert_allclose(R1, ref.Rarr_record, atol=1e-4, rtol=0)
+    np.testing.assert_allclose(R2, ref.Rarr2_record, atol=1e-4, rtol=0)
 
 
 @pytest.mark.parametrize("alg", ["genfire", "dr"])
