@@ -172,6 +172,16 @@ struct HalfIterArgs {
     int dmode;
     float2* csr_d;
     const int64_t* csr_mate;
+    // Set on the y-inverse launch of an iteration only: its first workgroup
+    // reduces the z pass's errK partials (err_n of them, fixed order) into
+    // errk[*errctr] = sum / err_den and advances *errctr; with dherm its first
+    // workgroups also replace D on the kx = 0 and n1/2 planes by its Hermitian
+    // part.  Null / 0 on every other launch.
+    float* errk;
+    int* errctr;
+    double err_den;
+    int64_t err_n;
+    int dherm;
 };
 // Rfree[s + nbins * (*ctr)] = sum of terms of shell s (entries order[shell_off[s]
 // .. shell_off[s+1]), fixed order) / den[s], or -1 for an empty shell; *ctr is
@@ -198,8 +208,6 @@ void half_z(const HalfBatch& b, hipStream_t st);
 void half_last(const HalfBatch& b, hipStream_t st);
 void half_last_first(const HalfBatch& b, hipStream_t st);
 // dmode jobs: D -> its Hermitian part on the kx = 0 and n1/2 planes (after half_z)
-void half_dherm(const HalfBatch& b, hipStream_t st);
-void half_dherm(const HalfIterArgs& a, hipStream_t st);
 // dmode set-up: in-plane mate entry of every kx = 0 and n1/2 plane entry
 void hcsr_mates(const int64_t* d_off, const uint32_t* d_meta, int n1, int n2, int n3, int64_t n_plane0,
                 int64_t plane_m_begin, int64_t n_plane, int64_t* d_mate, hipStream_t st);

@@ -472,6 +472,48 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
 // ---------------------------------------------------------------------------
 // P4: y inverse, B -> A.  Tile: one kx, 16 consecutive z (strided read).
 // ---------------------------------------------------------------------------
+// errK of the iteration from the z pass's per-tile partials: lane sums over a
+// fixed stride, wave sums, then the waves in order (one workgroup).
+__device__ void errk_reduce(const HalfIterArgs& a, double* red)
+{
+    const int t = threadIdx.x, nt = blockDim.x;
+    double s = 0.0;
+    for (int64_t i = t; i < a.err_n; i += nt) s += a.err_part[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((t & 63) == 0) red[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < nt / 64; ++w) sum += red[w];
+        a.errk[*a.errctr] = (float)(sum / a.err_den);
+        *a.errctr += 1;
+    }
+    __syncthreads();   // red is the tile's LDS
+}
+
+// D on the kx = 0 and n1/2 planes -> its Hermitian part (the mate of entry e on
+// the same plane is csr_mate; each pair is handled from its lower entry).
+// Workgroups b of nb take the entries in a grid stride.
+__device__ void dherm_part(const HalfIterArgs& a, int b, int nb)
+{
+    const int64_t n0 = a.csr_off[a.n2];
+    const int64_t mb = a.csr_off[(int64_t)(a.n1 / 2) * a.n2], me = a.csr_off[(int64_t)(a.n1 / 2 + 1) * a.n2];
+    const int64_t cnt = n0 + (me - mb);
+    for (int64_t i = (int64_t)b * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)nb * blockDim.x) {
+        const int64_t e = i < n0 ? i : mb + (i - n0);
+        const int64_t m = a.csr_mate[i];
+        if (m < e) continue;   // the pair is handled from its lower entry (or has no mate)
+        const float2 de = a.csr_d[e];
+        if (m == e) {
+            a.csr_d[e] = make_float2(de.x, 0.0f);
+            continue;
+        }
+        const float2 dm = a.csr_d[m];
+        a.csr_d[e] = make_float2(0.5f * (de.x + dm.x), 0.5f * (de.y - dm.y));
+        a.csr_d[m] = make_float2(0.5f * (dm.x + de.x), 0.5f * (dm.y - de.y));
+    }
+}
+
 template <int N>
 __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, int cz, const float2* __restrict__ tw)
 {
@@ -479,6 +521,13 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     using PL = RegPlan<N>;
     constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
     extern __shared__ float2 lds[];
+    // the iteration's errK and D's Hermitian part: both need the whole z pass,
+    // which this launch follows, and nothing before the next z pass reads them
+    if (a.errk && blockIdx.x == 0) errk_reduce(a, reinterpret_cast<double*>(lds));
+    if (a.dherm) {
+        const int nb = min(16, (int)gridDim.x);
+        if ((int)blockIdx.x < nb) dherm_part(a, (int)blockIdx.x, nb);
+    }
     const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = N, n3 = a.n3, h = a.n1 / 2 + 1;
     int kx, i3_0;
@@ -868,28 +917,6 @@ __global__ __launch_bounds__(256) void k_rfree_reduce(const float* __restrict__ 
 // Re D(s) for a voxel that is its own mate.  Plane entries are the list's
 // first run (pencils kx = 0) and its last (kx = n1/2); mate[i] is the entry of
 // plane entry i's in-plane mate.
-__global__ void k_dherm(HalfBatch hb)
-{
-    const HalfIterArgs& a = hb.job[blockIdx.y];
-    if (!a.dmode) return;
-    const int64_t n0 = a.csr_off[a.n2];
-    const int64_t mb = a.csr_off[(int64_t)(a.n1 / 2) * a.n2], me = a.csr_off[(int64_t)(a.n1 / 2 + 1) * a.n2];
-    const int64_t cnt = n0 + (me - mb);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t e = i < n0 ? i : mb + (i - n0);
-        const int64_t m = a.csr_mate[i];
-        if (m < e) continue;   // the pair is handled from its lower entry (or has no mate)
-        const float2 de = a.csr_d[e];
-        if (m == e) {
-            a.csr_d[e] = make_float2(de.x, 0.0f);
-            continue;
-        }
-        const float2 dm = a.csr_d[m];
-        a.csr_d[e] = make_float2(0.5f * (de.x + dm.x), 0.5f * (de.y - dm.y));
-        a.csr_d[m] = make_float2(0.5f * (dm.x + de.x), 0.5f * (dm.y - de.y));
-    }
-}
-
 __global__ void k_hcsr_mates(const int64_t* __restrict__ off, const uint32_t* __restrict__ meta, int n1, int n2,
                              int n3, int64_t n0, int64_t mb, int64_t cnt, int64_t* __restrict__ mate)
 {
@@ -1121,16 +1148,6 @@ void half_last_first(const HalfBatch& b, hipStream_t st)
 #undef C
 }
 
-void half_dherm(const HalfBatch& b, hipStream_t st)
-{
-    check_batch(b);
-    bool any = false;
-    for (int i = 0; i < b.count; ++i) any = any || b.job[i].dmode;
-    if (!any) return;
-    k_dherm<<<dim3(64, b.count), 256, 0, st>>>(b);
-    GF_LAUNCH_CHECK();
-}
-
 void hcsr_mates(const int64_t* d_off, const uint32_t* d_meta, int n1, int n2, int n3, int64_t n_plane0,
                 int64_t plane_m_begin, int64_t n_plane, int64_t* d_mate, hipStream_t st)
 {
@@ -1141,7 +1158,6 @@ void hcsr_mates(const int64_t* d_off, const uint32_t* d_meta, int n1, int n2, in
 }
 
 void half_first(const HalfIterArgs& a, hipStream_t st) { half_first(one(a), st); }
-void half_dherm(const HalfIterArgs& a, hipStream_t st) { half_dherm(one(a), st); }
 void half_y(const HalfIterArgs& a, int inverse, hipStream_t st) { half_y(one(a), inverse, st); }
 void half_z(const HalfIterArgs& a, hipStream_t st) { half_z(one(a), st); }
 void half_last(const HalfIterArgs& a, hipStream_t st) { half_last(one(a), st); }

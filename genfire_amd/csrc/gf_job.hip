@@ -767,6 +767,20 @@ void rfree_step(genfire_job* j, hipStream_t st)
                  j->rf_den.as<double>(), j->rf_bins, j->Rfree.as<float>(), j->errctr.as<int>(), st);
 }
 
+// The args of an iteration's y-inverse launch: it also reduces errK and forms
+// D's Hermitian part (DR on the box).
+HalfIterArgs y_inv_args(const genfire_job* j)
+{
+    const ReconState& r = j->rs;
+    HalfIterArgs a = r.ha;
+    a.errk = const_cast<float*>(j->errK.as<float>());
+    a.errctr = const_cast<int*>(j->errctr.as<int>());
+    a.err_den = r.den;
+    a.err_n = r.nparts;
+    a.dherm = a.dmode;
+    return a;
+}
+
 void recon_body(genfire_job* j, int it)
 {
     ReconState& r = j->rs;
@@ -780,9 +794,8 @@ void recon_body(genfire_job* j, int it)
         kstat_begin(j, GENFIRE_KSTAT_FFT_FUSED, &e_fused);
         half_z(r.ha, st);
         kstat_end(j, GENFIRE_KSTAT_FFT_FUSED, e_fused);
-        half_dherm(r.ha, st);
         rfree_step(j, st);
-        half_y(r.ha, 1, st);
+        half_y(y_inv_args(j), 1, st);
         if (it < r.N) half_last_first(p5_args(j, it), st);
         else half_last(p5_args(j, it), st);
     } else {
@@ -806,19 +819,19 @@ void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStr
 {
     for (int c0 = 0; c0 < count; c0 += kHalfBatchMax) {
         const int c1 = std::min(count, c0 + kHalfBatchMax);
-        HalfBatch all{}, cont{}, fin{};
+        HalfBatch all{}, inv{}, cont{}, fin{};
         for (int i = c0; i < c1; ++i) {
             const genfire_job* j = jobs[i];
             GF_CHECK(j->rs.half, GENFIRE_ERR_ARG, "half_body_batch: job is not on the half-spectrum path");
             all.job[all.count++] = j->rs.ha;
+            inv.job[inv.count++] = y_inv_args(j);
             if (its[i] < j->rs.N) cont.job[cont.count++] = p5_args(j, its[i]);
             else fin.job[fin.count++] = p5_args(j, its[i]);
         }
         half_y(all, 0, st);
         half_z(all, st);
-        half_dherm(all, st);
         for (int i = c0; i < c1; ++i) rfree_step(jobs[i], st);
-        half_y(all, 1, st);
+        half_y(inv, 1, st);
         if (cont.count) half_last_first(cont, st);
         if (fin.count) half_last(fin, st);
     }
@@ -827,8 +840,9 @@ void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStr
 void recon_post(genfire_job* j, int it)
 {
     ReconState& r = j->rs;
-    iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(), j->errctr.as<int>(),
-                          j->run_st);
+    if (!r.half)   // the half-spectrum y-inverse pass reduced errK already
+        iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(), j->errctr.as<int>(),
+                              j->run_st);
     if (r.monitor && (it % r.L == 0) && it / r.L - 1 < r.nR) {
         const float* model = (r.dr || r.half) ? j->pfbuf.as<float>() : j->x.as<float>();
         run_rfactor(j, model, it / r.L - 1);
@@ -904,8 +918,9 @@ void do_reconstruct(genfire_job* j)
             if (!r.gexec) {
                 GF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                 recon_body(j, it);
-                iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(),
-                                      j->errctr.as<int>(), st);
+                if (!r.half)
+                    iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(),
+                                          j->errctr.as<int>(), st);
                 GF_HIP(hipStreamEndCapture(st, &r.graph));
                 GF_HIP(hipGraphInstantiate(&r.gexec, r.graph, nullptr, nullptr, 0));
             }
@@ -914,8 +929,9 @@ void do_reconstruct(genfire_job* j)
         } else {
             recon_pre(j, it);
             recon_body(j, it);
-            iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(),
-                                  j->errctr.as<int>(), st);
+            if (!r.half)
+                iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(),
+                                      j->errctr.as<int>(), st);
         }
         kstat_end(j, GENFIRE_KSTAT_ITERATION, e_it);
         if (j->rs.monitor && (it % j->rs.L == 0) && it / j->rs.L - 1 < j->rs.nR) {
