@@ -113,17 +113,15 @@ __device__ __forceinline__ bool make_column(const PlaneConst& pc, const HalfGrid
 
 __device__ __forceinline__ void coords(const Column& col, int ka, float K[3])
 {
-    int k[3];
-    k[col.a] = ka;
-    k[col.b] = col.kb;
-    k[col.c] = col.kc;
-    K[0] = (float)k[0]; K[1] = (float)k[1]; K[2] = (float)k[2];
+    // selects, not a store at a runtime index (which puts the array in scratch)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) K[q] = (float)(col.a == q ? ka : (col.b == q ? col.kb : col.kc));
 }
 
 __global__ __launch_bounds__(kThreads) void k_index_count(IndexArgs args, uint32_t* blk_counts)
 {
     const int p = blockIdx.y;
-    const PlaneConst pc = args.planes[p];
+    const PlaneConst& pc = args.planes[p];   // read in place: a private copy indexed by axis sits in scratch
     const int64_t col = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     uint32_t cnt = 0;
     Column cl;
@@ -180,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void k_index_emit(IndexArgs args, const i
                                                          float* un, float* vn)
 {
     const int p = blockIdx.y;
-    const PlaneConst pc = args.planes[p];
+    const PlaneConst& pc = args.planes[p];
     const int64_t col = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     Column cl;
     const bool have = make_column(pc, args.hg, args.cutoff, col, &cl);

@@ -375,7 +375,8 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
             const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
             float2* slot = lds + pl * N + swz(z, pl);
-            const float2 dv = (e == e0) ? d0 : a.csr_d[e];
+            float2 dv = d0;   // by value: a select of two float2 lvalues selects addresses
+            if (e != e0) dv = a.csr_d[e];
             const float2 kv = *slot;
             *slot = make_float2(kv.x - dv.x, kv.y - dv.y);
         }
@@ -412,7 +413,8 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             float2* slot = lds + pl * N + swz(z, pl);
             float2 kv = *slot;
             if (a.dmode && !a.rf_off) {
-                const float2 dv = (e == e0) ? d0 : a.csr_d[e];
+                float2 dv = d0;
+                if (e != e0) dv = a.csr_d[e];
                 kv = make_float2(kv.x - dv.x, kv.y - dv.y);
             }
             const float ak = sqrtf(kv.x * kv.x + kv.y * kv.y);
@@ -447,7 +449,9 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
         for (int e = e0; e < end; e += NT) {
             const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
             const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
-            lds[pl * N + swz(z, pl)] = (e == e0) ? dn0 : a.csr_d[e];
+            float2 dv = dn0;
+            if (e != e0) dv = a.csr_d[e];
+            lds[pl * N + swz(z, pl)] = dv;
         }
         __syncthreads();
     }
