@@ -248,6 +248,8 @@ bool project_y_exact(const float* R);
 void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipStream_t st);
 int64_t zpair_floats(int d1, int d2, int d3);   // floats of one z-pair copy
 void volume_zpairs(const float* d_vol_y, float* d_dst, int d1, int d2, int d3, hipStream_t st);
+// the z-pair copies of the centred d1 x d2 x d3 crop of an FFT-order n1 x n2 x n3 volume
+void crop_zpairs(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st);
 void project(const ProjArgs& a, hipStream_t st);
 void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st);
 

@@ -121,7 +121,9 @@ struct genfire_job {
     gf::DevBuf rf_off, rf_meta, rf_val, rf_terms, rf_order, rf_shell_off, rf_den, Rfree;
     int rf_bins = 0;
     std::vector<float> Rfree_h;
-    bool rmat_any_y = false;
+    bool rmat_any_y = false;      // some projection reads a y-fastest form of the model
+    bool rmat_any_x = false;      // some projection reads the x-fastest model
+    bool rmat_any_exact = false;  // some projection maps y exactly onto y (the y-fastest copy itself)
     std::vector<float> errK_h, errR_h;
     std::vector<double> Rarr_h, Rarr2_h;
     bool reconstructed = false;

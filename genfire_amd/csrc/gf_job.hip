@@ -378,15 +378,16 @@ void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
     const int P = j->P;
     EventTimer& tm = *j->rs.rf[slot];
     GF_HIP(hipEventRecord(tm.a, st));
-    crop_centre(d_model_fft, 0, j->n1, j->n2, j->n1, j->vol.as<float>(), j->dim1, j->dim2, j->dim1, st);
-    if (j->rmat_any_y) {
-        volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
-        volume_zpairs(j->vol_y.as<float>(), j->vol_z.as<float>(), j->dim1, j->dim2, j->dim1, st);
-    }
+    // the model's forms the projections read: x-fastest, y-fastest (exact-y
+    // tilts) and the z-pair copies (every other y-fastest projection)
+    if (j->rmat_any_x || j->rmat_any_exact)
+        crop_centre(d_model_fft, 0, j->n1, j->n2, j->n1, j->vol.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    if (j->rmat_any_exact) volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    if (j->rmat_any_y) crop_zpairs(d_model_fft, j->n1, j->n2, j->n1, j->vol_z.as<float>(), j->dim1, j->dim2, j->dim1, st);
     const int nblk = project_blocks(j->dim1, j->dim2);
     ProjArgs pa{};
-    pa.vol = j->vol.as<float>();
-    pa.vol_y = j->rmat_any_y ? j->vol_y.as<float>() : nullptr;
+    pa.vol = (j->rmat_any_x || j->rmat_any_exact) ? j->vol.as<float>() : nullptr;
+    pa.vol_y = j->rmat_any_exact ? j->vol_y.as<float>() : nullptr;
     pa.vol_z = j->rmat_any_y ? j->vol_z.as<float>() : nullptr;
     pa.zodd = zpair_floats(j->dim1, j->dim2, j->dim1);
     pa.d1 = j->dim1; pa.d2 = j->dim2; pa.d3 = j->dim1;
@@ -656,13 +657,16 @@ void recon_begin(genfire_job* j)
     }
     if (r.nR > 0) {
         std::vector<float> Rm = proj_matrices(j->angles.data(), j->P, &j->rmat_any_y);
+        j->rmat_any_x = j->rmat_any_exact = false;
+        for (int p = 0; p < j->P; ++p) {
+            j->rmat_any_x = j->rmat_any_x || Rm[(size_t)kRStride * p + 9] == 0.0f;
+            j->rmat_any_exact = j->rmat_any_exact || Rm[(size_t)kRStride * p + 10] != 0.0f;
+        }
         j->Rmat.alloc(sizeof(float) * Rm.size());
         GF_HIP(hipMemcpyAsync(j->Rmat.p, Rm.data(), sizeof(float) * Rm.size(), hipMemcpyHostToDevice, st));
-        j->vol.alloc(sizeof(float) * box_size(j));
-        if (j->rmat_any_y) {
-            j->vol_y.alloc(sizeof(float) * box_size(j));
-            j->vol_z.alloc(sizeof(float) * 2 * zpair_floats(j->dim1, j->dim2, j->dim1));
-        }
+        if (j->rmat_any_x || j->rmat_any_exact) j->vol.alloc(sizeof(float) * box_size(j));
+        if (j->rmat_any_exact) j->vol_y.alloc(sizeof(float) * box_size(j));
+        if (j->rmat_any_y) j->vol_z.alloc(sizeof(float) * 2 * zpair_floats(j->dim1, j->dim2, j->dim1));
         j->rsums.alloc(sizeof(double) * (size_t)j->P * 4 * project_blocks(j->dim1, j->dim2));
         j->rout.alloc(sizeof(double) * (size_t)r.nR * j->P * 4);
         GF_HIP(hipStreamSynchronize(st));

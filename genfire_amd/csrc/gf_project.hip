@@ -41,7 +41,7 @@ constexpr int kRows = 4;     // waves per workgroup along the other pixel axis
 
 // A wave's 64 lanes are 64 neighbouring pixels along the pixel axis whose
 // direction in the volume is closest to the volume's fastest axis: pixel i with
-// the x-fastest copy when |R00| >= |R11|, else pixel j with a y-fastest copy
+// the x-fastest copy when |R00| > |R11|, else pixel j with a y-fastest copy
 // (R[9] of the projection selects it).  For tilts about y the pixel-y direction
 // is the volume's y axis, so each gather of a wave reads a few contiguous runs
 // instead of one cache line per lane.
@@ -334,13 +334,54 @@ __global__ __launch_bounds__(256) void k_zpair(const float* __restrict__ src, fl
     }
 }
 
-__global__ void k_project_reduce(const double* sums, int P, int nblk, double* out)
+// One wave per (projection, sum): lane sums over a stride of 64, then the wave,
+// in a fixed order.
+__global__ __launch_bounds__(256) void k_project_reduce(const double* sums, int P, int nblk, double* out)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P * 4) return;
+    const int w = (int)(blockIdx.x * blockDim.x + threadIdx.x) / 64, l = (int)threadIdx.x % 64;
+    if (w >= P * 4) return;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += sums[(int64_t)t * nblk + b];
-    out[t] = s;
+    for (int b = l; b < nblk; b += 64) s += sums[(int64_t)w * nblk + b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (l == 0) out[w] = s;
+}
+
+__device__ __forceinline__ int wrap_n(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// The z-pair copies (as k_zpair) straight from the FFT-order iterate: model
+// voxel (x, y, z) is src[wrap(x - d1/2) + n1 (wrap(y - d2/2) + n2 wrap(z - d3/2))]
+// (the centred crop of fftshift).  A workgroup takes a 32 x 32 (x, y) tile of
+// pair m: it stages slices 2m, 2m+1, 2m+2 through LDS (coalesced along x) and
+// writes pair m of both copies (coalesced along y).
+__global__ __launch_bounds__(256) void k_crop_zpairs(const float* __restrict__ src, int n1, int n2, int n3,
+                                                     float* __restrict__ dst, int d1, int d2, int d3,
+                                                     int64_t zodd)
+{
+    __shared__ float t[3][32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32, m = blockIdx.z;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int z = 2 * m + q;
+        for (int r = ty; r < 32; r += 8) {
+            const int x = bx + tx, y = by + r;
+            float v = 0.0f;
+            if (x < d1 && y < d2 && z < d3)
+                v = src[(int64_t)wrap_n(x - d1 / 2, n1) +
+                        (int64_t)n1 * (wrap_n(y - d2 / 2, n2) + (int64_t)n2 * wrap_n(z - d3 / 2, n3))];
+            t[q][r][tx] = v;
+        }
+    }
+    __syncthreads();
+    const int l = threadIdx.x % 64, yy = l >> 1, sp = l & 1;
+    const int y = by + yy;
+    for (int xr = threadIdx.x / 64; xr < 32; xr += 4) {
+        const int x = bx + xr;
+        if (x >= d1 || y >= d2) continue;
+        const int64_t o = sp + 2 * (y + (int64_t)d2 * (x + (int64_t)d1 * m));
+        dst[o] = t[sp][yy][xr];             // even copy: z = 2m + sp
+        dst[zodd + o] = t[1 + sp][yy][xr];  // odd copy: z = 2m + 1 + sp
+    }
 }
 
 }  // namespace
@@ -353,7 +394,9 @@ int project_blocks(int m1, int m2)
     return a > b ? a : b;
 }
 
-bool project_wants_yfast(const float* R) { return fabsf(R[4]) > fabsf(R[0]); }
+// ties (e.g. theta = 0) take the y-fastest forms, so that a tilt series about y
+// needs no x-fastest copy of the model
+bool project_wants_yfast(const float* R) { return fabsf(R[4]) >= fabsf(R[0]); }
 
 bool project_y_exact(const float* R)
 {
@@ -379,6 +422,13 @@ void volume_zpairs(const float* d_vol_y, float* d_dst, int d1, int d2, int d3, h
     }
 }
 
+void crop_zpairs(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st)
+{
+    dim3 grid((d1 + 31) / 32, (d2 + 31) / 32, d3 / 2 + 1);
+    k_crop_zpairs<<<grid, 256, 0, st>>>(d_src, n1, n2, n3, d_dst, d1, d2, d3, zpair_floats(d1, d2, d3));
+    GF_LAUNCH_CHECK();
+}
+
 void project(const ProjArgs& a, hipStream_t st)
 {
     GF_CHECK(a.d1 >= 2 && a.d2 >= 2 && a.d3 >= 2, GENFIRE_ERR_ARG, "projector needs every volume dimension >= 2");
@@ -393,7 +443,7 @@ void project(const ProjArgs& a, hipStream_t st)
 void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st)
 {
     const int n = P * 4;
-    k_project_reduce<<<(n + 255) / 256, 256, 0, st>>>(d_sums, P, nblk, d_out4);
+    k_project_reduce<<<(n + 3) / 4, 256, 0, st>>>(d_sums, P, nblk, d_out4);
     GF_LAUNCH_CHECK();
 }
 
