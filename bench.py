@@ -397,9 +397,9 @@ def main():
                                f"per launch, {pass_ms:.3f} ms mean over {nfp} launches"},
     }
     if args.served_jobs > 0:
-        out["served"] = serve_tilt(ctx, p, w, projs, ang, sup, args.served_jobs, _lib.PREC_SPLIT_BF16,
+        out["served"] = serve_tilt(ctx, p, w, projs, ang, sup, args.served_jobs, _lib.PREC_FP32,
                                    args.max_active)
-        out["served"]["precision"] = "split_bf16"
+        out["served"]["precision"] = "fp32"
         out["served"]["vs_this_line"] = out["served"]["recon_per_hour"] / out["value"]
         p.precision = prec
     if prec != _lib.PREC_FP32:

@@ -29,6 +29,18 @@ void grid_values_fp32(const float* d_proj, int dim1, int dim2, const float* d_un
 // MFMA form: the projections pre-transposed to row-major [a][b] (zero-padded),
 // then 64-point tiles (mfma_tile_points) through v_mfma_f32_16x16x4f32.
 int mfma_tile_points();
+// The same values as a type-2 non-uniform FFT (gf_grid_nufft.hip): fine grid
+// length M >= 2 Dim (0 when no supported FFT length fits), kernel shape beta,
+// the per-mode deconvolution 1/phi^, the padded-grid load and the w x w
+// interpolation over the same 64-point tiles.
+int nufft_kernel_width();
+int nufft_fine_len(int dim);
+double nufft_beta(int dim, int M);
+std::vector<float> nufft_deconv(int dim, int M);
+void nufft_load(const float* d_proj, int d1, int d2, int P, int M1, int M2, const float* d_inv1,
+                const float* d_inv2, float2* d_fine, hipStream_t st);
+void nufft_interp(const float2* d_G, int M1, int M2, double beta1, double beta2, const float* d_un,
+                  const float* d_vn, const PointTile* d_tiles, int ntiles, float2* d_F, hipStream_t st);
 size_t grid_prep_floats(int dim1, int dim2, int P);
 void grid_prep_proj(const float* d_proj, int dim1, int dim2, int P, float* d_Pt, hipStream_t st);
 void grid_values_mfma_fp32(const float* d_Pt, int dim1, int dim2, const float* d_un, const float* d_vn,

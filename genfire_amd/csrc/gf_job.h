@@ -105,7 +105,7 @@ struct genfire_job {
     bool have_inputs = false, have_support = false, have_init = false;
 
     // gridding
-    gf::DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles, Pt, Phi, Plo, Pscale;
+    gf::DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles, Pt, Phi, Plo, Pscale, nu_inv;
     gf::DevBuf acc, sw, hits, exact;
     gf::DevBuf M, mask, recifft;
     std::vector<int64_t> proj_off;

@@ -220,12 +220,17 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
     grid_index_emit(j->planes.as<PlaneConst>(), P, j->hg, cutoff, n1, n2, nblk, j->blk_offsets.as<int64_t>(),
                     j->ind.as<uint32_t>(), j->dist.as<float>(), j->un.as<float>(), j->vn.as<float>(), st);
 
-    // DFT values (:236-248).  Default: the MFMA GEMM form on 64-point tiles in
-    // the job's precision; with FP32, GENFIRE_GRID_KERNEL=valu selects the VALU
-    // kernel (256-point tiles).
+    // DFT values (:236-248).  FP32: the type-2 non-uniform FFT where the padded
+    // grid has a supported FFT length (both edges up to 512), else the direct
+    // sum as an MFMA GEMM; GENFIRE_GRID_KERNEL=mfma or =valu selects the direct
+    // sum on MFMA or on the VALU (256-point tiles).  The reduced precisions
+    // always run the direct sum on MFMA.  64-point tiles otherwise.
     const int prec = j->prm.precision;
     const char* gk = std::getenv("GENFIRE_GRID_KERNEL");
-    const bool use_valu = prec == GENFIRE_PREC_FP32 && gk && std::string(gk) == "valu";
+    const std::string gks = gk ? gk : "";
+    const bool use_valu = prec == GENFIRE_PREC_FP32 && gks == "valu";
+    const int nu1 = nufft_fine_len(j->dim1), nu2 = nufft_fine_len(j->dim2);
+    const bool use_nufft = prec == GENFIRE_PREC_FP32 && !use_valu && gks != "mfma" && nu1 > 0 && nu2 > 0;
     const int tile_pts = use_valu ? 256 : mfma_tile_points();
     std::vector<PointTile> tl;
     for (int p = 0; p < P; ++p)
@@ -241,13 +246,27 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
             j->Pscale.alloc(sizeof(float) * P);
             grid_prep_lp(prec, j->proj.as<float>(), j->dim1, j->dim2, P, j->Phi.as<uint16_t>(),
                          j->Plo.as<uint16_t>(), j->Pscale.as<float>(), st);
+        } else if (use_nufft) {
+            std::vector<float> inv = nufft_deconv(j->dim1, nu1), inv2 = nufft_deconv(j->dim2, nu2);
+            inv.insert(inv.end(), inv2.begin(), inv2.end());
+            j->nu_inv.alloc(sizeof(float) * inv.size());
+            GF_HIP(hipMemcpyAsync(j->nu_inv.p, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice, st));
+            j->Pt.alloc(sizeof(float2) * (size_t)nu1 * nu2 * P);   // the padded spectra
         } else if (!use_valu) {
             j->Pt.alloc(sizeof(float) * grid_prep_floats(j->dim1, j->dim2, P));
             grid_prep_proj(j->proj.as<float>(), j->dim1, j->dim2, P, j->Pt.as<float>(), st);
         }
         std::pair<hipEvent_t, hipEvent_t> ev;
         kstat_begin(j, GENFIRE_KSTAT_GRID_VALUES, &ev);
-        if (prec != GENFIRE_PREC_FP32)
+        if (use_nufft) {
+            float2* G = j->Pt.as<float2>();
+            nufft_load(j->proj.as<float>(), j->dim1, j->dim2, P, nu1, nu2, j->nu_inv.as<float>(),
+                       j->nu_inv.as<float>() + j->dim1, G, st);
+            fft_axis(G, nu1, nu2, P, 0, 0, st);
+            fft_axis(G, nu1, nu2, P, 1, 0, st);
+            nufft_interp(G, nu1, nu2, nufft_beta(j->dim1, nu1), nufft_beta(j->dim2, nu2), j->un.as<float>(),
+                         j->vn.as<float>(), j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
+        } else if (prec != GENFIRE_PREC_FP32)
             grid_values_mfma_lp(prec, j->Phi.as<uint16_t>(), j->Plo.as<uint16_t>(), j->Pscale.as<float>(), j->dim1,
                                 j->dim2, j->un.as<float>(), j->vn.as<float>(), j->tiles.as<PointTile>(),
                                 (int)tl.size(), j->F.as<float2>(), st);
@@ -358,6 +377,7 @@ void do_grid(genfire_job* j)
     j->sw.release();
     j->exact.release();
     j->Pt.release();
+    j->nu_inv.release();
     j->Phi.release();
     j->Plo.release();
     j->gridded = true;
@@ -1087,7 +1107,9 @@ int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P)
     const int64_t inputs = (int64_t)4 * dim1 * dim2 * P + box * 5;
     const int64_t grid = p.griddingMethod == 1
                              ? n1 * n1 * P * 8 + nf * (16 + 8 + 4 + 8 + 8) + box * 4   // spectra, d^3 sums, M, K
-                             : pts * 28 + kp * np * P * 4 + half_grid_vox * 36 + nf * (8 + 1 + 8) + box * 4;
+                             : pts * 28 +
+                                   std::max<int64_t>(kp * np * 4, (int64_t)nufft_fine_len(dim1) * nufft_fine_len(dim2) * 8) * P +
+                                   half_grid_vox * 36 + nf * (8 + 1 + 8) + box * 4;
     const bool half = half_supported((int)n1, (int)n2, (int)n3);
     const int64_t hn = (n1 / 2 + 1) * n2 * n3;
     const int64_t spectrum = 8 * (half ? std::max<int64_t>(2 * hn, nf) : nf);

@@ -110,12 +110,59 @@ def _gpu_vals_sorted(pts, p, ref):
     return pts["val"][g][go]
 
 
+@pytest.fixture
+def grid_kernel(request, monkeypatch):
+    """The FP32 value kernel: the default (type-2 non-uniform FFT where the
+    padded grid has a supported FFT length) or the direct sum on MFMA / VALU."""
+    if request.param:
+        monkeypatch.setenv("GENFIRE_GRID_KERNEL", request.param)
+    else:
+        monkeypatch.delenv("GENFIRE_GRID_KERNEL", raising=False)
+    return request.param or "default"
+
+
+@pytest.mark.parametrize("grid_kernel", [None, "mfma", "valu"], indirect=True)
+@pytest.mark.parametrize("name", ["three_euler_32", "odd_15x15_ratio3", "mixed_16x15_ratio3", "noncentro_16_ratio3"])
+def test_values_by_kernel(ctx, O, name, grid_kernel):
+    print(f"[grid] values {name} kernel={grid_kernel}", flush=True)
+    job, projs, ang, (d1, d2, ratio, cutoff, centro, v3) = _run(ctx, name)
+    pts = job.points()
+    for p in range(len(ang)):
+        g = pts["proj"] == p
+        F64 = O.dft64(projs[:, :, p], pts["un"][g], pts["vn"][g])
+        err = np.linalg.norm(pts["val"][g].astype(np.complex128) - F64) / np.linalg.norm(F64)
+        assert err < VALUE_TOL, (p, err)
+
+
+@pytest.mark.parametrize("d1,d2", [(256, 256), (96, 48)])
+def test_nufft_agrees_with_direct_sum(ctx, monkeypatch, d1, d2):
+    # the two FP32 forms of the same values: every point, to 1e-6 relative
+    # (each is within about 5e-7 of the FP64 sum)
+    print(f"[grid] NUFFT vs MFMA {d1}x{d2}", flush=True)
+    ang = np.random.default_rng(24).uniform([-5, -70, -5], [5, 70, 5], (3, 3))
+    projs = np.random.default_rng(14).uniform(0.0, 1.0, (d1, d2, len(ang))).astype(np.float32)
+    vals = {}
+    for kern in ("nufft", "mfma"):
+        if kern == "mfma":
+            monkeypatch.setenv("GENFIRE_GRID_KERNEL", "mfma")
+        else:
+            monkeypatch.delenv("GENFIRE_GRID_KERNEL", raising=False)
+        job = _lib.Job(ctx, _params(ratio=2.0, cutoff=0.3, v3=(0, 0, 1)), d1, d2, len(ang))
+        job.set_inputs(projs, ang)
+        job.grid()
+        vals[kern] = job.points()["val"].astype(np.complex128)
+    a, b = vals["nufft"], vals["mfma"]
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+
+
+@pytest.mark.parametrize("grid_kernel", [None, "mfma"], indirect=True)
 @pytest.mark.parametrize("d1,d2", [(256, 256), (320, 192)])
-def test_fp32_values_large_projections(ctx, O, d1, d2):
-    # Projections past 128 rows and columns use more than one entry of the
-    # coarsest phase tables (rows a = 128i + 16j + k); 320 x 192 also ends in a
-    # partial 128-column block.  3000 points per projection against the FP64 DFT.
-    print(f"[grid] FP32 values {d1}x{d2}", flush=True)
+def test_fp32_values_large_projections(ctx, O, d1, d2, grid_kernel):
+    # Projections past 128 rows and columns: the NUFFT's 512 and 640 x 384
+    # padded grids, and for the direct sum more than one entry of the coarsest
+    # phase tables (rows a = 128i + 16j + k), 320 x 192 ending in a partial
+    # 128-column block.  3000 points per projection against the FP64 DFT.
+    print(f"[grid] FP32 values {d1}x{d2} kernel={grid_kernel}", flush=True)
     ang = np.random.default_rng(23).uniform([-5, -70, -5], [5, 70, 5], (2, 3))
     projs = np.random.default_rng(13).uniform(0.0, 1.0, (d1, d2, len(ang))).astype(np.float32)
     job = _lib.Job(ctx, _params(ratio=2.0, cutoff=0.3, v3=(0, 0, 1)), d1, d2, len(ang))
