@@ -373,7 +373,7 @@ void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, con
     GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: volume dimensions must be >= 2");
     hipStream_t st;
     GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    DevBuf v, vy, r, o;
+    DevBuf v, vy, vz, r, o;
     const size_t nv = (size_t)d1 * d2 * d3;
     v.alloc(sizeof(float) * nv);
     o.alloc(sizeof(float) * (size_t)d1 * d2);
@@ -382,13 +382,17 @@ void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, con
     r.alloc(sizeof(float) * Rf.size());
     GF_HIP(hipMemcpyAsync(v.p, vol, sizeof(float) * nv, hipMemcpyHostToDevice, st));
     GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
-    if (any_y) {
+    if (any_y) {   // y-fastest copy and its z-pair copies (the projector's two-gather form)
         vy.alloc(sizeof(float) * nv);
         volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
+        vz.alloc(sizeof(float) * 2 * zpair_floats(d1, d2, d3));
+        volume_zpairs(vy.as<float>(), vz.as<float>(), d1, d2, d3, st);
     }
     ProjArgs pa{};
     pa.vol = v.as<float>();
     pa.vol_y = vy.as<float>();
+    pa.vol_z = vz.as<float>();
+    pa.zodd = zpair_floats(d1, d2, d3);
     pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
     pa.R = r.as<float>();
     pa.P = 1;
@@ -408,7 +412,7 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
     GF_CHECK(p1 <= d1 && p2 <= d2, GENFIRE_ERR_ARG, "GENFIRE: projection larger than the model");
     hipStream_t st;
     GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    DevBuf v, vy, r, o, m, s, s4;
+    DevBuf v, vy, vz, r, o, m, s, s4;
     const size_t nv = (size_t)d1 * d2 * d3, np = (size_t)p1 * p2 * P;
     v.alloc(sizeof(float) * nv);
     m.alloc(sizeof(float) * np);
@@ -419,9 +423,11 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
     GF_HIP(hipMemcpyAsync(v.p, model, sizeof(float) * nv, hipMemcpyHostToDevice, st));
     GF_HIP(hipMemcpyAsync(m.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, st));
     GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
-    if (any_y) {
+    if (any_y) {   // y-fastest copy and its z-pair copies (the projector's two-gather form)
         vy.alloc(sizeof(float) * nv);
         volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
+        vz.alloc(sizeof(float) * 2 * zpair_floats(d1, d2, d3));
+        volume_zpairs(vy.as<float>(), vz.as<float>(), d1, d2, d3, st);
     }
     const int nblk = project_blocks(p1, p2);
     s.alloc(sizeof(double) * (size_t)P * 4 * nblk);
@@ -429,6 +435,8 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
     ProjArgs pa{};
     pa.vol = v.as<float>();
     pa.vol_y = vy.as<float>();
+    pa.vol_z = vz.as<float>();
+    pa.zodd = zpair_floats(d1, d2, d3);
     pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
     pa.R = r.as<float>();
     pa.P = P;
