@@ -10,7 +10,7 @@ the final R-factor are reported beside it.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu-baseline]
 
-Prints ONE JSON line.  --gpus > 1 is refused: this build measures one GPU.
+Prints ONE JSON line.  --gpus > 1 is refused: the benchmark measures one GPU.
 """
 from __future__ import annotations
 
@@ -265,7 +265,7 @@ def main():
                     help="first run a 1 GiB device-to-device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "the library runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     from genfire_amd import _lib, phantom

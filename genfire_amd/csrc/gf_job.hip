@@ -1025,7 +1025,7 @@ void validate_job(const genfire_params* p, int dim1, int dim2, int P)
     GF_CHECK(n1 >= dim1 && n2 >= dim2, GENFIRE_ERR_ARG, "GENFIRE: oversampled size smaller than projection");
     GF_CHECK(fft_supported(n1) && fft_supported(n2), GENFIRE_ERR_UNSUPPORTED,
              "GENFIRE: oversampled grid " + std::to_string(n1) + "x" + std::to_string(n2) +
-                 " has an FFT length this build does not support");
+                 " has an FFT length the library does not support");
     (void)delay_indicators(p->constraintEnforcementMode);   // raises the reference's message for a bad mode
 }
 

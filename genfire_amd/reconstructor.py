@@ -114,7 +114,7 @@ class GENFIRE_Reconstructor:
         self.save_temp = 0
         self.save_loopLength = 50
         self.saveFilename = "./results/temp_rec"
-        # additions of this build
+        # additions beyond the reference
         self.algorithm = "genfire"      # 'genfire' | 'dr'
         self.precision = "fp32"         # gridding GEMM precision
         self._job = None

@@ -39,7 +39,7 @@ extern "C" {
 #define GENFIRE_ERR_HIP (-2)         /* HIP runtime error                         */
 #define GENFIRE_ERR_NOMEM (-3)       /* device allocation failed                  */
 #define GENFIRE_ERR_STATE (-4)       /* call out of order (e.g. reconstruct before grid) */
-#define GENFIRE_ERR_UNSUPPORTED (-5) /* size or option this build does not handle  */
+#define GENFIRE_ERR_UNSUPPORTED (-5) /* size or option the library does not handle */
 
 #define GENFIRE_ALG_GENFIRE 0        /* reconstruct.m:117-186                    */
 #define GENFIRE_ALG_DR 1             /* Douglas-Rachford on the same projections */

@@ -682,7 +682,7 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
                 loop_length: int = 10, projections=None, angles=None, rfree=None) -> ReconResult:
     """reconstruct.m:28-194 (GENFIRE, algorithm='genfire').
 
-    algorithm='dr' is this build's Douglas-Rachford variant composed of the same two
+    algorithm='dr' is the library's Douglas-Rachford variant composed of the same two
     projections (P_R = positivity/support, :126-131; P_F = replacement + real(ifftn),
     :163-165): y = 2 P_R(x) - x, x <- x + P_F(y) - P_R(x); errK is evaluated on
     fftn(y), the input of P_F, exactly as GENFIRE evaluates it on fftn(P_R(x)); the
