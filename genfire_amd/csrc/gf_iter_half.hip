@@ -322,7 +322,17 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
 // 8 consecutive ky (8 contiguous pencils; the kernel holds two transforms' worth
 // of registers, so smaller workgroups keep three of them resident per CU).
 // ---------------------------------------------------------------------------
-template <int N>
+// The box is exactly half of each z pencil (bz = N/2, the usual oversampling
+// ratio of 2) and the plan's first and last radices are at least 4: then the
+// rows that lie outside the box are whole radix slots, known at compile time,
+// and the compiler drops the zero inputs' arithmetic in the first forward stage
+// and the unused outputs' in the last inverse stage.
+template <int N> constexpr bool zhalf_ok() { return RegPlan<N>::R0 >= 4 && reg_last_radix<N>() >= 4 && RegPlan<N>::R0 % 4 == 0 && reg_last_radix<N>() % 4 == 0; }
+// slot r of a radix-R stage covers z in [r N/R, (r+1) N/R): inside the half box
+// [0, N/4) u [3N/4, N) when r < R/4 or r >= 3R/4
+template <int R> __device__ __forceinline__ constexpr bool half_slot(int r) { return r < R / 4 || r >= 3 * R / 4; }
+
+template <int N, bool HALF>
 __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch hb, const float2* __restrict__ tw)
 {
     const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
@@ -344,7 +354,7 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int z = j + T * i + r * (N / R0);
-            v[i * R0 + r] = inb(z, N, a.bz) ? pen[z] : make_float2(0.0f, 0.0f);
+            v[i * R0 + r] = (HALF ? half_slot<R0>(r) : inb(z, N, a.bz)) ? pen[z] : make_float2(0.0f, 0.0f);
         }
     // The tile's list range and each lane's first entry are loaded before the
     // forward transform, which hides their latency (a tile has fewer entries
@@ -469,7 +479,7 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
 #pragma unroll
         for (int q = 0; q < RL; ++q) {
             const int z = j2 + T * i + q * (N / RL);
-            if (inb(z, N, a.bz)) pen2[z] = cscale(v[i * RL + q], sc);
+            if (HALF ? half_slot<RL>(q) : inb(z, N, a.bz)) pen2[z] = cscale(v[i * RL + q], sc);
         }
 }
 
@@ -1041,9 +1051,16 @@ template <int N> void launch_z(const HalfBatch& b, hipStream_t st)
     // tile + one double per wave for the errK partial + twiddle table
     const size_t lds = (size_t)kZP * N * sizeof(float2) + sizeof(double) * (zfused_threads<N>() / 64) +
                        (size_t)reg_tw_size<N>() * sizeof(float2);
-    set_lds(k_half_z_fused<N>, lds);
-    k_half_z_fused<N><<<dim3((a.n1 / 2 + 1) * (a.n2 / kZP), b.count), zfused_threads<N>(), lds, st>>>(
-        b, fft_twiddles(N));
+    const dim3 grid((a.n1 / 2 + 1) * (a.n2 / kZP), b.count);
+    bool half = zhalf_ok<N>();
+    for (int i = 0; i < b.count; ++i) half = half && b.job[i].bz == N / 2;
+    if (half) {
+        set_lds(k_half_z_fused<N, true>, lds);
+        k_half_z_fused<N, true><<<grid, zfused_threads<N>(), lds, st>>>(b, fft_twiddles(N));
+    } else {
+        set_lds(k_half_z_fused<N, false>, lds);
+        k_half_z_fused<N, false><<<grid, zfused_threads<N>(), lds, st>>>(b, fft_twiddles(N));
+    }
     GF_LAUNCH_CHECK();
 }
 
