@@ -373,7 +373,7 @@ void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, con
     GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: volume dimensions must be >= 2");
     hipStream_t st;
     GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    DevBuf v, vy, vz, r, o;
+    DevBuf v, vy, vz, vzx, r, o;
     const size_t nv = (size_t)d1 * d2 * d3;
     v.alloc(sizeof(float) * nv);
     o.alloc(sizeof(float) * (size_t)d1 * d2);
@@ -393,6 +393,14 @@ void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, con
     pa.vol_y = vy.as<float>();
     pa.vol_z = vz.as<float>();
     pa.zodd = zpair_floats(d1, d2, d3);
+    // z-pair copies of the x-fastest volume for the x-fastest projections
+    bool any_x = false;
+    for (size_t q = 9; q < Rf.size(); q += kRStride) any_x = any_x || Rf[q] == 0.0f;
+    if (any_x) {
+        vzx.alloc(sizeof(float) * 2 * zpair_floats(d1, d2, d3));
+        volume_zpairs(v.as<float>(), vzx.as<float>(), d1, d2, d3, st);
+    }
+    pa.vol_zx = vzx.as<float>();
     pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
     pa.R = r.as<float>();
     pa.P = 1;
@@ -412,7 +420,7 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
     GF_CHECK(p1 <= d1 && p2 <= d2, GENFIRE_ERR_ARG, "GENFIRE: projection larger than the model");
     hipStream_t st;
     GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    DevBuf v, vy, vz, r, o, m, s, s4;
+    DevBuf v, vy, vz, vzx, r, o, m, s, s4;
     const size_t nv = (size_t)d1 * d2 * d3, np = (size_t)p1 * p2 * P;
     v.alloc(sizeof(float) * nv);
     m.alloc(sizeof(float) * np);
@@ -437,6 +445,14 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
     pa.vol_y = vy.as<float>();
     pa.vol_z = vz.as<float>();
     pa.zodd = zpair_floats(d1, d2, d3);
+    // z-pair copies of the x-fastest volume for the x-fastest projections
+    bool any_x = false;
+    for (size_t q = 9; q < Rf.size(); q += kRStride) any_x = any_x || Rf[q] == 0.0f;
+    if (any_x) {
+        vzx.alloc(sizeof(float) * 2 * zpair_floats(d1, d2, d3));
+        volume_zpairs(v.as<float>(), vzx.as<float>(), d1, d2, d3, st);
+    }
+    pa.vol_zx = vzx.as<float>();
     pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
     pa.R = r.as<float>();
     pa.P = P;

@@ -238,12 +238,15 @@ void hcsr_den(const float2* d_mag, const uint32_t* d_meta, int64_t n, double* d_
 // ---- projector (gf_project.hip) ----
 // Per projection the rotation is stored as kRStride floats: the column-major
 // 3x3 ZYZ matrix; at [9], 1 when the y-fastest volume copy is to be used; at
-// [10], 1 when the matrix maps the y axis exactly onto itself (then [9] is 1).
+// [10], 1 when the matrix maps the y axis exactly onto itself (then [9] and
+// [11] are 1); at [11], 1 when a wave's lanes run along pixel j, 0 along i,
+// 2 when a wave covers an 8 x 8 pixel patch.
 constexpr int kRStride = 12;
 struct ProjArgs {
     const float* vol;  // d1 x d2 x d3 column-major (x fastest)
     const float* vol_y; // the same volume y-fastest (index y + d2*(x + d1*z)), nullable when unused
     const float* vol_z; // z-pair copies of vol_y (volume_zpairs), nullable: then vol_y alone serves
+    const float* vol_zx; // z-pair copies of vol (x fastest), nullable: then vol alone serves
     int64_t zodd;       // offset of the odd-pair copy in vol_z (floats)
     int d1, d2, d3;
     const float* R;    // P x kRStride, single
@@ -255,13 +258,16 @@ struct ProjArgs {
     int post_hoc;      // 1: sum ||b|-|m|| instead of |b-m| (Tian_calc_Rfactor_realspace)
 };
 int project_blocks(int m1, int m2);
-bool project_wants_yfast(const float* R);
+// sets R[9] (y-fastest forms) and R[11] (lanes along pixel j) of a projection
+void project_orientation(float* R);
 bool project_y_exact(const float* R);
 void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipStream_t st);
 int64_t zpair_floats(int d1, int d2, int d3);   // floats of one z-pair copy
 void volume_zpairs(const float* d_vol_y, float* d_dst, int d1, int d2, int d3, hipStream_t st);
 // the z-pair copies of the centred d1 x d2 x d3 crop of an FFT-order n1 x n2 x n3 volume
 void crop_zpairs(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st);
+// the same pairs over the x-fastest crop (volume_zpairs of an x-fastest volume gives them too)
+void crop_zpairs_x(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st);
 void project(const ProjArgs& a, hipStream_t st);
 void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st);
 

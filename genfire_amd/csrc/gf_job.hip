@@ -146,8 +146,8 @@ void kstat_end(genfire_job* j, int k, std::pair<hipEvent_t, hipEvent_t>& ev)
 }
 
 // ZYZ matrices of the projector (calculate3Dprojection_RealSpaceinterp.m:13-15),
-// kRStride floats per projection with the y-fastest flag at [9] and the
-// exact-y flag at [10].
+// kRStride floats per projection with the y-fastest flag at [9], the exact-y
+// flag at [10] and the lane-axis flag at [11].
 std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const double* vectors)
 {
     std::vector<float> Rm((size_t)kRStride * P, 0.0f);
@@ -160,8 +160,12 @@ std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const
             zyz_matrix(angles[p], angles[P + p], angles[2 * P + p], R);
         float* r = Rm.data() + (size_t)kRStride * p;
         for (int q = 0; q < 9; ++q) r[q] = (float)R[q];
-        if (project_y_exact(r)) { r[9] = 1.0f; r[10] = 1.0f; *any_y = true; }
-        else if (project_wants_yfast(r)) { r[9] = 1.0f; *any_y = true; }
+        if (project_y_exact(r)) {
+            r[9] = 1.0f; r[10] = 1.0f; r[11] = 1.0f;
+        } else {
+            project_orientation(r);
+        }
+        *any_y = *any_y || r[9] != 0.0f;
     }
     return Rm;
 }
@@ -400,15 +404,19 @@ void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
     GF_HIP(hipEventRecord(tm.a, st));
     // the model's forms the projections read: x-fastest, y-fastest (exact-y
     // tilts) and the z-pair copies (every other y-fastest projection)
-    if (j->rmat_any_x || j->rmat_any_exact)
+    if (j->rmat_any_exact) {
         crop_centre(d_model_fft, 0, j->n1, j->n2, j->n1, j->vol.as<float>(), j->dim1, j->dim2, j->dim1, st);
-    if (j->rmat_any_exact) volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
+        volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    }
     if (j->rmat_any_y) crop_zpairs(d_model_fft, j->n1, j->n2, j->n1, j->vol_z.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    if (j->rmat_any_x)
+        crop_zpairs_x(d_model_fft, j->n1, j->n2, j->n1, j->vol_zx.as<float>(), j->dim1, j->dim2, j->dim1, st);
     const int nblk = project_blocks(j->dim1, j->dim2);
     ProjArgs pa{};
-    pa.vol = (j->rmat_any_x || j->rmat_any_exact) ? j->vol.as<float>() : nullptr;
+    pa.vol = nullptr;   // every x-fastest projection reads vol_zx
     pa.vol_y = j->rmat_any_exact ? j->vol_y.as<float>() : nullptr;
     pa.vol_z = j->rmat_any_y ? j->vol_z.as<float>() : nullptr;
+    pa.vol_zx = j->rmat_any_x ? j->vol_zx.as<float>() : nullptr;
     pa.zodd = zpair_floats(j->dim1, j->dim2, j->dim1);
     pa.d1 = j->dim1; pa.d2 = j->dim2; pa.d3 = j->dim1;
     pa.R = j->Rmat.as<float>();
@@ -684,9 +692,12 @@ void recon_begin(genfire_job* j)
         }
         j->Rmat.alloc(sizeof(float) * Rm.size());
         GF_HIP(hipMemcpyAsync(j->Rmat.p, Rm.data(), sizeof(float) * Rm.size(), hipMemcpyHostToDevice, st));
-        if (j->rmat_any_x || j->rmat_any_exact) j->vol.alloc(sizeof(float) * box_size(j));
-        if (j->rmat_any_exact) j->vol_y.alloc(sizeof(float) * box_size(j));
+        if (j->rmat_any_exact) {
+            j->vol.alloc(sizeof(float) * box_size(j));
+            j->vol_y.alloc(sizeof(float) * box_size(j));
+        }
         if (j->rmat_any_y) j->vol_z.alloc(sizeof(float) * 2 * zpair_floats(j->dim1, j->dim2, j->dim1));
+        if (j->rmat_any_x) j->vol_zx.alloc(sizeof(float) * 2 * zpair_floats(j->dim1, j->dim2, j->dim1));
         j->rsums.alloc(sizeof(double) * (size_t)j->P * 4 * project_blocks(j->dim1, j->dim2));
         j->rout.alloc(sizeof(double) * (size_t)r.nR * j->P * 4);
         GF_HIP(hipStreamSynchronize(st));

@@ -38,11 +38,12 @@ typedef float quad_t __attribute__((ext_vector_type(4), aligned(8)));
 constexpr int kXcd = 8;
 constexpr int kLanes = 64;   // pixels of one wave along the fast pixel axis
 constexpr int kRows = 4;     // waves per workgroup along the other pixel axis
+constexpr float kRowAlign = 0.9f;   // lane rows when a pairing's |R| reaches this, else patches
 
-// A wave's 64 lanes are 64 neighbouring pixels along the pixel axis whose
-// direction in the volume is closest to the volume's fastest axis: pixel i with
-// the x-fastest copy when |R00| > |R11|, else pixel j with a y-fastest copy
-// (R[9] of the projection selects it).  For tilts about y the pixel-y direction
+// A wave's 64 lanes are 64 neighbouring pixels along the pixel axis (i or j,
+// R[11]) whose direction in the volume is closest to the fast axis of one of the
+// model's forms (x- or y-fastest, R[9]): the largest of |R00|, |R11|, |R10|,
+// |R01|.  For tilts about y the pixel-y direction
 // is the volume's y axis, so each gather of a wave reads a few contiguous runs
 // instead of one cache line per lane.
 //
@@ -156,9 +157,13 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
         tile_sums(a, p, tile, tiles, q0, q1, q2, q3);
         return;
     }
-    const bool yfast = R[9] != 0.0f;
+    const bool yfast = R[9] != 0.0f;     // the model's y-fastest forms
     int i, j;
-    if (yfast) {
+    if (R[11] == 2.0f) {                  // 8 x 8 pixel patch per wave, 16 x 16 per workgroup
+        const int tp1 = (a.m1 + 15) / 16, w = (int)threadIdx.x / kLanes, l = (int)threadIdx.x % kLanes;
+        i = (tile % tp1) * 16 + (w & 1) * 8 + (l & 7);
+        j = (tile / tp1) * 16 + (w >> 1) * 8 + (l >> 3);
+    } else if (R[11] != 0.0f) {           // lanes along pixel j
         const int tiles2 = (a.m2 + kLanes - 1) / kLanes;
         i = (tile / tiles2) * kRows + (int)(threadIdx.x / kLanes);
         j = (tile % tiles2) * kLanes + (int)(threadIdx.x % kLanes);
@@ -236,6 +241,27 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
                 // same interpolation order as the x-fastest branch: along x, y, then z
                 const float x00 = q00.x + ta * (q10.x - q00.x), x10 = q00.y + ta * (q10.y - q00.y);
                 const float x01 = q01.x + ta * (q11.x - q01.x), x11 = q01.y + ta * (q11.y - q01.y);
+                const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
+                acc += y0 + te * (y1 - y0);
+            }
+        } else if (a.vol_zx) {
+            // z-pair copies of the x-fastest model: the cell's four voxels at
+            // y = b0 are one 16-byte load, (a0, e0) (a0, e0+1) (a0+1, e0)
+            // (a0+1, e0+1); the same at y = b0 + 1
+            const int sy = 2 * d1;
+            for (int k = k0; k <= k1; ++k) {
+                const float Z = (float)(k - d3 / 2);
+                const float fx = bx + rx * Z, fy = by + ry * Z, fz = bz + rz * Z;
+                int a0, b0, e0;
+                float ta, tb, te;
+                if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
+                    continue;
+                const int odd = e0 & 1;
+                const float* v = a.vol_zx + (odd ? a.zodd : 0) + 2 * (a0 + d1 * (b0 + d2 * (e0 >> 1)));
+                const quad_t q0 = *reinterpret_cast<const quad_t*>(v);        // y = b0
+                const quad_t q1 = *reinterpret_cast<const quad_t*>(v + sy);   // y = b0 + 1
+                const float x00 = q0.x + ta * (q0.z - q0.x), x10 = q1.x + ta * (q1.z - q1.x);
+                const float x01 = q0.y + ta * (q0.w - q0.y), x11 = q1.y + ta * (q1.w - q1.y);
                 const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
                 acc += y0 + te * (y1 - y0);
             }
@@ -334,6 +360,27 @@ __global__ __launch_bounds__(256) void k_zpair(const float* __restrict__ src, fl
     }
 }
 
+__device__ __forceinline__ int wrap_n(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// z-pair copies of the x-fastest crop: pair m of copy `odd` holds z = 2m + odd + s
+// at dst[s + 2 (x + d1 (y + d2 m))], straight from the FFT-order volume.
+__global__ __launch_bounds__(256) void k_crop_zpairs_x(const float* __restrict__ src, int n1, int n2, int n3,
+                                                       float* __restrict__ dst, int d1, int d2, int d3,
+                                                       int64_t per)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 2 * per;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = t % per, odd = t / per;
+        const int s = (int)(o & 1);
+        const int64_t xy = (o >> 1) % ((int64_t)d1 * d2), m = (o >> 1) / ((int64_t)d1 * d2);
+        const int x = (int)(xy % d1), y = (int)(xy / d1);
+        const int z = (int)(2 * m + odd + s);
+        dst[t] = z < d3 ? src[(int64_t)wrap_n(x - d1 / 2, n1) +
+                              (int64_t)n1 * (wrap_n(y - d2 / 2, n2) + (int64_t)n2 * wrap_n(z - d3 / 2, n3))]
+                        : 0.0f;
+    }
+}
+
 // One wave per (projection, sum): lane sums over a stride of 64, then the wave,
 // in a fixed order.
 __global__ __launch_bounds__(256) void k_project_reduce(const double* sums, int P, int nblk, double* out)
@@ -345,8 +392,6 @@ __global__ __launch_bounds__(256) void k_project_reduce(const double* sums, int 
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (l == 0) out[w] = s;
 }
-
-__device__ __forceinline__ int wrap_n(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 // The z-pair copies (as k_zpair) straight from the FFT-order iterate: model
 // voxel (x, y, z) is src[wrap(x - d1/2) + n1 (wrap(y - d2/2) + n2 wrap(z - d3/2))]
@@ -388,15 +433,37 @@ __global__ __launch_bounds__(256) void k_crop_zpairs(const float* __restrict__ s
 
 int project_blocks(int m1, int m2)
 {
-    // tiles are kLanes x kRows pixels, oriented either way: the larger count
+    // tiles are kLanes x kRows pixels, oriented either way, or 16 x 16 patches:
+    // the largest count
     const int a = ((m1 + kLanes - 1) / kLanes) * ((m2 + kRows - 1) / kRows);
     const int b = ((m2 + kLanes - 1) / kLanes) * ((m1 + kRows - 1) / kRows);
-    return a > b ? a : b;
+    const int c = ((m1 + 15) / 16) * ((m2 + 15) / 16);
+    return std::max(a, std::max(b, c));
 }
 
-// ties (e.g. theta = 0) take the y-fastest forms, so that a tilt series about y
-// needs no x-fastest copy of the model
-bool project_wants_yfast(const float* R) { return fabsf(R[4]) >= fabsf(R[0]); }
+void project_orientation(float* R)
+{
+    // lanes along pixel i (column 0 of R in the volume) or j (column 1), the
+    // model x- or y-fastest: the pairing whose lane direction is closest to the
+    // layout's fast axis, |R[3c + r]| largest; ties keep (j, y) then (i, x), so a
+    // tilt series about y (theta = 0 included) needs no x-fastest form.  When no
+    // pairing is within about 25 degrees (|R| < 0.9) a row of 64 lanes drifts
+    // across tens of cache lines per gather, and a wave takes an 8 x 8 pixel
+    // patch instead (R[11] = 2) on the layout its two pixel axes favour.
+    const float jy = fabsf(R[4]), ix = fabsf(R[0]), iy = fabsf(R[1]), jx = fabsf(R[3]);
+    float lanes = 1.0f;
+    bool layout_y = true;
+    float best = jy;
+    if (ix > best) { best = ix; lanes = 0.0f; layout_y = false; }
+    if (iy > best) { best = iy; lanes = 0.0f; layout_y = true; }
+    if (jx > best) { best = jx; lanes = 1.0f; layout_y = false; }
+    if (best < kRowAlign) {
+        lanes = 2.0f;
+        layout_y = iy + jy >= ix + jx;
+    }
+    R[9] = layout_y ? 1.0f : 0.0f;
+    R[11] = lanes;
+}
 
 bool project_y_exact(const float* R)
 {
@@ -420,6 +487,14 @@ void volume_zpairs(const float* d_vol_y, float* d_dst, int d1, int d2, int d3, h
         k_zpair<<<(unsigned)blocks, 256, 0, st>>>(d_vol_y, d_dst + odd * total, d1, d2, d3, odd, total);
         GF_LAUNCH_CHECK();
     }
+}
+
+void crop_zpairs_x(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st)
+{
+    const int64_t per = zpair_floats(d1, d2, d3);
+    const int64_t blocks = std::min<int64_t>((2 * per + 255) / 256, 1 << 20);
+    k_crop_zpairs_x<<<(unsigned)blocks, 256, 0, st>>>(d_src, n1, n2, n3, d_dst, d1, d2, d3, per);
+    GF_LAUNCH_CHECK();
 }
 
 void crop_zpairs(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st)

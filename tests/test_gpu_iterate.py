@@ -8,7 +8,7 @@ from genfire_amd import _lib
 pytestmark = pytest.mark.gpu
 
 
-def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None, euler=False):
+def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None, euler=False, phi_psi=0.0):
     """Blob phantom, trilinear tilt series and the oracle's measuredK.  dim2 != dim
     makes the projections (and the grid) non-square: vol is dim x dim2 x dim.
     euler adds phi and psi in [-5, 5] degrees to the tilts about y."""
@@ -24,8 +24,8 @@ def _setup(O, dim=32, ratio=2.0, P=15, seed=0, dim2=None, euler=False):
         vol += (rng.uniform(0.5, 1) * g[0][:, None, None] * g[1][None, :, None] * g[2][None, None, :]).astype(np.float32)
     ang = np.stack([np.zeros(P), np.linspace(-60, 60, P), np.zeros(P)], 1)
     if euler:
-        ang[:, 0] = rng.uniform(-5, 5, P)
-        ang[:, 2] = rng.uniform(-5, 5, P)
+        ang[:, 0] = phi_psi + rng.uniform(-5, 5, P)
+        ang[:, 2] = phi_psi + rng.uniform(-5, 5, P)
     projs = np.stack([O.project(vol, *a) for a in ang], 2).astype(np.float32)
     n1, n2 = int(round(dim * ratio)), int(round(dim2 * ratio))
     g = O.grid_dft(projs, ang, n1, n2, 0.3)
@@ -156,13 +156,14 @@ def test_box_pruning_matches_full_passes(ctx, O, alg, shape, monkeypatch):
         np.testing.assert_allclose(out["1"][2], out["0"][2], atol=1e-5, rtol=0)
 
 
-@pytest.mark.parametrize("dim,ratio", [(32, 2.0), (33, 64 / 33)])
-def test_monitor_three_euler_matches_oracle(ctx, O, dim, ratio):
+@pytest.mark.parametrize("dim,ratio,phi_psi", [(32, 2.0, 0.0), (33, 64 / 33, 0.0), (32, 2.0, 60.0), (33, 64 / 33, 60.0)])
+def test_monitor_three_euler_matches_oracle(ctx, O, dim, ratio, phi_psi):
     """The R-factor monitor on tilts with phi and psi off zero (one ray per lane
     on the z-pair copies of the model): an even volume edge and an odd one,
-    whose last even z pair ends in the zero padding."""
-    print(f"[iter] monitor three Euler dim={dim}", flush=True)
-    vol, projs, ang, mk, sup = _setup(O, dim=dim, ratio=ratio, P=13, seed=9, euler=True)
+    whose last even z pair ends in the zero padding; phi = psi = 60 degrees
+    puts most projections on the x-fastest copies."""
+    print(f"[iter] monitor three Euler dim={dim} phi_psi={phi_psi}", flush=True)
+    vol, projs, ang, mk, sup = _setup(O, dim=dim, ratio=ratio, P=13, seed=9, euler=True, phi_psi=phi_psi)
     iters = 20
     job = _job(ctx, mk, projs, ang, sup, iters, mode=1, alg="dr", monitor=1, L=5)
     ref = O.reconstruct(mk, mk != 0, sup, dim, dim, iters, mode=1, algorithm="dr", precision="fp64",

@@ -11,6 +11,11 @@ pytestmark = pytest.mark.gpu
     ((32, 32, 32), (0.0, 90.0, 0.0)),
     ((33, 30, 31), (12.0, -47.0, 33.0)),
     ((64, 64, 64), (3.0, 65.0, -4.0)),
+    ((33, 30, 31), (60.0, 60.0, 60.0)),    # no pairing within 0.9: 8 x 8 pixel patches
+    ((32, 36, 30), (70.0, 60.0, 50.0)),    # patches, x-fastest copies
+    ((31, 32, 33), (0.0, 30.0, 90.0)),     # |R10| = 1: lanes along i, y-fastest copies
+    ((30, 34, 32), (90.0, -30.0, 0.0)),    # |R01| = 1: lanes along j, x-fastest copies
+    ((30, 34, 32), (0.0, 0.0, 0.0)),       # identity: y exact, four-ray form off (d2 % 4 != 0)
 ])
 def test_project_matches_oracle(ctx, O, shape, angles):
     print(f"[proj] {shape} {angles}", flush=True)
