@@ -10,8 +10,11 @@ namespace gf {
 int index_threads();
 void grid_index_count(const PlaneConst* d_planes, int P, const HalfGrid& hg, double cutoff, int n1,
                       int n2, int nblk, uint32_t* d_blk_counts, hipStream_t st);
+// exclusive scan of n counts into offsets (int64), *total the sum; d_scratch
+// holds grid_index_scan_scratch(n) bytes
+size_t grid_index_scan_scratch(int64_t n);
 void grid_index_scan(const uint32_t* d_counts, int64_t n, int64_t* d_offsets, int64_t* d_total,
-                     hipStream_t st);
+                     int64_t* d_scratch, hipStream_t st);
 void grid_index_emit(const PlaneConst* d_planes, int P, const HalfGrid& hg, double cutoff, int n1,
                      int n2, int nblk, const int64_t* d_blk_offsets, uint32_t* ind, float* dist,
                      float* un, float* vn, hipStream_t st);
@@ -228,8 +231,13 @@ void half_y(const HalfIterArgs& a, int inverse, hipStream_t st);
 void half_z(const HalfIterArgs& a, hipStream_t st);
 void half_last(const HalfIterArgs& a, hipStream_t st);
 void half_last_first(const HalfIterArgs& a, hipStream_t st);   // P5 of one iteration + P1 of the next
+// The half-spectrum lists are built per (pencil, z chunk), hcsr_chunks() chunks:
+// counts at (pencil * chunks + chunk), scanned into chunk offsets d_coff, the
+// pencil offsets gathered from them (hcsr_pencil_off), the fill from d_coff.
+int hcsr_chunks();
 void hcsr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st);
-void hcsr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_off,
+void hcsr_pencil_off(const int64_t* d_coff, int64_t np, int64_t* d_off, hipStream_t st);
+void hcsr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_coff,
                uint32_t* d_meta, float2* d_val, float2* d_mag, hipStream_t st);
 void hcsr_enforce(int n1, int n2, int n3, const int64_t* d_off, uint32_t* d_meta, double delay, hipStream_t st);
 void hcsr_den(const float2* d_mag, const uint32_t* d_meta, int64_t n, double* d_tmp, int ntmp, double* d_out,

@@ -144,32 +144,75 @@ __global__ __launch_bounds__(kThreads) void k_index_count(IndexArgs args, uint32
     }
 }
 
-// Exclusive scan of n block counts in one workgroup (n is at most a few 1e5).
-__global__ __launch_bounds__(1024) void k_index_scan(const uint32_t* counts, int64_t n,
-                                                      int64_t* offsets, int64_t* total)
+// Exclusive scan of n counts (int64 offsets) in three launches: each 1024-thread
+// workgroup sums a chunk of kScanChunk counts, one workgroup scans the chunk
+// sums, each workgroup rescans its chunk from its carry.  Integer sums, so the
+// result does not depend on the split.
+constexpr int kScanPer = 8;                       // counts per thread per chunk
+constexpr int kScanChunk = 1024 * kScanPer;
+
+// inclusive scan of one value per thread over the 1024-thread workgroup
+__device__ __forceinline__ int64_t block_scan_incl(int64_t v, int64_t* wsum)
 {
-    __shared__ int64_t part[1024];
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t beg = threadIdx.x * per;
-    const int64_t end = min(n, beg + per);
-    int64_t s = 0;
-    for (int64_t i = beg; i < end; ++i) s += counts[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t run = 0;
-        for (int i = 0; i < 1024; ++i) {
-            const int64_t t = part[i];
-            part[i] = run;
-            run += t;
-        }
-        *total = run;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
     }
+    if (lane == 63) wsum[w] = v;
     __syncthreads();
-    int64_t run = part[threadIdx.x];
-    for (int64_t i = beg; i < end; ++i) {
-        offsets[i] = run;
-        run += counts[i];
+    int64_t pre = 0;
+    for (int q = 0; q < w; ++q) pre += wsum[q];
+    __syncthreads();
+    return v + pre;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(const uint32_t* __restrict__ counts, int64_t n,
+                                                    int64_t* __restrict__ csum)
+{
+    __shared__ int64_t wsum[16];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+    int64_t v = 0;
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) v += base + q < n ? counts[base + q] : 0u;
+    const int64_t inc = block_scan_incl(v, wsum);
+    if (threadIdx.x == 1023) csum[blockIdx.x] = inc;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_carry(int64_t* __restrict__ csum, int nchunk, int64_t* total)
+{
+    __shared__ int64_t wsum[16];
+    int64_t run = 0;
+    for (int b0 = 0; b0 < nchunk; b0 += 1024) {
+        const int b = b0 + (int)threadIdx.x;
+        const int64_t v = b < nchunk ? csum[b] : 0;
+        const int64_t inc = block_scan_incl(v, wsum);
+        if (b < nchunk) csum[b] = run + inc - v;   // exclusive carry of chunk b
+        int64_t all = 0;                             // this round's total: the wave sums
+        for (int q = 0; q < 16; ++q) all += wsum[q];
+        __syncthreads();
+        run += all;
+    }
+    if (threadIdx.x == 0) *total = run;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_write(const uint32_t* __restrict__ counts, int64_t n,
+                                                     const int64_t* __restrict__ carry, int64_t* __restrict__ offsets)
+{
+    __shared__ int64_t wsum[16];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+    uint32_t c[kScanPer];
+    int64_t v = 0;
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) {
+        c[q] = base + q < n ? counts[base + q] : 0u;
+        v += c[q];
+    }
+    int64_t run = carry[blockIdx.x] + block_scan_incl(v, wsum) - v;
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) {
+        if (base + q < n) offsets[base + q] = run;
+        run += c[q];
     }
 }
 
@@ -227,10 +270,21 @@ void grid_index_count(const PlaneConst* d_planes, int P, const HalfGrid& hg, dou
     GF_LAUNCH_CHECK();
 }
 
-void grid_index_scan(const uint32_t* d_counts, int64_t n, int64_t* d_offsets, int64_t* d_total,
-                     hipStream_t st)
+size_t grid_index_scan_scratch(int64_t n)
 {
-    k_index_scan<<<1, 1024, 0, st>>>(d_counts, n, d_offsets, d_total);
+    return sizeof(int64_t) * (size_t)std::max<int64_t>((n + kScanChunk - 1) / kScanChunk, 1);
+}
+
+void grid_index_scan(const uint32_t* d_counts, int64_t n, int64_t* d_offsets, int64_t* d_total,
+                     int64_t* d_scratch, hipStream_t st)
+{
+    const int nchunk = (int)std::max<int64_t>((n + kScanChunk - 1) / kScanChunk, 1);
+    int64_t* csum = d_scratch;   // the chunk sums, then their exclusive carries
+    k_scan_sums<<<nchunk, 1024, 0, st>>>(d_counts, n, csum);
+    GF_LAUNCH_CHECK();
+    k_scan_carry<<<1, 1024, 0, st>>>(csum, nchunk, d_total);
+    GF_LAUNCH_CHECK();
+    k_scan_write<<<nchunk, 1024, 0, st>>>(d_counts, n, csum, d_offsets);
     GF_LAUNCH_CHECK();
 }
 

@@ -796,36 +796,53 @@ __device__ __forceinline__ HalfVox half_vox(int kx, int ky, int z, int n1, int n
     return v;
 }
 
+// The lists are built per (pencil, z chunk): kHcsrChunks chunks of a pencil's z
+// range, counted, scanned in (pencil, chunk) order and filled, so a pencil's
+// entries stay in z order while four times as many lanes walk the grid.
+constexpr int kHcsrChunks = 4;
+
 __global__ void k_hcsr_count(const float2* __restrict__ M, const uint8_t* __restrict__ mask, int n1, int n2,
                              int n3, uint32_t* __restrict__ cnt)
 {
     const int h = n1 / 2 + 1;
     const int64_t np = (int64_t)h * n2;
     // lanes take consecutive kx (contiguous voxels of M and of the mates); the
-    // count goes to pencil ky + n2*kx
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-        const int kx = (int)(q % h), ky = (int)(q / h);
+    // count goes to slot (ky + n2*kx) * kHcsrChunks + chunk
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np * kHcsrChunks;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(q % h), ky = (int)((q / h) % n2), ch = (int)(q / np);
+        const int z0 = (int)((int64_t)n3 * ch / kHcsrChunks), z1 = (int)((int64_t)n3 * (ch + 1) / kHcsrChunks);
         uint32_t c = 0;
-        for (int z = 0; z < n3; ++z) {
+        for (int z = z0; z < z1; ++z) {
             const HalfVox hv = half_vox(kx, ky, z, n1, n2, n3);
             const bool A = hmeas(M, mask, hv.s);
             const bool Bm = hmeas(M, mask, hv.mate);   // on the kx = 0, n1/2 planes: the in-plane mate
             c += (A || Bm) ? 1u : 0u;
         }
-        cnt[(int64_t)ky + (int64_t)n2 * kx] = c;
+        cnt[((int64_t)ky + (int64_t)n2 * kx) * kHcsrChunks + ch] = c;
     }
 }
 
+// csr_off[p] = the first chunk offset of pencil p (csr_off[np], the total, is
+// written by the scan)
+__global__ void k_hcsr_pencil_off(const int64_t* __restrict__ coff, int64_t np, int64_t* __restrict__ off)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
+        off[p] = coff[p * kHcsrChunks];
+}
+
 __global__ void k_hcsr_fill(const float2* __restrict__ M, const uint8_t* __restrict__ mask, int n1, int n2,
-                            int n3, const int64_t* __restrict__ off, uint32_t* __restrict__ meta,
+                            int n3, const int64_t* __restrict__ coff, uint32_t* __restrict__ meta,
                             float2* __restrict__ val, float2* __restrict__ mag)
 {
     const int h = n1 / 2 + 1;
     const int64_t np = (int64_t)h * n2;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-        const int kx = (int)(q % h), ky = (int)(q / h);   // as in k_hcsr_count
-        int64_t o = off[(int64_t)ky + (int64_t)n2 * kx];
-        for (int z = 0; z < n3; ++z) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < np * kHcsrChunks;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(q % h), ky = (int)((q / h) % n2), ch = (int)(q / np);   // as in k_hcsr_count
+        const int z0 = (int)((int64_t)n3 * ch / kHcsrChunks), z1 = (int)((int64_t)n3 * (ch + 1) / kHcsrChunks);
+        int64_t o = coff[((int64_t)ky + (int64_t)n2 * kx) * kHcsrChunks + ch];
+        for (int z = z0; z < z1; ++z) {
             const HalfVox hv = half_vox(kx, ky, z, n1, n2, n3);
             const bool A = hmeas(M, mask, hv.s);
             const bool Bm = hmeas(M, mask, hv.mate);
@@ -1191,16 +1208,22 @@ void rfree_reduce(const float* d_terms, const int32_t* d_order, const int32_t* d
     GF_LAUNCH_CHECK();
 }
 
+int hcsr_chunks() { return kHcsrChunks; }
 void hcsr_count(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, uint32_t* d_cnt, hipStream_t st)
 {
-    k_hcsr_count<<<pblocks((int64_t)(n1 / 2 + 1) * n2), 256, 0, st>>>(d_M, d_mask, n1, n2, n3, d_cnt);
+    k_hcsr_count<<<pblocks((int64_t)(n1 / 2 + 1) * n2 * kHcsrChunks), 256, 0, st>>>(d_M, d_mask, n1, n2, n3, d_cnt);
     GF_LAUNCH_CHECK();
 }
-void hcsr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_off,
+void hcsr_pencil_off(const int64_t* d_coff, int64_t np, int64_t* d_off, hipStream_t st)
+{
+    k_hcsr_pencil_off<<<pblocks(np), 256, 0, st>>>(d_coff, np, d_off);
+    GF_LAUNCH_CHECK();
+}
+void hcsr_fill(const float2* d_M, const uint8_t* d_mask, int n1, int n2, int n3, const int64_t* d_coff,
                uint32_t* d_meta, float2* d_val, float2* d_mag, hipStream_t st)
 {
-    k_hcsr_fill<<<pblocks((int64_t)(n1 / 2 + 1) * n2), 256, 0, st>>>(d_M, d_mask, n1, n2, n3, d_off, d_meta, d_val,
-                                                                       d_mag);
+    k_hcsr_fill<<<pblocks((int64_t)(n1 / 2 + 1) * n2 * kHcsrChunks), 256, 0, st>>>(d_M, d_mask, n1, n2, n3, d_coff,
+                                                                                     d_meta, d_val, d_mag);
     GF_LAUNCH_CHECK();
 }
 void hcsr_enforce(int n1, int n2, int n3, const int64_t* d_off, uint32_t* d_meta, double delay, hipStream_t st)

@@ -105,7 +105,7 @@ struct genfire_job {
     bool have_inputs = false, have_support = false, have_init = false;
 
     // gridding
-    gf::DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles, Pt, Phi, Plo, Pscale, nu_inv;
+    gf::DevBuf planes, blk_counts, blk_offsets, total, ind, dist, un, vn, F, tiles, Pt, Phi, Plo, Pscale, nu_inv, scan_tmp;
     gf::DevBuf acc, sw, hits, exact;
     gf::DevBuf M, mask, recifft;
     std::vector<int64_t> proj_off;
@@ -114,7 +114,7 @@ struct genfire_job {
 
     // iteration
     gf::DevBuf x, K, S, best, pfbuf, err_part, errK, errctr, den_tmp, den, recon;
-    gf::DevBuf csr_cnt, csr_off, csr_meta, csr_val, csr_mag, csr_d, csr_mate;
+    gf::DevBuf csr_cnt, csr_off, csr_coff, csr_meta, csr_val, csr_mag, csr_d, csr_mate;
     int64_t nmeas = 0;
     gf::DevBuf vol, vol_y, vol_z, vol_zx, Rmat, rsums, rout;
     // Rfree: withheld voxels per half pencil, their terms, shell order and denominators

@@ -204,8 +204,9 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
     const double cutoff = j->prm.interpolationCutoffDistance;
     grid_index_count(j->planes.as<PlaneConst>(), P, j->hg, cutoff, n1, n2, nblk,
                      j->blk_counts.as<uint32_t>(), st);
+    j->scan_tmp.alloc(grid_index_scan_scratch(nb_total));
     grid_index_scan(j->blk_counts.as<uint32_t>(), nb_total, j->blk_offsets.as<int64_t>(),
-                    j->total.as<int64_t>(), st);
+                    j->total.as<int64_t>(), j->scan_tmp.as<int64_t>(), st);
     std::vector<int64_t> offs(nb_total);
     int64_t npts = 0;
     GF_HIP(hipMemcpyAsync(offs.data(), j->blk_offsets.p, sizeof(int64_t) * nb_total, hipMemcpyDeviceToHost, st));
@@ -648,11 +649,21 @@ void recon_begin(genfire_job* j)
     // measured voxels (errInd, reconstruct.m:99) as per-pencil lists
     const uint8_t* mask = j->have_mask ? j->mask.as<uint8_t>() : nullptr;
     const int64_t npen = half ? (int64_t)(n1 / 2 + 1) * n2 : (int64_t)n1 * n2;
-    j->csr_cnt.alloc(sizeof(uint32_t) * npen);
+    const int64_t nseg = half ? npen * hcsr_chunks() : npen;   // counted segments
+    j->csr_cnt.alloc(sizeof(uint32_t) * nseg);
     j->csr_off.alloc(sizeof(int64_t) * (npen + 1));
-    if (half) hcsr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
-    else csr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
-    grid_index_scan(j->csr_cnt.as<uint32_t>(), npen, j->csr_off.as<int64_t>(), j->csr_off.as<int64_t>() + npen, st);
+    j->scan_tmp.alloc(grid_index_scan_scratch(nseg));
+    if (half) {
+        j->csr_coff.alloc(sizeof(int64_t) * nseg);
+        hcsr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
+        grid_index_scan(j->csr_cnt.as<uint32_t>(), nseg, j->csr_coff.as<int64_t>(), j->csr_off.as<int64_t>() + npen,
+                        j->scan_tmp.as<int64_t>(), st);
+        hcsr_pencil_off(j->csr_coff.as<int64_t>(), npen, j->csr_off.as<int64_t>(), st);
+    } else {
+        csr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
+        grid_index_scan(j->csr_cnt.as<uint32_t>(), npen, j->csr_off.as<int64_t>(), j->csr_off.as<int64_t>() + npen,
+                        j->scan_tmp.as<int64_t>(), st);
+    }
     GF_HIP(hipMemcpyAsync(&j->nmeas, j->csr_off.as<int64_t>() + npen, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     GF_HIP(hipStreamSynchronize(st));
     const int64_t ne = std::max<int64_t>(j->nmeas, 1);
@@ -660,8 +671,9 @@ void recon_begin(genfire_job* j)
     j->csr_val.alloc(sizeof(float2) * ne);
     if (half) {
         j->csr_mag.alloc(sizeof(float2) * ne);
-        hcsr_fill(j->M.as<float2>(), mask, n1, n2, n3, j->csr_off.as<int64_t>(), j->csr_meta.as<uint32_t>(),
+        hcsr_fill(j->M.as<float2>(), mask, n1, n2, n3, j->csr_coff.as<int64_t>(), j->csr_meta.as<uint32_t>(),
                   j->csr_val.as<float2>(), j->csr_mag.as<float2>(), st);
+        j->csr_coff.release();
         hcsr_den(j->csr_mag.as<float2>(), j->csr_meta.as<uint32_t>(), j->nmeas, j->den_tmp.as<double>(), ntmp,
                  j->den.as<double>(), st);
     } else {
