@@ -66,6 +66,7 @@ enum Mode { kPlain = 0, kFirst = 1, kFused = 2, kLast = 3 };
 struct PassArgs {
     IterArgs it;
     float2* data;        // plain pass data (in place)
+    const float2* src;   // plain pass input when not data (out of place), else null
     const float2* tw;    // exp(-2 pi i m / N), m < N
     int axis;
 };
@@ -89,6 +90,7 @@ __device__ __forceinline__ float2 pass_load(const PassArgs& a, float2* K, int64_
         if (it.best) it.best[addr] = r;
         return make_float2(it.dr ? (2.0f * r - xv) : r, 0.0f);
     }
+    if (MODE == kPlain && a.src) return a.src[addr];
     return K[addr];
 }
 
@@ -525,6 +527,20 @@ void fft_axis(float2* d, int n1, int n2, int n3, int axis, int inverse, hipStrea
     a.axis = axis;
     const int n = axis_len(a.it, axis);
     if (n == 1) return;
+    a.tw = fft_twiddles(n);
+    if (inverse) launch<kPlain, true>(n, a, st);
+    else launch<kPlain, false>(n, a, st);
+}
+
+void fft_axis_from(const float2* src, float2* d, int n1, int n2, int n3, int axis, int inverse, hipStream_t st)
+{
+    PassArgs a{};
+    a.it.n1 = n1; a.it.n2 = n2; a.it.n3 = n3;
+    a.data = d;
+    a.src = src;
+    a.axis = axis;
+    const int n = axis_len(a.it, axis);
+    GF_CHECK(n > 1, GENFIRE_ERR_ARG, "out-of-place pass along an axis of length 1");
     a.tw = fft_twiddles(n);
     if (inverse) launch<kPlain, true>(n, a, st);
     else launch<kPlain, false>(n, a, st);

@@ -113,6 +113,8 @@ const float2* fft_twiddles(int n);
 // Complex-to-complex pass along one axis of an n1 x n2 x n3 grid, in place.
 // inverse passes scale by 1/n_axis.
 void fft_axis(float2* d, int n1, int n2, int n3, int axis, int inverse, hipStream_t st);
+// the same pass reading src and writing d (src and d must not overlap)
+void fft_axis_from(const float2* src, float2* d, int n1, int n2, int n3, int axis, int inverse, hipStream_t st);
 void fft3(float2* d, int n1, int n2, int n3, int inverse, hipStream_t st);
 
 // ---- iteration passes (gf_fft.hip) ----

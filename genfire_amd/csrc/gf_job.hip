@@ -367,8 +367,9 @@ void do_grid(genfire_job* j)
     // recIFFT = crop(real(fftshift(ifftn(ifftshift(measuredK))))) (:304)
     if (j->prm.compute_recIFFT) {
         j->K.alloc(sizeof(float2) * full_size(j));
-        GF_HIP(hipMemcpyAsync(j->K.p, j->M.p, j->M.bytes, hipMemcpyDeviceToDevice, st));
-        fft3(j->K.as<float2>(), n1, n2, n1, 1, st);
+        fft_axis_from(j->M.as<float2>(), j->K.as<float2>(), n1, n2, n1, 0, 1, st);   // M stays as gridded
+        fft_axis(j->K.as<float2>(), n1, n2, n1, 1, 1, st);
+        fft_axis(j->K.as<float2>(), n1, n2, n1, 2, 1, st);
         j->recifft.alloc(sizeof(float) * box_size(j));
         crop_centre(j->K.p, 1, n1, n2, n1, j->recifft.as<float>(), j->dim1, j->dim2, j->dim1, st);
     }
