@@ -212,6 +212,31 @@ __global__ void k_pad(const T* __restrict__ src, int d1, int d2, int d3, T* __re
     }
 }
 
+// The same pad for bytes (the support mask): one workgroup per (row segment of
+// 1024, y, z), four consecutive x per thread written as one word, no division.
+__global__ __launch_bounds__(256) void k_pad_u8(const uint8_t* __restrict__ src, int d1, int d2, int d3,
+                                                uint8_t* __restrict__ dst, int n1, int n2, int n3)
+{
+    const int s2 = blockIdx.y, s3 = blockIdx.z, s1 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (s1 >= n1) return;
+    const int b = signed_freq(s2, n2) + d2 / 2, c = signed_freq(s3, n3) + d3 / 2;
+    const bool bc = b >= 0 && b < d2 && c >= 0 && c < d3;
+    const uint8_t* row = bc ? src + (int64_t)d1 * (b + (int64_t)d2 * c) : src;
+    uint8_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int a = s1 + q < n1 ? signed_freq(s1 + q, n1) + d1 / 2 : -1;
+        v[q] = (bc && a >= 0 && a < d1) ? row[a] : (uint8_t)0;
+    }
+    uint8_t* out = dst + s1 + (int64_t)n1 * (s2 + (int64_t)n2 * s3);
+    if (s1 + 4 <= n1 && (n1 & 3) == 0) {
+        *reinterpret_cast<uint32_t*>(out) = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) |
+                                            ((uint32_t)v[3] << 24);
+    } else {
+        for (int q = 0; q < 4 && s1 + q < n1; ++q) out[q] = v[q];
+    }
+}
+
 __global__ void k_expand_f16(const uint16_t* __restrict__ h, const float* __restrict__ scale, int64_t per_proj,
                              int64_t total, float* __restrict__ out)
 {
@@ -292,8 +317,9 @@ void pad_centre(const float* d_src, int d1, int d2, int d3, float* d_dst, int n1
 void pad_centre_u8(const uint8_t* d_src, int d1, int d2, int d3, uint8_t* d_dst, int n1, int n2,
                    int n3, hipStream_t st)
 {
-    const int64_t total = (int64_t)n1 * n2 * n3;
-    k_pad<uint8_t><<<grid_blocks(total, 256), 256, 0, st>>>(d_src, d1, d2, d3, d_dst, n1, n2, n3);
+    GF_CHECK(n2 <= 65535 && n3 <= 65535, GENFIRE_ERR_UNSUPPORTED, "grid edge above 65535");
+    const dim3 grid((n1 + 1023) / 1024, n2, n3);
+    k_pad_u8<<<grid, 256, 0, st>>>(d_src, d1, d2, d3, d_dst, n1, n2, n3);
     GF_LAUNCH_CHECK();
 }
 
