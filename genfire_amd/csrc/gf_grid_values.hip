@@ -133,33 +133,33 @@ __device__ __forceinline__ int signed_freq(int s, int n) { return s < (n + 1) / 
 // (GENFIRE_Reconstructor.m:182); CentroSymmetricity.m: k1 <= 0 from FS(k),
 // k1 > 0 from conj(FS(-k)) (all four parity branches reduce to this rule).
 // Written directly in FFT order: voxel s holds frequency k = s (mod n).
-__global__ void k_finalize(HalfGrid hg, int n1, int n2, int centro, const double2* __restrict__ acc,
-                           const double* __restrict__ sw, const uint32_t* __restrict__ hits,
-                           const float2* __restrict__ exact, float2* __restrict__ M)
+// One workgroup per (row segment of 256, y, z) of the FFT-order grid: no
+// division; a voxel with no hits (most of the grid) reads only its hit count.
+__global__ __launch_bounds__(256) void k_finalize(HalfGrid hg, int n1, int n2, int centro,
+                                                  const double2* __restrict__ acc, const double* __restrict__ sw,
+                                                  const uint32_t* __restrict__ hits, const float2* __restrict__ exact,
+                                                  float2* __restrict__ M)
 {
-    const int64_t total = (int64_t)n1 * n2 * n1;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total;
-         s += (int64_t)gridDim.x * blockDim.x) {
-        const int s1 = (int)(s % n1);
-        const int s2 = (int)((s / n1) % n2);
-        const int s3 = (int)(s / ((int64_t)n1 * n2));
-        int k1 = signed_freq(s1, n1), k2 = signed_freq(s2, n2), k3 = signed_freq(s3, n1);
-        bool cj = false;
-        if (centro && k1 > 0) { k1 = -k1; k2 = -k2; k3 = -k3; cj = true; }
-        const int64_t h = (int64_t)(k1 - hg.lo[0]) +
-                          (int64_t)hg.len[0] * ((k2 - hg.lo[1]) + (int64_t)hg.len[1] * (k3 - hg.lo[2]));
-        float2 v = make_float2(0.0f, 0.0f);
-        const uint32_t hh = hits[h];
-        if (hh & kExactBit) {
-            v = exact[h];
-        } else if (sw[h] > 0.0) {
+    const int s1 = blockIdx.x * 256 + threadIdx.x, s2 = blockIdx.y, s3 = blockIdx.z;
+    if (s1 >= n1) return;
+    int k1 = signed_freq(s1, n1), k2 = signed_freq(s2, n2), k3 = signed_freq(s3, n1);
+    bool cj = false;
+    if (centro && k1 > 0) { k1 = -k1; k2 = -k2; k3 = -k3; cj = true; }
+    const int64_t h = (int64_t)(k1 - hg.lo[0]) +
+                      (int64_t)hg.len[0] * ((k2 - hg.lo[1]) + (int64_t)hg.len[1] * (k3 - hg.lo[2]));
+    float2 v = make_float2(0.0f, 0.0f);
+    const uint32_t hh = hits[h];
+    if (hh & kExactBit) {
+        v = exact[h];
+    } else if (hh != 0) {   // hits > 0: positive weights were summed
+        const double w = sw[h];
+        if (w > 0.0) {
             const double2 a = acc[h];
-            const double w = sw[h];
             v = make_float2((float)(a.x / w), (float)(a.y / w));
         }
-        if (cj) v.y = -v.y;
-        M[s] = v;
     }
+    if (cj) v.y = -v.y;
+    M[(int64_t)s1 + (int64_t)n1 * (s2 + (int64_t)n2 * s3)] = v;
 }
 
 __global__ void k_shift(const float2* __restrict__ in, float2* __restrict__ out, int n1, int n2,
@@ -277,9 +277,9 @@ void grid_finalize(const HalfGrid& hg, int n1, int n2, int centro, const double2
                    const double* d_sw, const uint32_t* d_hits, const float2* d_exact, float2* d_M,
                    hipStream_t st)
 {
-    const int64_t total = (int64_t)n1 * n2 * n1;
-    k_finalize<<<grid_blocks(total, 256), 256, 0, st>>>(hg, n1, n2, centro, d_acc, d_sw, d_hits,
-                                                        d_exact, d_M);
+    GF_CHECK(n2 <= 65535 && n1 <= 65535, GENFIRE_ERR_UNSUPPORTED, "grid edge above 65535");
+    const dim3 grid((n1 + 255) / 256, n2, n1);
+    k_finalize<<<grid, 256, 0, st>>>(hg, n1, n2, centro, d_acc, d_sw, d_hits, d_exact, d_M);
     GF_LAUNCH_CHECK();
 }
 
