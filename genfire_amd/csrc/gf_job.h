@@ -121,6 +121,8 @@ struct genfire_job {
     gf::DevBuf rf_off, rf_meta, rf_val, rf_terms, rf_order, rf_shell_off, rf_den, Rfree;
     int rf_bins = 0;
     std::vector<float> Rfree_h;
+    std::vector<gf::PointTile> tiles_h;  // host source of tiles (outlives its asynchronous copy)
+    std::vector<float> nu_inv_h;         // host source of nu_inv, the NUFFT deconvolution factors
     bool rmat_any_y = false;      // some projection reads a y-fastest form of the model
     bool rmat_any_x = false;      // some projection reads the x-fastest model
     bool rmat_any_exact = false;  // some projection maps y exactly onto y (the y-fastest copy itself)

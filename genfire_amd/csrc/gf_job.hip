@@ -237,7 +237,9 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
     const int nu1 = nufft_fine_len(j->dim1), nu2 = nufft_fine_len(j->dim2);
     const bool use_nufft = prec == GENFIRE_PREC_FP32 && !use_valu && gks != "mfma" && nu1 > 0 && nu2 > 0;
     const int tile_pts = use_valu ? 256 : mfma_tile_points();
-    std::vector<PointTile> tl;
+    // kept in the job: the asynchronous copy below may read it after this scope
+    std::vector<PointTile>& tl = j->tiles_h;
+    tl.clear();
     for (int p = 0; p < P; ++p)
         for (int64_t b = j->proj_off[p]; b < j->proj_off[p + 1]; b += tile_pts)
             tl.push_back({b, (int32_t)std::min<int64_t>(tile_pts, j->proj_off[p + 1] - b), p});
@@ -252,10 +254,13 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
             grid_prep_lp(prec, j->proj.as<float>(), j->dim1, j->dim2, P, j->Phi.as<uint16_t>(),
                          j->Plo.as<uint16_t>(), j->Pscale.as<float>(), st);
         } else if (use_nufft) {
-            std::vector<float> inv = nufft_deconv(j->dim1, nu1), inv2 = nufft_deconv(j->dim2, nu2);
-            inv.insert(inv.end(), inv2.begin(), inv2.end());
-            j->nu_inv.alloc(sizeof(float) * inv.size());
-            GF_HIP(hipMemcpyAsync(j->nu_inv.p, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice, st));
+            // kept in the job, as the tiles are
+            j->nu_inv_h = nufft_deconv(j->dim1, nu1);
+            const std::vector<float> inv2 = nufft_deconv(j->dim2, nu2);
+            j->nu_inv_h.insert(j->nu_inv_h.end(), inv2.begin(), inv2.end());
+            j->nu_inv.alloc(sizeof(float) * j->nu_inv_h.size());
+            GF_HIP(hipMemcpyAsync(j->nu_inv.p, j->nu_inv_h.data(), sizeof(float) * j->nu_inv_h.size(),
+                                  hipMemcpyHostToDevice, st));
             j->Pt.alloc(sizeof(float2) * (size_t)nu1 * nu2 * P);   // the padded spectra
         } else if (!use_valu) {
             j->Pt.alloc(sizeof(float) * grid_prep_floats(j->dim1, j->dim2, P));
