@@ -32,10 +32,16 @@ typedef float pair_t __attribute__((ext_vector_type(2), aligned(4)));
 typedef float quad_t __attribute__((ext_vector_type(4), aligned(8)));
 
 // The workgroups of one projection run on one XCD: the hardware deals workgroups
-// to the 8 XCDs round-robin, so physical block b goes to XCD b % 8 and is mapped
-// to projection xcd + 8 * (slot / tiles) -- every XCD's L2 then streams whole
-// projections, neighbouring tiles sharing the voxels of their boundary rays.
+// to the 8 XCDs round-robin, so physical block b goes to XCD b % 8.  XCD x takes
+// the contiguous block of per = ceil(P / 8) projections starting at x * per, in
+// groups of G neighbouring projections whose tiles are interleaved (slot =
+// ((group * tiles) + tile) * G + member).  Neighbouring projections of a tilt
+// series send nearly the same rays through the model, so the G workgroups of
+// one tile read the same voxels at about the same time and share the XCD's L2
+// fills: each projection reads the whole model (twice the voxels in the z-pair
+// copies), which would otherwise stream from HBM or the MALL once per projection.
 constexpr int kXcd = 8;
+constexpr int kProjGroup = 6;   // G, tilt256's 91 projections: 12 per XCD, two groups
 constexpr int kLanes = 64;   // pixels of one wave along the fast pixel axis
 constexpr int kRows = 4;     // waves per workgroup along the other pixel axis
 constexpr float kRowAlign = 0.9f;   // lane rows when a pairing's |R| reaches this, else patches
@@ -144,12 +150,13 @@ __device__ __forceinline__ int ytilt_tiles(const ProjArgs& a)
 __device__ __forceinline__ void tile_sums(const ProjArgs& a, int p, int tile, int tiles, double q0, double q1,
                                           double q2, double q3);
 
-__global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles)
+__global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles, int per, int group)
 {
     const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
-    const int p = xcd + kXcd * (slot / tiles);
-    const int tile = slot % tiles;
-    if (p >= a.P) return;
+    const int k = (slot / (group * tiles)) * group + slot % group;   // projection within the XCD's block
+    const int p = xcd * per + k;
+    const int tile = (slot / group) % tiles;
+    if (k >= per || p >= a.P) return;
     const float* R = a.R + kRStride * p;
     if (R[10] != 0.0f && a.d2 % 4 == 0 && ytilt_tiles(a) <= tiles) {
         double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
@@ -510,8 +517,9 @@ void project(const ProjArgs& a, hipStream_t st)
     GF_CHECK(a.m1 <= a.d1 && a.m2 <= a.d2, GENFIRE_ERR_ARG, "measured projection larger than the volume");
     GF_CHECK((int64_t)a.d1 * a.d2 * a.d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "projector volume above 2^31 voxels");
     const int tiles = project_blocks(a.m1, a.m2);
-    const int groups = (a.P + kXcd - 1) / kXcd;
-    k_project<<<(unsigned)(kXcd * groups * tiles), kLanes * kRows, 0, st>>>(a, tiles);
+    const int per = (a.P + kXcd - 1) / kXcd, group = std::min(kProjGroup, per);
+    const int groups = (per + group - 1) / group;
+    k_project<<<(unsigned)(kXcd * groups * group * tiles), kLanes * kRows, 0, st>>>(a, tiles, per, group);
     GF_LAUNCH_CHECK();
 }
 
