@@ -78,6 +78,28 @@ def test_rfactor_matches_oracle(ctx, O):
     np.testing.assert_allclose(sims, sims_r, rtol=0, atol=2e-5 * np.abs(sims_r).max())
 
 
+@pytest.mark.parametrize("single_axis", [False, True])
+@pytest.mark.parametrize("P", [17, 53])
+def test_rfactor_many_projections_matches_oracle(ctx, O, P, single_axis):
+    # more than 8 projections: each XCD takes a block of ceil(P/8) and
+    # interleaves groups of neighbours (17: groups of 3, the last blocks short
+    # or empty; 53: groups of 6, every block's second group partly filled);
+    # a single-axis series takes the four-ray form
+    print(f"[proj] rfactor {P} projections, single axis {single_axis}", flush=True)
+    rng = np.random.default_rng(7)
+    vol = rng.uniform(0, 1, (24, 24, 24)).astype(np.float32)
+    pp = np.zeros(P) if single_axis else rng.uniform(-5, 5, P)
+    ps = np.zeros(P) if single_axis else rng.uniform(-5, 5, P)
+    ang = np.stack([pp, np.linspace(-70, 70, P), ps], 1)
+    projs = np.stack([O.project(vol, *a) for a in ang], 2)[2:22, 3:21]
+    projs = (projs * rng.uniform(0.9, 1.1, projs.shape)).astype(np.float32)
+    R, Rarr, sims = ctx.rfactor(vol, projs, ang)
+    Rr, Rarr_r, sims_r = O.rfactor_posthoc(vol, projs, ang)
+    assert abs(R - Rr) < 1e-5
+    np.testing.assert_allclose(Rarr, Rarr_r, atol=1e-5)
+    np.testing.assert_allclose(sims, sims_r, rtol=0, atol=2e-5 * np.abs(sims_r).max())
+
+
 @pytest.mark.parametrize("vectors,angles", [
     (((0, 0, 1), (0, 1, 0), (1, 0, 0)), (10.0, -40.0, 25.0)),
     (((0, 0, 1), (0, 1, 0), (0, 0, 1)), (3.0, 65.0, -4.0)),
