@@ -40,6 +40,8 @@ typedef float quad_t __attribute__((ext_vector_type(4), aligned(8)));
 // one tile read the same voxels at about the same time and share the XCD's L2
 // fills: each projection reads the whole model (twice the voxels in the z-pair
 // copies), which would otherwise stream from HBM or the MALL once per projection.
+// With at most 8 projections (per = 1) every XCD instead takes a contiguous
+// eighth of each projection's tiles, so a single projection uses the whole chip.
 constexpr int kXcd = 8;
 constexpr int kProjGroup = 6;   // G, tilt256's 91 projections: 12 per XCD, two groups
 constexpr int kLanes = 64;   // pixels of one wave along the fast pixel axis
@@ -153,10 +155,18 @@ __device__ __forceinline__ void tile_sums(const ProjArgs& a, int p, int tile, in
 __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles, int per, int group)
 {
     const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
-    const int k = (slot / (group * tiles)) * group + slot % group;   // projection within the XCD's block
-    const int p = xcd * per + k;
-    const int tile = (slot / group) % tiles;
-    if (k >= per || p >= a.P) return;
+    int p, tile;
+    if (per == 1) {   // at most 8 projections: XCD x takes the x-th eighth of every projection's tiles
+        const int tc = (tiles + kXcd - 1) / kXcd;
+        p = slot / tc;
+        tile = xcd * tc + slot % tc;
+        if (tile >= tiles || p >= a.P) return;
+    } else {
+        const int k = (slot / (group * tiles)) * group + slot % group;   // projection within the XCD's block
+        p = xcd * per + k;
+        tile = (slot / group) % tiles;
+        if (k >= per || p >= a.P) return;
+    }
     const float* R = a.R + kRStride * p;
     if (R[10] != 0.0f && a.d2 % 4 == 0 && ytilt_tiles(a) <= tiles) {
         double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
@@ -519,7 +529,9 @@ void project(const ProjArgs& a, hipStream_t st)
     const int tiles = project_blocks(a.m1, a.m2);
     const int per = (a.P + kXcd - 1) / kXcd, group = std::min(kProjGroup, per);
     const int groups = (per + group - 1) / group;
-    k_project<<<(unsigned)(kXcd * groups * group * tiles), kLanes * kRows, 0, st>>>(a, tiles, per, group);
+    const int64_t blocks = per == 1 ? (int64_t)kXcd * a.P * ((tiles + kXcd - 1) / kXcd)
+                                    : (int64_t)kXcd * groups * group * tiles;
+    k_project<<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
     GF_LAUNCH_CHECK();
 }
 
