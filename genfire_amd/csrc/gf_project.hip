@@ -531,6 +531,9 @@ void project(const ProjArgs& a, hipStream_t st)
     const int groups = (per + group - 1) / group;
     const int64_t blocks = per == 1 ? (int64_t)kXcd * a.P * ((tiles + kXcd - 1) / kXcd)
                                     : (int64_t)kXcd * groups * group * tiles;
+    // a dispatch counts its x extent in work items, in 32 bits
+    GF_CHECK(blocks * kLanes * kRows < (1ll << 32), GENFIRE_ERR_UNSUPPORTED,
+             "projector launch above 2^32 work items: too many projections of this size");
     k_project<<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
     GF_LAUNCH_CHECK();
 }
