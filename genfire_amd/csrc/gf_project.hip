@@ -31,11 +31,12 @@ typedef float pair_t __attribute__((ext_vector_type(2), aligned(4)));
 // Two pairs of the z-pair copies in one 16-byte access (8-byte aligned).
 typedef float quad_t __attribute__((ext_vector_type(4), aligned(8)));
 
-// The workgroups of one projection run on one XCD: the hardware deals workgroups
-// to the 8 XCDs round-robin, so physical block b goes to XCD b % 8.  XCD x takes
-// the contiguous block of per = ceil(P / 8) projections starting at x * per, in
-// groups of G neighbouring projections whose tiles are interleaved (slot =
-// ((group * tiles) + tile) * G + member).  Neighbouring projections of a tilt
+// With more than 8 projections the workgroups of one projection run on one XCD:
+// the hardware deals workgroups to the 8 XCDs round-robin, so physical block b
+// goes to XCD b % 8.  XCD x takes the contiguous block of per = ceil(P / 8)
+// projections starting at x * per, in groups of G neighbouring projections whose
+// tiles are interleaved (slot = (g * tiles + tile) * G + m for member m of the
+// XCD's group g).  Neighbouring projections of a tilt
 // series send nearly the same rays through the model, so the G workgroups of
 // one tile read the same voxels at about the same time and share the XCD's L2
 // fills: each projection reads the whole model (twice the voxels in the z-pair
@@ -523,6 +524,7 @@ void crop_zpairs(const float* d_src, int n1, int n2, int n3, float* d_dst, int d
 
 void project(const ProjArgs& a, hipStream_t st)
 {
+    GF_CHECK(a.P >= 1, GENFIRE_ERR_ARG, "projector needs at least one projection");
     GF_CHECK(a.d1 >= 2 && a.d2 >= 2 && a.d3 >= 2, GENFIRE_ERR_ARG, "projector needs every volume dimension >= 2");
     GF_CHECK(a.m1 <= a.d1 && a.m2 <= a.d2, GENFIRE_ERR_ARG, "measured projection larger than the volume");
     GF_CHECK((int64_t)a.d1 * a.d2 * a.d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "projector volume above 2^31 voxels");
