@@ -125,3 +125,15 @@ def test_project_general_matches_oracle(ctx, O, vectors, angles):
         pj = projs[:, :, i].astype(np.float64)
         ref_r = np.sum(np.abs(np.abs(back) - np.abs(pj))) / np.sum(np.abs(pj))
         assert abs(Rarr[i] - ref_r) < 1e-5
+
+
+def test_rfactor_rejects_empty_projection_set(ctx):
+    # Tian_calc_Rfactor_realspace over no projections has no R-factor: an
+    # argument error, before any launch
+    from genfire_amd._lib import GenfireError
+    vol = np.zeros((8, 8, 8), np.float32)
+    with pytest.raises(GenfireError):
+        ctx.rfactor(vol, np.zeros((8, 8, 0), np.float32), np.zeros((0, 3)))
+    # the context still serves a valid call afterwards
+    R, Rarr, _ = ctx.rfactor(vol + 1.0, np.ones((8, 8, 1), np.float32), np.zeros((1, 3)))
+    assert Rarr.shape == (1,) and np.isfinite(R)
