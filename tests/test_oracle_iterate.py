@@ -191,3 +191,92 @@ def test_general_projector_quarter_turn_about_x(O):
     expect = expect if np.sign(R[2, 1]) > 0 else expect[:, ::-1]
     np.testing.assert_allclose(got, expect, rtol=0, atol=1e-5)
     assert s != 0
+
+
+# Nonzero closed forms of the errK and R-factor normalisations
+# (reconstruct.m:135,176-177): a wrong denominator, a missing absolute value or
+# a missing square root changes each of these values.
+
+@pytest.mark.parametrize("alg", ["genfire", "dr"])
+def test_errk_is_one_at_zero_iterate(O, alg):
+    """x = 0 at the first iteration: k = fftn(0) = 0, so errK(1) = sum|M| / sum|M| = 1
+    over any measured set."""
+    obj, M = _obj(seed=4)
+    meas = np.random.default_rng(6).random(M.shape) < 0.3
+    r = O.reconstruct(np.where(meas, M, 0), meas, obj > 0, 8, 8, 1, mode=3, algorithm=alg, precision="fp64")
+    assert abs(r.errK[0] - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("c", [0.25, 0.6, 1.5, 3.0])
+def test_errk_scaled_object(O, c):
+    """initialObject = c x* (x* >= 0 in the support), M = fftn(x*) on a measured
+    subset: errK(1) = sum||c k| - |k|| / sum|k| = |1 - c| (a denominator of
+    sum|c k|, or squared magnitudes, gives a different value)."""
+    obj, M = _obj(seed=5)
+    meas = np.random.default_rng(7).random(M.shape) < 0.4
+    r = O.reconstruct(np.where(meas, M, 0), meas, obj > 0, 8, 8, 1, mode=3, precision="fp64",
+                      initial_object=(c * obj).astype(np.float32))
+    assert abs(r.errK[0] - abs(1.0 - c)) < 1e-6, (r.errK[0], c)
+
+
+def test_errk_uses_magnitudes_only(O):
+    """errK compares magnitudes (abs(abs(k) - abs(M))): measured values with their
+    phases scrambled leave errK(1) at 0 for the object they came from, and a
+    measured set whose magnitudes are doubled gives exactly 1/2."""
+    obj, M = _obj(seed=6)
+    meas = np.abs(M) > 1e-9
+    ph = np.exp(1j * np.random.default_rng(8).uniform(0, 2 * np.pi, M.shape))
+    r = O.reconstruct(np.where(meas, M * ph, 0), meas, obj > 0, 8, 8, 1, mode=3, precision="fp64",
+                      initial_object=obj)
+    assert r.errK[0] < 1e-6
+    r2 = O.reconstruct(np.where(meas, 2 * M, 0), meas, obj > 0, 8, 8, 1, mode=3, precision="fp64",
+                       initial_object=obj)
+    assert abs(r2.errK[0] - 0.5) < 1e-6
+
+
+def test_rfactors_zero_model(O):
+    """A zero model projects to zero: R1 = sum|pj| / sum|pj| = 1 and
+    R2 = ||pj|| / ||pj|| = 1 for every projection."""
+    rng = np.random.default_rng(10)
+    v = rng.uniform(0, 1, (12, 12, 12)).astype(np.float32)
+    ang = rng.uniform(-30, 30, (4, 3))
+    projs = np.stack([O.project(v, *a) for a in ang], 2)
+    R1, R2 = O.rfactors(np.zeros_like(v), projs, ang)
+    np.testing.assert_allclose(R1, 1.0, atol=1e-12)
+    np.testing.assert_allclose(R2, 1.0, atol=1e-12)
+    Rm, Rarr, _ = O.rfactor_posthoc(np.zeros_like(v), projs, ang)
+    np.testing.assert_allclose(Rarr, 1.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("c", [0.5, 0.9, 1.25, 2.0])
+def test_rfactors_scaled_model(O, c):
+    """The projector is linear: a model c x* on data projected from x* gives
+    back = c pj, so R1 = R2 = |1 - c| (an R2 without the square root would give
+    (1 - c)^2, a denominator over back would give |1 - c| / c).  With pj >= 0,
+    Tian's post-hoc ||back| - |pj|| form gives the same |1 - c|."""
+    rng = np.random.default_rng(11)
+    v = rng.uniform(0, 1, (12, 12, 12)).astype(np.float32)
+    ang = rng.uniform(-30, 30, (4, 3))
+    projs = np.stack([O.project(v, *a) for a in ang], 2)
+    R1, R2 = O.rfactors((c * v.astype(np.float64)).astype(np.float32), projs, ang)
+    np.testing.assert_allclose(R1, abs(1 - c), atol=1e-6)
+    np.testing.assert_allclose(R2, abs(1 - c), atol=1e-6)
+    Rm, Rarr, _ = O.rfactor_posthoc((c * v.astype(np.float64)).astype(np.float32), projs, ang)
+    np.testing.assert_allclose(Rarr, abs(1 - c), atol=1e-6)
+    assert abs(Rm - abs(1 - c)) < 1e-6
+
+
+def test_rfactor_monitor_records_scaled_iterate(O):
+    """The in-loop monitor (reconstruct.m:168-185) on a fixed point whose
+    measured values are c F(x*): after the first iteration the iterate is c x*
+    (every voxel enforced), so Rarr = Rarr2 = |1 - c| per projection and errR is
+    their mean."""
+    c = 0.7
+    obj, M = _obj(seed=9)
+    ang = np.random.default_rng(12).uniform(-30, 30, (3, 3))
+    projs = np.stack([O.project(obj, *a) for a in ang], 2)
+    r = O.reconstruct(c * M, np.ones(M.shape, bool), obj > 0, 8, 8, 4, mode=3, precision="fp64",
+                      monitor_R=True, loop_length=2, projections=projs, angles=ang)
+    np.testing.assert_allclose(r.Rarr_record, abs(1 - c), atol=1e-5)
+    np.testing.assert_allclose(r.Rarr2_record, abs(1 - c), atol=1e-5)
+    np.testing.assert_allclose(r.errR, abs(1 - c), atol=1e-5)
