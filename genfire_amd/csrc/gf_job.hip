@@ -601,7 +601,10 @@ void recon_begin(genfire_job* j)
     r.L = std::max(1, j->prm.monitorR_loopLength);
     const bool dr = r.dr, monitor = r.monitor;
 
-    r.half = half_supported(n1, n2, n3);
+    // GENFIRE_HALF=0 runs the full-spectrum (complex) passes on a grid the
+    // half-spectrum passes also handle (the two forms compared in the tests).
+    const char* hf = std::getenv("GENFIRE_HALF");
+    r.half = half_supported(n1, n2, n3) && !(hf && std::string(hf) == "0");
     const bool half = r.half;
     r.hn = (int64_t)(n1 / 2 + 1) * n2 * n3;   // half-spectrum size
     const int64_t hn = r.hn;

@@ -94,6 +94,33 @@ def test_half_spectrum_iteration_matches_oracle(ctx, O, alg, shape):
     np.testing.assert_allclose(errR, ref.errR, atol=1e-4, rtol=0)
 
 
+@pytest.mark.parametrize("alg,shape", [("genfire", (256, 32, 10)), ("dr", (256, 32, 10)),
+                                       ("genfire", (64, 256, 10)), ("dr", (64, 256, 10)),
+                                       ("dr", (512, 32, 6)), ("genfire", (64, 512, 6)), ("dr", (64, 512, 6))])
+def test_half_spectrum_large_axes_match_oracle(ctx, O, alg, shape):
+    """The pass lengths of the 512^3 and 1024^3 grids on grids the FP64 oracle
+    finishes in seconds: 512 x 64 x 512 and 1024 x 64 x 1024 (x pass at 256 and
+    512 complex points, the half-box z pass at 512 and 1024, whose tiles need
+    more than 64 KiB of LDS at 1024), 128 x 512 x 128 and 128 x 1024 x 128 (the
+    y passes at 512 and 1024).  The box is Dim = n/2 along every axis, as in the
+    benchmarked workloads; R-factor monitor every 5 or 3 iterations."""
+    d1, d2, iters = shape
+    L = 5 if iters >= 10 else 3
+    print(f"[iter] large axes alg={alg} dims={shape}", flush=True)
+    vol, projs, ang, mk, sup = _setup(O, dim=d1, dim2=d2, P=13, seed=3)
+    job = _job(ctx, mk, projs, ang, sup, iters, mode=1, alg=alg, monitor=1, L=L)
+    ref = O.reconstruct(mk, mk != 0, sup, d1, d2, iters, mode=1, algorithm=alg, precision="fp64",
+                        monitor_R=True, loop_length=L, projections=projs, angles=ang)
+    rec = job.reconstruction()
+    err = np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction)
+    assert err < 1e-4, err
+    np.testing.assert_allclose(job.errK(), ref.errK, atol=2e-5, rtol=0)
+    errR, R1, R2 = job.errR()
+    np.testing.assert_allclose(errR, ref.errR, atol=1e-4, rtol=0)
+    np.testing.assert_allclose(R1, ref.Rarr_record, atol=1e-4, rtol=0)
+    np.testing.assert_allclose(R2, ref.Rarr2_record, atol=1e-4, rtol=0)
+
+
 def test_fixed_point(ctx, O):
     """M = fftn of a positive supported object, every point enforced: the object is
     reproduced and errK -> 0 (GENFIRE fixed point)."""
