@@ -4,9 +4,13 @@
 A step is one whole reconstruction job of the tilt256 workload with its inputs
 already resident in HBM: DFT gridding of 91 projections (256 x 256) onto the
 512^3 oversampled grid, 200 Douglas-Rachford iterations, and the real-space
-R-factor of all 91 projections every 10 iterations.  The metric is
+R-factor of all 91 projections every 10 iterations.  The timed steps run the
+library's default path (steady-state iterations replay a captured HIP graph);
+per-kernel HIP-event timings come from one more, untimed step.  The metric is
 reconstructions per hour; gridded k-points per second, p50/p99 job latency and
-the final R-factor are reported beside it.
+the final R-factor are reported beside it, with the served forms (FP32 and
+reduced-precision gridding), a serving-mix summary, the CPU baseline and the
+FP64-oracle checks.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu-baseline]
 
@@ -33,67 +37,145 @@ HBM_PEAK_GBS = 8000.0
 # on a known 1 GiB fill), in profiles/tilt256_hbm_traffic.md: 65853 KiB
 # fetched, 263168 KiB written.
 TRAFFIC = {512: 2 * 65853 * 1024 + 263168 * 1024}
+# HBM/MALL bytes per launch of the R-factor projector (same counters and
+# factors), keyed by (workload, three-Euler series); None where not measured.
+# tilt256 (three Euler angles): FETCH_SIZE 5,836,096 KiB (x2.00) + WRITE_SIZE
+# 2,992 KiB per launch, profiles/project_tilt256_traffic.md.
+PROJ_TRAFFIC = {("tilt256", True): 2 * 5836096 * 1024 + 2992 * 1024}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, projs, ang, sup, mk_centred, model, n_cores):
-    """The oracle (C + numpy restatement of the reference CPU path) on a bounded
-    sample of the same workload, extrapolated to one whole job."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_sample(w, projs, ang, sup, mk_centred, model, threads, n_grid, n_iter, n_proj):
+    """Seconds per unit of the oracle's restatement of the reference CPU path on
+    `threads` cores: per gridded projection (full-grid FP32 index scan, then the
+    values by the CPU NUFFT and, reported beside it, by the FP64 direct DFT),
+    per DR iteration on the n^3 grid, per projected R-factor projection."""
     from oracle import oracle as O
     import scipy.fft
 
+    O.set_threads(threads)
     n = int(round(w.dim * w.oversampling))
-    vecs = w.vectors
-    sample = {}
-    # gridding: 2 projections through the full-grid index scan + FP64 direct DFT
-    t0 = time.perf_counter()
-    npts = 0
-    for p in (0, w.num_projs // 2):
-        Rf, _ = O.rotation(*ang[p], *vecs)
+    t_scan = t_nufft = t_dft = 0.0
+    for p in np.linspace(0, w.num_projs - 1, n_grid).astype(int):
+        t0 = time.perf_counter()
+        Rf, _ = O.rotation(*ang[p], *w.vectors)
         ind, D, u, v, _ = O.select(n, n, True, Rf, w.cutoff)
         un = (u / np.float32(n)).astype(np.float32)
         vn = (v / np.float32(n)).astype(np.float32)
+        t1 = time.perf_counter()
+        O.nufft2d2(projs[:, :, p], un, vn, workers=threads)
+        t2 = time.perf_counter()
         O.dft64(projs[:, :, p], un, vn)
-        npts += len(ind)
-    t_grid_proj = (time.perf_counter() - t0) / 2
-    # iteration: 2 DR iterations' worth of FFT + constraint + replacement on the n^3 grid
+        t3 = time.perf_counter()
+        t_scan += t1 - t0
+        t_nufft += t2 - t1
+        t_dft += t3 - t2
     shape = (n, n, n)
     x = np.zeros(shape, np.float32)
     S = np.fft.ifftshift(O.paddzero(sup.astype(np.float32), shape))
     M = np.fft.ifftshift(mk_centred)
     meas = M != 0
+    Mm = M[meas]
     t0 = time.perf_counter()
-    for _ in range(2):
+    for _ in range(n_iter):
         r = np.where(x < 0, np.float32(0), x) * S
         y = 2 * r - x
-        k = scipy.fft.fftn(y.astype(np.complex64), workers=n_cores)
-        _ = np.sum(np.abs(np.abs(k[meas]) - np.abs(M[meas])))
-        k[meas] = M[meas]
-        pf = np.real(scipy.fft.ifftn(k, workers=n_cores)).astype(np.float32)
+        k = scipy.fft.fftn(y.astype(np.complex64), workers=threads)
+        _ = np.sum(np.abs(np.abs(k[meas]) - np.abs(Mm)))
+        k[meas] = Mm
+        pf = np.real(scipy.fft.ifftn(k, workers=threads)).astype(np.float32)
         x = x + pf - r
-    t_iter = (time.perf_counter() - t0) / 2
-    del x, S, M, meas, k, pf, r, y
-    # R-factor: 2 projections of the Dim^3 model with the FP64 trilinear projector
+    t_iter = (time.perf_counter() - t0) / n_iter
+    del x, S, M, meas, Mm, k, pf, r, y
     t0 = time.perf_counter()
-    for p in (0, 1):
+    for p in range(n_proj):
         O.project(model, *ang[p])
-    t_proj = (time.perf_counter() - t0) / 2
+    t_proj = (time.perf_counter() - t0) / n_proj
+    return {"scan": t_scan / n_grid, "nufft": t_nufft / n_grid, "dft": t_dft / n_grid, "iter": t_iter,
+            "proj": t_proj}
+
+
+def cpu_baseline(w, projs, ang, sup, mk_centred, model, n_cores):
+    """The oracle (C/OpenMP + numpy/pocketfft restatement of the reference CPU
+    path) on a bounded sample of the same workload, extrapolated to one whole
+    job: on n_cores threads (3 projections gridded, 2 iterations, 3 projector
+    calls) and on one core (1, 1, 1)."""
+    n = int(round(w.dim * w.oversampling))
     n_eval = w.iterations // w.monitor_every if w.monitor_R else 0
-    total = w.num_projs * t_grid_proj + w.iterations * t_iter + n_eval * w.num_projs * t_proj
+
+    def job_s(t, values):
+        return (w.num_projs * (t["scan"] + t[values]) + w.iterations * t["iter"] +
+                n_eval * w.num_projs * t["proj"])
+
+    tm = _cpu_sample(w, projs, ang, sup, mk_centred, model, n_cores, 3, 2, 3)
+    t1 = _cpu_sample(w, projs, ang, sup, mk_centred, model, 1, 1, 1, 1)
+    from oracle import oracle as O
+    O.set_threads(n_cores)
+    total = job_s(tm, "nufft")
     return {
         "value": 3600.0 / total,
         "unit": "recon/hour",
         "cores": n_cores,
         "kind": "port",
-        "sample": (f"oracle (C/OpenMP + numpy/pocketfft) on 2 of {w.num_projs} projections gridded "
-                   f"({t_grid_proj:.2f} s each, full-grid FP32 index scan + FP64 direct DFT), "
-                   f"2 DR iterations on the {n}^3 grid ({t_iter:.2f} s each), 2 of {w.num_projs} trilinear "
-                   f"projections ({t_proj:.2f} s each); extrapolated linearly to {w.num_projs} projections, "
-                   f"{w.iterations} iterations and {n_eval} R-factor evaluations = {total:.0f} s per job"),
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "value_1core": 3600.0 / job_s(t1, "nufft"),
+        "value_direct_dft": 3600.0 / job_s(tm, "dft"),
+        "sample": (f"oracle (C/OpenMP + numpy/pocketfft restatement of the reference CPU path) on {n_cores} threads: "
+                   f"3 of {w.num_projs} projections gridded (full-grid FP32 index scan {tm['scan']:.2f} s, "
+                   f"values by the CPU type-2 NUFFT {tm['nufft']:.2f} s, by the FP64 direct DFT {tm['dft']:.2f} s "
+                   f"per projection), 2 DR iterations on the {n}^3 grid ({tm['iter']:.2f} s each), 3 trilinear "
+                   f"projections ({tm['proj']:.3f} s each); extrapolated linearly to {w.num_projs} projections, "
+                   f"{w.iterations} iterations and {n_eval} R-factor evaluations = {total:.0f} s per job with NUFFT "
+                   f"values ({job_s(tm, 'dft'):.0f} s with the direct DFT); one core: 1 projection, 1 iteration, "
+                   f"1 projector call = {job_s(t1, 'nufft'):.0f} s per job"),
     }
+
+
+def fp64_iteration_check(ctx, p, w, projs, ang, sup, mk_centred, iters):
+    """The timed workload's first `iters` DR iterations with the R-factor monitor
+    at every iteration, on the GPU and in the oracle's FP64 restatement of
+    reconstruct.m:117-185 from the same measuredK: relative error of the
+    returned volume and the largest |errR| and |errK| differences."""
+    from genfire_amd import _lib
+    from oracle import oracle as O
+    import ctypes
+    q = _lib.GenfireParams()
+    ctypes.memmove(ctypes.addressof(q), ctypes.addressof(p), ctypes.sizeof(q))
+    q.numIterations = iters
+    q.monitor_R = 1
+    q.monitorR_loopLength = 1
+    job = _lib.Job(ctx, q, w.dim, w.dim, w.num_projs)
+    job.set_inputs(projs, ang, sup)
+    job.set_measuredK(mk_centred)
+    job.reconstruct()
+    mode = int(p.constraintEnforcementMode)
+    ref = O.reconstruct(mk_centred, mk_centred != 0, sup, w.dim, w.dim, iters, mode=mode, algorithm=w.algorithm,
+                        precision="fp64", monitor_R=True, loop_length=1, projections=projs, angles=ang)
+    rec = job.reconstruction()
+    errR = job.errR()[0]
+    out = {
+        "iterations": iters,
+        "volume_rel_err": float(np.linalg.norm(rec - ref.reconstruction) / np.linalg.norm(ref.reconstruction)),
+        "rfactor_max_abs_delta": float(np.abs(errR - ref.errR).max()),
+        "errK_max_abs_delta": float(np.abs(job.errK() - ref.errK).max()),
+        "reference": "oracle FP64 restatement (numpy/pocketfft), same measuredK",
+    }
+    job.close()
+    return out
 
 
 def rfactor_fp64_delta(ctx, model, projs, ang, idx):
@@ -234,6 +316,17 @@ def serve_mix(ctx, nreq, rate, seed, max_active):
     }
 
 
+def values_route(args, prec):
+    """How the job's gridded values are computed (the library's choice)."""
+    if args.gridding == "fft":
+        return "fft gridding (no per-point values)"
+    forced = os.environ.get("GENFIRE_GRID_KERNEL")
+    if prec == "fp32":
+        return {"mfma": "direct sum on MFMA (FP32)", "valu": "direct sum on VALU (FP32)"}.get(
+            forced, "type-2 NUFFT (FP32, exponential-of-semicircle kernel, width 8)")
+    return f"direct sum on MFMA ({prec})"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,8 +334,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="tilt256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-kernels", type=int, default=1,
-                    help="HIP-event timing of the FFT pass kernels for the roofline line")
     ap.add_argument("--iterations", type=int, default=None,
                     help="override the workload's iteration count (short profiling runs only)")
     ap.add_argument("--precision", choices=["fp32", "split_bf16", "fp16"], default="fp32",
@@ -251,10 +342,16 @@ def main():
                     help="gridding method: DFT (griddingMethod 2, the headline path) or FFT (griddingMethod 1)")
     ap.add_argument("--served-jobs", type=int, default=6,
                     help="requests of the served form on the same workload (0: skip)")
+    ap.add_argument("--served-precision", choices=["fp32", "split_bf16", "fp16"], default="split_bf16",
+                    help="gridding precision of the second served line (the mixed-precision served form)")
     ap.add_argument("--serve-mix", type=int, default=0,
                     help="run only the serving mix with this many requests and print its line")
+    ap.add_argument("--mix-summary", type=int, default=16,
+                    help="requests of the serving-mix summary appended to the line (0: skip)")
     ap.add_argument("--mix-rate", type=float, default=4.0)
     ap.add_argument("--max-active", type=int, default=4)
+    ap.add_argument("--fp64-iterations", type=int, default=3,
+                    help="iterations of the GPU-vs-FP64-oracle reconstruction check (0: skip)")
     ap.add_argument("--fp16-inputs", action="store_true",
                     help="store the projections as FP16 with one max-abs scale per projection")
     ap.add_argument("--kspace-crop", type=int, default=0,
@@ -265,7 +362,7 @@ def main():
                     help="first run a 1 GiB device-to-device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "the library runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this benchmark measures one GPU: --gpus must be 1"}))
         sys.exit(2)
 
     from genfire_amd import _lib, phantom
@@ -308,7 +405,8 @@ def main():
     for i, v in enumerate(w.vectors):
         for j in range(3):
             getattr(p, f"vector{i + 1}")[j] = v[j]
-    prec = {"fp32": _lib.PREC_FP32, "split_bf16": _lib.PREC_SPLIT_BF16, "fp16": _lib.PREC_FP16}[args.precision]
+    precs = {"fp32": _lib.PREC_FP32, "split_bf16": _lib.PREC_SPLIT_BF16, "fp16": _lib.PREC_FP16}
+    prec = precs[args.precision]
     p.precision = prec
     job = _lib.Job(ctx, p, w.dim, w.dim, w.num_projs)
     if args.fp16_inputs:
@@ -321,27 +419,29 @@ def main():
     else:
         job.set_inputs(projs, ang, sup)          # host -> HBM, outside the timed region
 
+    # timed steps: the library's default path (steady-state iterations replay a
+    # captured HIP graph), inputs resident in HBM
     lat = []
-    stats = None
     for s in range(args.warmup + args.steps):
         timed = s >= args.warmup
-        if timed and args.profile_kernels:
-            job.set_profiling(True)
         t = time.perf_counter()
         job.grid()
         job.reconstruct()                    # returns after the job's stream drained
         dt = time.perf_counter() - t
         if timed:
             lat.append(dt)
-            stats = stats or {}
-            for k in range(5):
-                n_, ms_ = job.kernel_stats(k)
-                a, b = stats.get(k, (0, 0.0))
-                stats[k] = (a + n_, b + ms_)
-            job.set_profiling(False)
         log(f"step {s} {'timed' if timed else 'warmup'}: {dt * 1e3:.1f} ms {job.timing()}")
-
     tm = job.timing()
+    replays = job.graph_replays()
+    # per-kernel HIP-event timing in one more, untimed step (profiling launches
+    # every iteration directly)
+    job.set_profiling(True)
+    job.grid()
+    job.reconstruct()
+    stats = {k: job.kernel_stats(k) for k in range(5)}
+    job.set_profiling(False)
+    log(f"profiled step: {job.timing()}")
+
     # DFT gridding: points in the master list; FFT gridding: lattice points of all spectra
     npts = job.num_points() if args.gridding == "dft" else w.num_projs * job.n1 * job.n1
     errR, R1, R2 = job.errR()
@@ -350,20 +450,28 @@ def main():
     ms_step = float(np.mean(lat_ms))
     n = job.n1
     h = n // 2 + 1
+    # dominant kernel: the projector of the R-factor monitor, P (4 Dim^3 + 8 Dim1 Dim2)
+    # algorithmic bytes per launch (every projection reads the model once and its
+    # measured projection once, writing nothing back but R sums)
+    npj, mspj = stats[_lib.KSTAT_PROJECT]
+    proj_bytes = float(w.num_projs) * (4.0 * w.dim ** 3 + 8.0 * w.dim * w.dim)
+    proj_ms = mspj / max(npj, 1)
+    proj_achieved = proj_bytes / (proj_ms * 1e-3) / 1e9 if npj else float("nan")
     # P2 (y-forward pass of the half-spectrum iteration) with the support box
     # pruning: it reads the rows with y and z in the Dim box and writes every ky
     # of the z rows its tiles cover (complex64)
     zrows = box_rows(n, w.dim, 16)
     pass_bytes = 8.0 * h * (w.dim * w.dim + n * zrows)
     nfp, msfp = stats[_lib.KSTAT_FFT_PASS]
-    nit, msit = stats[_lib.KSTAT_ITERATION]
     pass_ms = msfp / max(nfp, 1)
-    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if nfp else float("nan")
+    pass_achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if nfp else float("nan")
+    nit, msit = stats[_lib.KSTAT_ITERATION]
     it_ms = msit / max(nit, 1) if nit else tm["iterate_ms"] / w.iterations
     # algorithmic bytes per iteration with half-spectrum storage:
     # 16 n^3 + 12 N_meas,half + n^3/8, N_meas,half ~ N_meas / 2
     nmeas = int(np.count_nonzero(job.measuredK()))
     b_it = 16.0 * n ** 3 + 12.0 * (nmeas / 2) + n ** 3 / 8.0
+    proj_traffic = PROJ_TRAFFIC.get((w.name, w.three_euler))
 
     out = {
         "metric": METRIC,
@@ -381,27 +489,46 @@ def main():
         "config": {"workload": w.name, "volume": w.dim, "projections": w.num_projs, "projection_size": w.dim,
                    "grid": n, "iterations": w.iterations, "algorithm": w.algorithm,
                    "rfactor_every": w.monitor_every, "gridding": args.gridding, "gridding_precision": args.precision,
+                   "values": values_route(args, args.precision),
                    "cutoff": w.cutoff, "oversampling": w.oversampling,
                    "tilt_series": "three Euler angles" if w.three_euler else "single axis (y)",
-                   "projection_storage": "fp16 + per-projection scale" if args.fp16_inputs else "fp32"},
+                   "projection_storage": "fp16 + per-projection scale" if args.fp16_inputs else "fp32",
+                   "iterations_path": f"HIP graph replay for {replays} of {w.iterations} iterations"},
         "gridded_kpts_per_s": npts / (tm["grid_ms"] * 1e-3),
         "gridded_points": npts,
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "stage_ms": {"grid": tm["grid_ms"], "iterations": tm["iterate_ms"], "rfactor": tm["rfactor_ms"],
-                     "iteration_avg": it_ms},
+                     "iteration_avg": tm["iterate_ms"] / max(w.iterations, 1),
+                     "iteration_avg_profiled": it_ms},
         "rfactor_final": float(errR[-1]) if len(errR) else None,
         "iteration_roofline_frac": (b_it / (it_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(n),
-                     "kernel": f"k_half_y_fwd<{n}> (y-forward pass, box-pruned), {pass_bytes / 1e9:.3f} GB algorithmic "
-                               f"per launch, {pass_ms:.3f} ms mean over {nfp} launches"},
+        "roofline": {"bound": "hbm", "achieved": proj_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": proj_achieved / HBM_PEAK_GBS, "traffic": proj_traffic,
+                     "kernel": f"k_project (R-factor projector, the largest kernel), {proj_bytes / 1e9:.3f} GB "
+                               f"algorithmic per launch (P (4 Dim^3 + 8 Dim^2)), {proj_ms:.3f} ms mean over "
+                               f"{npj} launches"},
+        "roofline_passes": {"bound": "hbm", "achieved": pass_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": pass_achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(n),
+                            "kernel": f"k_half_y_fwd<{n}> (y-forward pass, box-pruned), {pass_bytes / 1e9:.3f} GB "
+                                      f"algorithmic per launch, {pass_ms:.3f} ms mean over {nfp} launches"},
     }
     if args.served_jobs > 0:
-        out["served"] = serve_tilt(ctx, p, w, projs, ang, sup, args.served_jobs, _lib.PREC_FP32,
-                                   args.max_active)
-        out["served"]["precision"] = "fp32"
-        out["served"]["vs_this_line"] = out["served"]["recon_per_hour"] / out["value"]
+        served = []
+        for sp in ("fp32", args.served_precision):
+            r = serve_tilt(ctx, p, w, projs, ang, sup, args.served_jobs, precs[sp], args.max_active)
+            r["precision"] = sp
+            r["vs_this_line"] = r["recon_per_hour"] / out["value"]
+            served.append(r)
+        out["served"] = served[0]
+        out["served_mixed"] = served[1]
+        if served[0]["rfactor_final"] is not None and served[1]["rfactor_final"] is not None:
+            out["served_mixed"]["rfactor_delta_vs_fp32"] = abs(served[1]["rfactor_final"] - served[0]["rfactor_final"])
         p.precision = prec
+    if args.mix_summary > 0:
+        m = serve_mix(ctx, args.mix_summary, args.mix_rate, 7, args.max_active)
+        out["serving_mix"] = {k: m[k] for k in ("value", "unit", "p50_ms", "p99_ms", "mean_ms", "requests",
+                                                "arrival_rate_per_s", "completed", "failed", "max_active_seen",
+                                                "batched_launches", "batched_jobs", "sizes")}
     if prec != _lib.PREC_FP32:
         # the same job with FP32 gridding, untimed: volume and R-factor deltas
         p.precision = _lib.PREC_FP32
@@ -420,12 +547,14 @@ def main():
             "volume_rel_err": float(np.linalg.norm(rec - rec32) / np.linalg.norm(rec32)),
             "rfactor_delta": float(abs(errR[-1] - errR32[-1])) if len(errR) else None,
         }
+        p.precision = prec
         del ref
     if not args.no_cpu_baseline:
         ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log("cpu baseline (oracle sample)")
         try:
-            out["cpu_baseline"] = cpu_baseline(w, projs, ang, sup, job.measuredK(), rec, ncores)
+            mk = job.measuredK()
+            out["cpu_baseline"] = cpu_baseline(w, projs, ang, sup, mk, rec, ncores)
             d, r = rfactor_fp64_delta(ctx, rec, projs, ang, [0, w.num_projs // 2, w.num_projs - 1])
             out["rfactor_delta_vs_fp64"] = d
             out["rfactor_posthoc_sample"] = r
@@ -434,8 +563,13 @@ def main():
                     job, projs, [0, w.num_projs // 2, w.num_projs - 1], crop=args.kspace_crop)
             if args.kspace_crop:
                 out["grid_value_error_region"] = f"central {args.kspace_crop}^3 of k-space"
+            if args.fp64_iterations > 0:
+                log(f"FP64 oracle check: {args.fp64_iterations} iterations")
+                out["fp64_iteration_check"] = fp64_iteration_check(ctx, p, w, projs, ang, sup, mk,
+                                                                   args.fp64_iterations)
         except Exception as e:  # the baseline is reported, never the target
-            out["cpu_baseline"] = {"error": str(e)}
+            out["cpu_baseline"] = out.get("cpu_baseline") or {"error": str(e)}
+            out["fp64_check_error"] = str(e)
     print(json.dumps(out))
 
 

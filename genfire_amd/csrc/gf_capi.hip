@@ -366,41 +366,70 @@ int genfire_job_kernel_stats(const genfire_job* j, int32_t k, int64_t* launches,
 
 namespace {
 
+// The model's forms the projections read (project_forms), on the device.
+struct ModelForms {
+    DevBuf vy, zy, zx;
+    void build(const ProjForms& f, const float* d_vol, int d1, int d2, int d3, hipStream_t st)
+    {
+        if (f.four_ray) {
+            vy.alloc(sizeof(float) * (size_t)d1 * d2 * d3);
+            volume_yfast(d_vol, vy.as<float>(), d1, d2, d3, st);
+        }
+        if (f.zy) {
+            zy.alloc(sizeof(float) * zdup_floats(d1, d2, d3));
+            volume_zdup(d_vol, d1, d2, d3, zy.as<float>(), true, st);
+        }
+        if (f.zx) {
+            zx.alloc(sizeof(float) * zdup_floats(d1, d2, d3));
+            volume_zdup(d_vol, d1, d2, d3, zx.as<float>(), false, st);
+        }
+    }
+    void set(const ProjForms& f, ProjArgs* pa) const
+    {
+        pa->four_ray = f.four_ray ? 1 : 0;
+        pa->vol_y = vy.as<float>();
+        pa->zy = zy.as<float>();
+        pa->zx = zx.as<float>();
+    }
+};
+
+// A stream that is destroyed on every way out of the call.
+struct OwnStream {
+    hipStream_t s = nullptr;
+    OwnStream() { GF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~OwnStream()
+    {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+    OwnStream(const OwnStream&) = delete;
+    OwnStream& operator=(const OwnStream&) = delete;
+};
+
 void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, const double ang[3],
                  const double* vectors, float* proj)
 {
     use_device(ctx);
     GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: volume dimensions must be >= 2");
-    hipStream_t st;
-    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    DevBuf v, vy, vz, vzx, r, o;
+    GF_CHECK((int64_t)d1 * d2 * d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "GENFIRE: volume above 2^31 voxels");
+    DevBuf v, r, o;
+    ModelForms mf;
+    OwnStream os;   // declared last: synchronised before the buffers are freed
+    hipStream_t st = os.s;
     const size_t nv = (size_t)d1 * d2 * d3;
     v.alloc(sizeof(float) * nv);
     o.alloc(sizeof(float) * (size_t)d1 * d2);
     bool any_y = false;
     std::vector<float> Rf = proj_matrices(ang, 1, &any_y, vectors);
+    const ProjForms f = project_forms(Rf.data(), 1, d2);
     r.alloc(sizeof(float) * Rf.size());
     GF_HIP(hipMemcpyAsync(v.p, vol, sizeof(float) * nv, hipMemcpyHostToDevice, st));
     GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
-    if (any_y) {   // y-fastest copy and its z-pair copies (the projector's two-gather form)
-        vy.alloc(sizeof(float) * nv);
-        volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
-        vz.alloc(sizeof(float) * 2 * zpair_floats(d1, d2, d3));
-        volume_zpairs(vy.as<float>(), vz.as<float>(), d1, d2, d3, st);
-    }
+    mf.build(f, v.as<float>(), d1, d2, d3, st);
     ProjArgs pa{};
-    pa.vol = v.as<float>();
-    pa.vol_y = vy.as<float>();
-    pa.vol_z = vz.as<float>();
-    pa.zodd = zpair_floats(d1, d2, d3);
-    // z-pair copies of the x-fastest volume for the x-fastest projections
-    bool any_x = false;
-    for (size_t q = 9; q < Rf.size(); q += kRStride) any_x = any_x || Rf[q] == 0.0f;
-    if (any_x) {
-        vzx.alloc(sizeof(float) * 2 * zpair_floats(d1, d2, d3));
-        volume_zpairs(v.as<float>(), vzx.as<float>(), d1, d2, d3, st);
-    }
-    pa.vol_zx = vzx.as<float>();
+    mf.set(f, &pa);
     pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
     pa.R = r.as<float>();
     pa.P = 1;
@@ -409,7 +438,6 @@ void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, con
     project(pa, st);
     GF_HIP(hipMemcpyAsync(proj, o.p, o.bytes, hipMemcpyDeviceToHost, st));
     GF_HIP(hipStreamSynchronize(st));
-    GF_HIP(hipStreamDestroy(st));
 }
 
 void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, const float* projections, int p1,
@@ -417,42 +445,32 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
                  float* simu)
 {
     use_device(ctx);
+    GF_CHECK(P >= 1, GENFIRE_ERR_ARG, "GENFIRE: no projections");
+    GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: model dimensions must be >= 2");
+    GF_CHECK(p1 >= 1 && p2 >= 1, GENFIRE_ERR_ARG, "GENFIRE: empty projections");
     GF_CHECK(p1 <= d1 && p2 <= d2, GENFIRE_ERR_ARG, "GENFIRE: projection larger than the model");
-    hipStream_t st;
-    GF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    DevBuf v, vy, vz, vzx, r, o, m, s, s4;
+    GF_CHECK((int64_t)d1 * d2 * d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "GENFIRE: model above 2^31 voxels");
+    DevBuf v, r, o, m, s, s4;
+    ModelForms mf;
+    OwnStream os;   // declared last: synchronised before the buffers are freed
+    hipStream_t st = os.s;
     const size_t nv = (size_t)d1 * d2 * d3, np = (size_t)p1 * p2 * P;
     v.alloc(sizeof(float) * nv);
     m.alloc(sizeof(float) * np);
     o.alloc(sizeof(float) * np);
     bool any_y = false;
     std::vector<float> Rf = proj_matrices(angles, P, &any_y, vectors);
+    const ProjForms f = project_forms(Rf.data(), P, d2);
     r.alloc(sizeof(float) * Rf.size());
     GF_HIP(hipMemcpyAsync(v.p, model, sizeof(float) * nv, hipMemcpyHostToDevice, st));
     GF_HIP(hipMemcpyAsync(m.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, st));
     GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
-    if (any_y) {   // y-fastest copy and its z-pair copies (the projector's two-gather form)
-        vy.alloc(sizeof(float) * nv);
-        volume_yfast(v.as<float>(), vy.as<float>(), d1, d2, d3, st);
-        vz.alloc(sizeof(float) * 2 * zpair_floats(d1, d2, d3));
-        volume_zpairs(vy.as<float>(), vz.as<float>(), d1, d2, d3, st);
-    }
+    mf.build(f, v.as<float>(), d1, d2, d3, st);
     const int nblk = project_blocks(p1, p2);
     s.alloc(sizeof(double) * (size_t)P * 4 * nblk);
     s4.alloc(sizeof(double) * (size_t)P * 4);
     ProjArgs pa{};
-    pa.vol = v.as<float>();
-    pa.vol_y = vy.as<float>();
-    pa.vol_z = vz.as<float>();
-    pa.zodd = zpair_floats(d1, d2, d3);
-    // z-pair copies of the x-fastest volume for the x-fastest projections
-    bool any_x = false;
-    for (size_t q = 9; q < Rf.size(); q += kRStride) any_x = any_x || Rf[q] == 0.0f;
-    if (any_x) {
-        vzx.alloc(sizeof(float) * 2 * zpair_floats(d1, d2, d3));
-        volume_zpairs(v.as<float>(), vzx.as<float>(), d1, d2, d3, st);
-    }
-    pa.vol_zx = vzx.as<float>();
+    mf.set(f, &pa);
     pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
     pa.R = r.as<float>();
     pa.P = P;
@@ -467,7 +485,6 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
     GF_HIP(hipMemcpyAsync(h.data(), s4.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
     if (simu) GF_HIP(hipMemcpyAsync(simu, o.p, o.bytes, hipMemcpyDeviceToHost, st));
     GF_HIP(hipStreamSynchronize(st));
-    GF_HIP(hipStreamDestroy(st));
     double mean = 0.0;
     for (int p = 0; p < P; ++p) {
         const double rr = h[4 * p] / h[4 * p + 2];

@@ -253,11 +253,10 @@ void hcsr_den(const float2* d_mag, const uint32_t* d_meta, int64_t n, double* d_
 // 2 when a wave covers an 8 x 8 pixel patch.
 constexpr int kRStride = 12;
 struct ProjArgs {
-    const float* vol;  // d1 x d2 x d3 column-major (x fastest)
-    const float* vol_y; // the same volume y-fastest (index y + d2*(x + d1*z)), nullable when unused
-    const float* vol_z; // z-pair copies of vol_y (volume_zpairs), nullable: then vol_y alone serves
-    const float* vol_zx; // z-pair copies of vol (x fastest), nullable: then vol alone serves
-    int64_t zodd;       // offset of the odd-pair copy in vol_z (floats)
+    const float* vol_y; // y-fastest model (index y + d2*(x + d1*z)) for the four-ray form, else nullable
+    const float* zy;    // z-duplicated y-fastest model (crop_zdup / volume_zdup), nullable when unused
+    const float* zx;    // z-duplicated x-fastest model, nullable when unused
+    int four_ray;       // every projection is an exact tilt about y and d2 % 4 == 0: read vol_y only
     int d1, d2, d3;
     const float* R;    // P x kRStride, single
     int P;
@@ -267,17 +266,25 @@ struct ProjArgs {
     double* sums;      // P x 4 x nblk partials: sum|b-m|, sum (b-m)^2, sum|m|, sum m^2
     int post_hoc;      // 1: sum ||b|-|m|| instead of |b-m| (Tian_calc_Rfactor_realspace)
 };
+// The model forms a set of projector matrices (host copy, P x kRStride) reads.
+struct ProjForms {
+    bool four_ray = false;   // vol_y only
+    bool zy = false;         // the z-duplicated y-fastest form
+    bool zx = false;         // the z-duplicated x-fastest form
+};
+ProjForms project_forms(const float* Rm, int P, int d2);
 int project_blocks(int m1, int m2);
 // sets R[9] (y-fastest forms) and R[11] (lanes along pixel j) of a projection
 void project_orientation(float* R);
 bool project_y_exact(const float* R);
 void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipStream_t st);
-int64_t zpair_floats(int d1, int d2, int d3);   // floats of one z-pair copy
-void volume_zpairs(const float* d_vol_y, float* d_dst, int d1, int d2, int d3, hipStream_t st);
-// the z-pair copies of the centred d1 x d2 x d3 crop of an FFT-order n1 x n2 x n3 volume
-void crop_zpairs(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st);
-// the same pairs over the x-fastest crop (volume_zpairs of an x-fastest volume gives them too)
-void crop_zpairs_x(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st);
+// floats of one z-duplicated form of a d1 x d2 x d3 model
+int64_t zdup_floats(int d1, int d2, int d3);
+// the z-duplicated form of a plain column-major volume, or of the centred
+// d1 x d2 x d3 crop of an FFT-order n1 x n2 x n3 volume (fftshift then crop)
+void volume_zdup(const float* d_vol, int d1, int d2, int d3, float* d_dst, bool yfast, hipStream_t st);
+void crop_zdup(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, bool yfast,
+               hipStream_t st);
 void project(const ProjArgs& a, hipStream_t st);
 void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st);
 

@@ -116,16 +116,14 @@ struct genfire_job {
     gf::DevBuf x, K, S, best, pfbuf, err_part, errK, errctr, den_tmp, den, recon;
     gf::DevBuf csr_cnt, csr_off, csr_coff, csr_meta, csr_val, csr_mag, csr_d, csr_mate;
     int64_t nmeas = 0;
-    gf::DevBuf vol, vol_y, vol_z, vol_zx, Rmat, rsums, rout;
+    gf::DevBuf vol, vol_y, zy, zx, Rmat, rsums, rout;
     // Rfree: withheld voxels per half pencil, their terms, shell order and denominators
     gf::DevBuf rf_off, rf_meta, rf_val, rf_terms, rf_order, rf_shell_off, rf_den, Rfree;
     int rf_bins = 0;
     std::vector<float> Rfree_h;
     std::vector<gf::PointTile> tiles_h;  // host source of tiles (outlives its asynchronous copy)
     std::vector<float> nu_inv_h;         // host source of nu_inv, the NUFFT deconvolution factors
-    bool rmat_any_y = false;      // some projection reads a y-fastest form of the model
-    bool rmat_any_x = false;      // some projection reads the x-fastest model
-    bool rmat_any_exact = false;  // some projection maps y exactly onto y (the y-fastest copy itself)
+    gf::ProjForms rforms;         // the model forms the monitor's projections read
     std::vector<float> errK_h, errR_h;
     std::vector<double> Rarr_h, Rarr2_h;
     bool reconstructed = false;

@@ -409,22 +409,21 @@ void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
     const int P = j->P;
     EventTimer& tm = *j->rs.rf[slot];
     GF_HIP(hipEventRecord(tm.a, st));
-    // the model's forms the projections read: x-fastest, y-fastest (exact-y
-    // tilts) and the z-pair copies (every other y-fastest projection)
-    if (j->rmat_any_exact) {
+    // the model's forms the projections read: the y-fastest crop (every
+    // projection an exact tilt about y), else the z-duplicated forms
+    const ProjForms& f = j->rforms;
+    if (f.four_ray) {
         crop_centre(d_model_fft, 0, j->n1, j->n2, j->n1, j->vol.as<float>(), j->dim1, j->dim2, j->dim1, st);
         volume_yfast(j->vol.as<float>(), j->vol_y.as<float>(), j->dim1, j->dim2, j->dim1, st);
     }
-    if (j->rmat_any_y) crop_zpairs(d_model_fft, j->n1, j->n2, j->n1, j->vol_z.as<float>(), j->dim1, j->dim2, j->dim1, st);
-    if (j->rmat_any_x)
-        crop_zpairs_x(d_model_fft, j->n1, j->n2, j->n1, j->vol_zx.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    if (f.zy) crop_zdup(d_model_fft, j->n1, j->n2, j->n1, j->zy.as<float>(), j->dim1, j->dim2, j->dim1, true, st);
+    if (f.zx) crop_zdup(d_model_fft, j->n1, j->n2, j->n1, j->zx.as<float>(), j->dim1, j->dim2, j->dim1, false, st);
     const int nblk = project_blocks(j->dim1, j->dim2);
     ProjArgs pa{};
-    pa.vol = nullptr;   // every x-fastest projection reads vol_zx
-    pa.vol_y = j->rmat_any_exact ? j->vol_y.as<float>() : nullptr;
-    pa.vol_z = j->rmat_any_y ? j->vol_z.as<float>() : nullptr;
-    pa.vol_zx = j->rmat_any_x ? j->vol_zx.as<float>() : nullptr;
-    pa.zodd = zpair_floats(j->dim1, j->dim2, j->dim1);
+    pa.four_ray = f.four_ray ? 1 : 0;
+    pa.vol_y = f.four_ray ? j->vol_y.as<float>() : nullptr;
+    pa.zy = f.zy ? j->zy.as<float>() : nullptr;
+    pa.zx = f.zx ? j->zx.as<float>() : nullptr;
     pa.d1 = j->dim1; pa.d2 = j->dim2; pa.d3 = j->dim1;
     pa.R = j->Rmat.as<float>();
     pa.P = P;
@@ -705,20 +704,18 @@ void recon_begin(genfire_job* j)
         r.rf.back()->init();
     }
     if (r.nR > 0) {
-        std::vector<float> Rm = proj_matrices(j->angles.data(), j->P, &j->rmat_any_y);
-        j->rmat_any_x = j->rmat_any_exact = false;
-        for (int p = 0; p < j->P; ++p) {
-            j->rmat_any_x = j->rmat_any_x || Rm[(size_t)kRStride * p + 9] == 0.0f;
-            j->rmat_any_exact = j->rmat_any_exact || Rm[(size_t)kRStride * p + 10] != 0.0f;
-        }
+        bool any_y = false;
+        std::vector<float> Rm = proj_matrices(j->angles.data(), j->P, &any_y);
+        j->rforms = project_forms(Rm.data(), j->P, j->dim2);
         j->Rmat.alloc(sizeof(float) * Rm.size());
         GF_HIP(hipMemcpyAsync(j->Rmat.p, Rm.data(), sizeof(float) * Rm.size(), hipMemcpyHostToDevice, st));
-        if (j->rmat_any_exact) {
+        const ProjForms& f = j->rforms;
+        if (f.four_ray) {
             j->vol.alloc(sizeof(float) * box_size(j));
             j->vol_y.alloc(sizeof(float) * box_size(j));
         }
-        if (j->rmat_any_y) j->vol_z.alloc(sizeof(float) * 2 * zpair_floats(j->dim1, j->dim2, j->dim1));
-        if (j->rmat_any_x) j->vol_zx.alloc(sizeof(float) * 2 * zpair_floats(j->dim1, j->dim2, j->dim1));
+        if (f.zy) j->zy.alloc(sizeof(float) * zdup_floats(j->dim1, j->dim2, j->dim1));
+        if (f.zx) j->zx.alloc(sizeof(float) * zdup_floats(j->dim1, j->dim2, j->dim1));
         j->rsums.alloc(sizeof(double) * (size_t)j->P * 4 * project_blocks(j->dim1, j->dim2));
         j->rout.alloc(sizeof(double) * (size_t)r.nR * j->P * 4);
         GF_HIP(hipStreamSynchronize(st));
@@ -1147,7 +1144,7 @@ int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P)
     const int64_t spectrum = 8 * (half ? std::max<int64_t>(2 * hn, nf) : nf);
     const int64_t meas = std::min<int64_t>(nf, 2 * pts);
     const int64_t iter = nf * (4 + 1 + 4 + 4 + 8) + spectrum + meas * 28 + n1 * n2 * 12 + box * 8 +
-                         8 * zpair_floats(dim1, dim2, dim1) +   // the projector's two z-pair copies
+                         8 * zdup_floats(dim1, dim2, dim1) +    // the projector's two z-duplicated forms
                          (int64_t)P * 4 * 8 * (1 + p.numIterations / std::max(1, p.monitorR_loopLength));
     return inputs + std::max(grid, iter) + (64ll << 20);
 }

@@ -6,15 +6,35 @@
 // X = i - floor(d1/2) (..RealSpaceinterp.m:18-20) and trilinear interpolation with
 // extrapolation value 0: a sample is zero unless every coordinate lies in the
 // closed interval [0, d-1] of its axis.  One lane marches one ray (one output
-// pixel) along z; a 16x16 pixel tile keeps neighbouring rays on neighbouring
-// voxels so the eight-voxel gathers hit L1/L2.  The R-factor sums of each tile
-// are written as per-tile partials and reduced in a fixed order.
+// pixel) along z, summing its samples in ascending k.  The R-factor sums of
+// each tile are written as per-tile partials and reduced in a fixed order.
+//
+// The model is read in a z-duplicated form: element (x, y, z) of 8 bytes holds
+// the voxel (x, y, z) and its neighbour (x, y, z + 1), laid out y-fastest
+// (index y + (d2+1) (x + (d1+1) z)) or x-fastest (x + (d1+1) (y + (d2+1) z)),
+// with a zero row past the last x and y and a zero neighbour past the last z.
+// A sample's eight voxels are then two 16-byte loads: the four (y, y+1) x
+// (z, z+1) values at x = a0 and at a0 + 1 (y-fastest form), or the four
+// (x, x+1) x (z, z+1) values at y = b0 and b0 + 1 (x-fastest form), with no
+// clamp at the top face: a coordinate equal to d - 1 takes cell d - 1 with
+// weight 0 on the zero row.
+#include <algorithm>
+
 #include "gf_grid.h"
 
 namespace gf {
 
 namespace {
 
+constexpr int kXcd = 8;
+constexpr int kProjGroup = 6;       // four-ray form: neighbouring projections whose tiles interleave
+constexpr int kRayGroup = 12;       // the same for one ray per lane
+constexpr int kLanes = 64;          // pixels of one wave along the fast pixel axis
+constexpr int kRows = 4;            // waves per workgroup along the other pixel axis
+constexpr float kRowAlign = 0.9f;   // lane rows when a pairing's |R| reaches this, else patches
+
+// Two (voxel, z + 1 neighbour) pairs in one 16-byte access (8-byte aligned).
+typedef float quad_t __attribute__((ext_vector_type(4), aligned(8)));
 
 __device__ __forceinline__ bool lin_axis(float f, int n, int* i0, float* t)
 {
@@ -26,36 +46,6 @@ __device__ __forceinline__ bool lin_axis(float f, int n, int* i0, float* t)
     return true;
 }
 
-// Unaligned pair load: two adjacent floats of the volume in one 8-byte access.
-typedef float pair_t __attribute__((ext_vector_type(2), aligned(4)));
-// Two pairs of the z-pair copies in one 16-byte access (8-byte aligned).
-typedef float quad_t __attribute__((ext_vector_type(4), aligned(8)));
-
-// With more than 8 projections the workgroups of one projection run on one XCD:
-// the hardware deals workgroups to the 8 XCDs round-robin, so physical block b
-// goes to XCD b % 8.  XCD x takes the contiguous block of per = ceil(P / 8)
-// projections starting at x * per, in groups of G neighbouring projections whose
-// tiles are interleaved (slot = (g * tiles + tile) * G + m for member m of the
-// XCD's group g).  Neighbouring projections of a tilt
-// series send nearly the same rays through the model, so the G workgroups of
-// one tile read the same voxels at about the same time and share the XCD's L2
-// fills: each projection reads the whole model (twice the voxels in the z-pair
-// copies), which would otherwise stream from HBM or the MALL once per projection.
-// With at most 8 projections (per = 1) every XCD instead takes a contiguous
-// eighth of each projection's tiles, so a single projection uses the whole chip.
-constexpr int kXcd = 8;
-constexpr int kProjGroup = 6;   // G, tilt256's 91 projections: 12 per XCD, two groups
-constexpr int kLanes = 64;   // pixels of one wave along the fast pixel axis
-constexpr int kRows = 4;     // waves per workgroup along the other pixel axis
-constexpr float kRowAlign = 0.9f;   // lane rows when a pairing's |R| reaches this, else patches
-
-// A wave's 64 lanes are 64 neighbouring pixels along the pixel axis (i or j,
-// R[11]) whose direction in the volume is closest to the fast axis of one of the
-// model's forms (x- or y-fastest, R[9]): the largest of |R00|, |R11|, |R10|,
-// |R01|.  For tilts about y the pixel-y direction
-// is the volume's y axis, so each gather of a wave reads a few contiguous runs
-// instead of one cache line per lane.
-//
 // Rotations that map the volume's y axis onto itself (R[10] set: tilts about y)
 // take a second form when d2 is a multiple of 4: every ray of a pixel column i
 // then samples the same (x, z) cell with the same weights at each step, its y
@@ -153,11 +143,21 @@ __device__ __forceinline__ int ytilt_tiles(const ProjArgs& a)
 __device__ __forceinline__ void tile_sums(const ProjArgs& a, int p, int tile, int tiles, double q0, double q1,
                                           double q2, double q3);
 
-__global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles, int per, int group)
+// Every projection maps y exactly onto y (the four-ray form).  With more than 8
+// projections the workgroups of one projection run on one XCD: the hardware
+// deals workgroups to the 8 XCDs round-robin, so physical block b goes to XCD
+// b % 8.  XCD x takes the contiguous block of per = ceil(P / 8) projections
+// starting at x * per, in groups of G neighbouring projections whose tiles are
+// interleaved (slot = (g * tiles + tile) * G + m for member m of the XCD's
+// group g), so that the G workgroups of one tile read nearly the same voxels
+// at about the same time and share the XCD's L2 fills.  With at most 8
+// projections (per = 1) every XCD instead takes a contiguous eighth of each
+// projection's tiles.
+__global__ __launch_bounds__(kLanes* kRows) void k_project_ytilt(ProjArgs a, int tiles, int per, int group)
 {
     const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
     int p, tile;
-    if (per == 1) {   // at most 8 projections: XCD x takes the x-th eighth of every projection's tiles
+    if (per == 1) {
         const int tc = (tiles + kXcd - 1) / kXcd;
         p = slot / tc;
         tile = xcd * tc + slot % tc;
@@ -168,141 +168,189 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
         tile = (slot / group) % tiles;
         if (k >= per || p >= a.P) return;
     }
-    const float* R = a.R + kRStride * p;
-    if (R[10] != 0.0f && a.d2 % 4 == 0 && ytilt_tiles(a) <= tiles) {
-        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
-        if (tile < ytilt_tiles(a)) project_ytilt(a, R, p, tile, q0, q1, q2, q3);
-        tile_sums(a, p, tile, tiles, q0, q1, q2, q3);
-        return;
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+    if (tile < ytilt_tiles(a)) project_ytilt(a, a.R + kRStride * p, p, tile, q0, q1, q2, q3);
+    tile_sums(a, p, tile, tiles, q0, q1, q2, q3);
+}
+
+// ---------------------------------------------------------------------------
+// One ray per lane over the z-duplicated model (every other rotation)
+// ---------------------------------------------------------------------------
+
+struct Ray {
+    float bx, by, bz;   // sample coordinates at Z = 0
+    float rx, ry, rz;   // their step per unit of Z (the third column of R)
+};
+
+// The sample coordinates of step k are fma(r, Z, b) with Z = k - floor(d3/2),
+// the same expression in the range search and in the march.
+__device__ __forceinline__ bool ray_inside(const Ray& r, float Z, float hx, float hy, float hz)
+{
+    const float fx = __builtin_fmaf(r.rx, Z, r.bx);
+    const float fy = __builtin_fmaf(r.ry, Z, r.by);
+    const float fz = __builtin_fmaf(r.rz, Z, r.bz);
+    return fx >= 0.0f && fx <= hx && fy >= 0.0f && fy <= hy && fz >= 0.0f && fz <= hz;
+}
+
+// Steps [k0, k1] whose sample lies inside the box.  Each coordinate is a
+// correctly rounded fma of a linear function of Z, hence monotonic in k, so the
+// inside steps form one interval: estimate it from the three slab intervals,
+// then move each end to the exact boundary with the inside test itself.
+__device__ __forceinline__ void ray_range(const Ray& r, int d1, int d2, int d3, int* k0, int* k1)
+{
+    const int c3 = d3 / 2;
+    const float hx = (float)(d1 - 1), hy = (float)(d2 - 1), hz = (float)(d3 - 1);
+    float klo = 0.0f, khi = hz;
+    const float bb[3] = {r.bx, r.by, r.bz}, rr[3] = {r.rx, r.ry, r.rz}, hi[3] = {hx, hy, hz};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (rr[q] != 0.0f) {
+            const float t0 = (0.0f - bb[q]) / rr[q] + (float)c3, t1 = (hi[q] - bb[q]) / rr[q] + (float)c3;
+            klo = fmaxf(klo, fminf(t0, t1) - 1.0f);
+            khi = fminf(khi, fmaxf(t0, t1) + 1.0f);
+        } else if (!(bb[q] >= 0.0f && bb[q] <= hi[q])) {
+            khi = -1.0f;
+        }
     }
-    const bool yfast = R[9] != 0.0f;     // the model's y-fastest forms
-    int i, j;
-    if (R[11] == 2.0f) {                  // 8 x 8 pixel patch per wave, 16 x 16 per workgroup
-        const int tp1 = (a.m1 + 15) / 16, w = (int)threadIdx.x / kLanes, l = (int)threadIdx.x % kLanes;
-        i = (tile % tp1) * 16 + (w & 1) * 8 + (l & 7);
-        j = (tile / tp1) * 16 + (w >> 1) * 8 + (l >> 3);
-    } else if (R[11] != 0.0f) {           // lanes along pixel j
+    int a = (int)fminf((float)d3, fmaxf(0.0f, ceilf(klo)));
+    int b = (int)fmaxf(-1.0f, fminf((float)(d3 - 1), floorf(khi)));
+    if (a <= b) {
+        while (a > 0 && ray_inside(r, (float)(a - 1 - c3), hx, hy, hz)) --a;
+        while (b < d3 - 1 && ray_inside(r, (float)(b + 1 - c3), hx, hy, hz)) ++b;
+        while (a <= b && !ray_inside(r, (float)(a - c3), hx, hy, hz)) ++a;
+        while (b >= a && !ray_inside(r, (float)(b - c3), hx, hy, hz)) --b;
+    }
+    *k0 = a;
+    *k1 = b;
+}
+
+// Pixel of this lane in tile `tile` of a projection whose lane arrangement is
+// R[11] (1: 64 lanes along j, 0: along i, 2: 8 x 8 patches, 16 x 16 per tile).
+__device__ __forceinline__ void ray_pixel(const ProjArgs& a, float form, int tile, int* i, int* j)
+{
+    const int w = (int)threadIdx.x / kLanes, l = (int)threadIdx.x % kLanes;
+    if (form == 2.0f) {
+        const int tp1 = (a.m1 + 15) / 16;
+        *i = (tile % tp1) * 16 + (w & 1) * 8 + (l & 7);
+        *j = (tile / tp1) * 16 + (w >> 1) * 8 + (l >> 3);
+    } else if (form != 0.0f) {
         const int tiles2 = (a.m2 + kLanes - 1) / kLanes;
-        i = (tile / tiles2) * kRows + (int)(threadIdx.x / kLanes);
-        j = (tile % tiles2) * kLanes + (int)(threadIdx.x % kLanes);
+        *i = (tile / tiles2) * kRows + w;
+        *j = (tile % tiles2) * kLanes + l;
     } else {
         const int tiles1 = (a.m1 + kLanes - 1) / kLanes;
-        i = (tile % tiles1) * kLanes + (int)(threadIdx.x % kLanes);
-        j = (tile / tiles1) * kRows + (int)(threadIdx.x / kLanes);
+        *i = (tile % tiles1) * kLanes + l;
+        *j = (tile / tiles1) * kRows + w;
     }
-    const bool valid = i < a.m1 && j < a.m2;
-    const int d1 = a.d1, d2 = a.d2, d3 = a.d3;
-    // pixel (i,j) of the measured window sits at (i + s1, j + s2) of the d1 x d2
-    // back projection (My_stripzero of the full projection, Tian...:16)
-    const int s1 = d1 / 2 - a.m1 / 2, s2 = d2 / 2 - a.m2 / 2;
-    const float X = (float)(i + s1 - d1 / 2), Y = (float)(j + s2 - d2 / 2);
-    const float c1 = (float)(d1 / 2), c2 = (float)(d2 / 2), c3 = (float)(d3 / 2);
-    const float bx = R[0] * X + R[3] * Y + c1;
-    const float by = R[1] * X + R[4] * Y + c2;
-    const float bz = R[2] * X + R[5] * Y + c3;
-    const float rx = R[6], ry = R[7], rz = R[8];
+}
+
+// Pixel (i, j) of the measured window sits at (i + s1, j + s2) of the d1 x d2
+// back projection (My_stripzero of the full projection, Tian...:16).
+__device__ __forceinline__ Ray make_ray(const ProjArgs& a, const float* R, int i, int j)
+{
+    const int s1 = a.d1 / 2 - a.m1 / 2, s2 = a.d2 / 2 - a.m2 / 2;
+    const float X = (float)(i + s1 - a.d1 / 2), Y = (float)(j + s2 - a.d2 / 2);
+    Ray r;
+    r.bx = __builtin_fmaf(R[0], X, __builtin_fmaf(R[3], Y, (float)(a.d1 / 2)));
+    r.by = __builtin_fmaf(R[1], X, __builtin_fmaf(R[4], Y, (float)(a.d2 / 2)));
+    r.bz = __builtin_fmaf(R[2], X, __builtin_fmaf(R[5], Y, (float)(a.d3 / 2)));
+    r.rx = R[6];
+    r.ry = R[7];
+    r.rz = R[8];
+    return r;
+}
+
+// Byte offset of cell (a0, b0, e0)'s element: index c + s1 r + s12 e0 with
+// (c, r) = (b0, a0), s1 = d2 + 1 in the y-fastest form and (a0, b0), d1 + 1 in
+// the x-fastest one, s12 = (d1 + 1)(d2 + 1); in 32 bits (24-bit products)
+// unless the form exceeds 4 GiB (WIDE).
+template <bool WIDE>
+__device__ __forceinline__ auto zdup_offset(int c, int r, int e0, uint32_t s1, uint32_t s12)
+{
+    if constexpr (WIDE) {
+        return ((uint64_t)s12 * (uint64_t)e0 + (uint64_t)s1 * (uint64_t)r + (uint64_t)c) * 8;
+    } else {
+        const uint32_t i = __umul24(s12, (uint32_t)e0) + (__umul24(s1, (uint32_t)r) + (uint32_t)c);
+        return i << 3;
+    }
+}
+
+// March the ray from Z to Z1 (Z = k - floor(d3/2) counts whole steps, exact in
+// FP32), summing its samples in ascending k.  v0 is the form's base, v1 the
+// next x row (YF) or y row.
+template <bool YF, bool WIDE>
+__device__ __forceinline__ float march(const char* __restrict__ v0, const char* __restrict__ v1, uint32_t s1,
+                                       uint32_t s12, const Ray& r, float Z, float Z1)
+{
     float acc = 0.0f;
-    if (valid) {
-        // steps whose sample can lie inside the box, from the three slab
-        // intervals widened by one step; the exact test stays per step
-        float klo = 0.0f, khi = (float)(d3 - 1);
-        const float bb[3] = {bx, by, bz}, rr[3] = {rx, ry, rz};
-        const float hi[3] = {(float)(d1 - 1), (float)(d2 - 1), (float)(d3 - 1)};
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            if (rr[q] != 0.0f) {
-                const float t0 = (0.0f - bb[q]) / rr[q] + c3, t1 = (hi[q] - bb[q]) / rr[q] + c3;
-                klo = fmaxf(klo, fminf(t0, t1) - 1.0f);
-                khi = fminf(khi, fmaxf(t0, t1) + 1.0f);
-            } else if (!(bb[q] >= 0.0f && bb[q] <= hi[q])) {
-                khi = -1.0f;
-            }
+    for (; Z <= Z1; Z += 1.0f) {
+        const float fx = __builtin_fmaf(r.rx, Z, r.bx), fy = __builtin_fmaf(r.ry, Z, r.by);
+        const float fz = __builtin_fmaf(r.rz, Z, r.bz);
+        const int a0 = (int)fx, b0 = (int)fy, e0 = (int)fz;   // f >= 0: truncation is floor
+        const float ta = __builtin_amdgcn_fractf(fx), tb = __builtin_amdgcn_fractf(fy);
+        const float te = __builtin_amdgcn_fractf(fz);
+        const auto off = YF ? zdup_offset<WIDE>(b0, a0, e0, s1, s12) : zdup_offset<WIDE>(a0, b0, e0, s1, s12);
+        const quad_t q0 = *reinterpret_cast<const quad_t*>(v0 + off);
+        const quad_t q1 = *reinterpret_cast<const quad_t*>(v1 + off);
+        float x00, x10, x01, x11;   // x-interpolated values at (y, z) = (b0, e0), (b0+1, e0), (b0, e0+1), (b0+1, e0+1)
+        if constexpr (YF) {         // q = (b0, e0) (b0, e0+1) (b0+1, e0) (b0+1, e0+1) at x = a0 and a0 + 1
+            x00 = q0.x + ta * (q1.x - q0.x);
+            x10 = q0.z + ta * (q1.z - q0.z);
+            x01 = q0.y + ta * (q1.y - q0.y);
+            x11 = q0.w + ta * (q1.w - q0.w);
+        } else {                    // q = (a0, e0) (a0, e0+1) (a0+1, e0) (a0+1, e0+1) at y = b0 and b0 + 1
+            x00 = q0.x + ta * (q0.z - q0.x);
+            x10 = q1.x + ta * (q1.z - q1.x);
+            x01 = q0.y + ta * (q0.w - q0.y);
+            x11 = q1.y + ta * (q1.w - q1.y);
         }
-        const int k0 = (int)fminf((float)d3, fmaxf(0.0f, ceilf(klo)));
-        const int k1 = (int)fmaxf(-1.0f, fminf((float)(d3 - 1), floorf(khi)));
-        if (yfast && a.vol_z) {
-            // z-pair copies: the cell's four voxels at x = a0 are one 16-byte
-            // load, (b0, e0) (b0, e0+1) (b0+1, e0) (b0+1, e0+1), from the copy
-            // whose pairs start at e0's parity; the same at x = a0 + 1
-            const int sx = 2 * d2;
-            for (int k = k0; k <= k1; ++k) {
-                const float Z = (float)(k - d3 / 2);
-                const float fx = bx + rx * Z, fy = by + ry * Z, fz = bz + rz * Z;
-                int a0, b0, e0;
-                float ta, tb, te;
-                if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
-                    continue;
-                const int odd = e0 & 1;
-                const float* v = a.vol_z + (odd ? a.zodd : 0) + 2 * (b0 + d2 * (a0 + d1 * (e0 >> 1)));
-                const quad_t q0 = *reinterpret_cast<const quad_t*>(v);        // x = a0
-                const quad_t q1 = *reinterpret_cast<const quad_t*>(v + sx);   // x = a0 + 1
-                const float x00 = q0.x + ta * (q1.x - q0.x), x10 = q0.z + ta * (q1.z - q0.z);
-                const float x01 = q0.y + ta * (q1.y - q0.y), x11 = q0.w + ta * (q1.w - q0.w);
-                const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
-                acc += y0 + te * (y1 - y0);
-            }
-        } else if (yfast) {
-            const float* vol = a.vol_y;           // index y + d2*(x + d1*z)
-            const int sx = d2, sz = d1 * d2;
-            for (int k = k0; k <= k1; ++k) {
-                const float Z = (float)(k - d3 / 2);
-                const float fx = bx + rx * Z, fy = by + ry * Z, fz = bz + rz * Z;
-                int a0, b0, e0;
-                float ta, tb, te;
-                if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
-                    continue;
-                const float* v = vol + (b0 + d2 * (a0 + d1 * e0));
-                const pair_t q00 = *reinterpret_cast<const pair_t*>(v);            // (a0,   b0..b0+1, e0)
-                const pair_t q10 = *reinterpret_cast<const pair_t*>(v + sx);       // (a0+1, ., e0)
-                const pair_t q01 = *reinterpret_cast<const pair_t*>(v + sz);       // (a0,   ., e0+1)
-                const pair_t q11 = *reinterpret_cast<const pair_t*>(v + sz + sx);  // (a0+1, ., e0+1)
-                // same interpolation order as the x-fastest branch: along x, y, then z
-                const float x00 = q00.x + ta * (q10.x - q00.x), x10 = q00.y + ta * (q10.y - q00.y);
-                const float x01 = q01.x + ta * (q11.x - q01.x), x11 = q01.y + ta * (q11.y - q01.y);
-                const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
-                acc += y0 + te * (y1 - y0);
-            }
-        } else if (a.vol_zx) {
-            // z-pair copies of the x-fastest model: the cell's four voxels at
-            // y = b0 are one 16-byte load, (a0, e0) (a0, e0+1) (a0+1, e0)
-            // (a0+1, e0+1); the same at y = b0 + 1
-            const int sy = 2 * d1;
-            for (int k = k0; k <= k1; ++k) {
-                const float Z = (float)(k - d3 / 2);
-                const float fx = bx + rx * Z, fy = by + ry * Z, fz = bz + rz * Z;
-                int a0, b0, e0;
-                float ta, tb, te;
-                if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
-                    continue;
-                const int odd = e0 & 1;
-                const float* v = a.vol_zx + (odd ? a.zodd : 0) + 2 * (a0 + d1 * (b0 + d2 * (e0 >> 1)));
-                const quad_t q0 = *reinterpret_cast<const quad_t*>(v);        // y = b0
-                const quad_t q1 = *reinterpret_cast<const quad_t*>(v + sy);   // y = b0 + 1
-                const float x00 = q0.x + ta * (q0.z - q0.x), x10 = q1.x + ta * (q1.z - q1.x);
-                const float x01 = q0.y + ta * (q0.w - q0.y), x11 = q1.y + ta * (q1.w - q1.y);
-                const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
-                acc += y0 + te * (y1 - y0);
-            }
+        const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
+        acc += y0 + te * (y1 - y0);
+    }
+    return acc;
+}
+
+// Every projection of a launch that is not all exact tilts about y: one ray
+// per lane on the z-duplicated forms, the lanes of a wave along the pixel axis
+// R[11] picks (or 8 x 8 patches), and the block mapping of the four-ray form:
+// XCD x takes a contiguous block of ceil(P/8) projections and interleaves the
+// tiles of groups of G neighbouring ones, so that the G workgroups of one tile
+// share the XCD's L2 fills.  With at most 8 projections each XCD takes an
+// eighth of every projection's tiles.
+template <bool WIDE>
+__global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles, int per, int group)
+{
+    const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
+    int p, tile;
+    if (per == 1) {
+        const int tc = (tiles + kXcd - 1) / kXcd;
+        p = slot / tc;
+        tile = xcd * tc + slot % tc;
+        if (tile >= tiles || p >= a.P) return;
+    } else {
+        const int k = (slot / (group * tiles)) * group + slot % group;
+        p = xcd * per + k;
+        tile = (slot / group) % tiles;
+        if (k >= per || p >= a.P) return;
+    }
+    const int d1 = a.d1, d2 = a.d2, d3 = a.d3, c3 = d3 / 2;
+    const uint32_t sx = d1 + 1, sy = d2 + 1, sxy = sx * sy;
+    const float* R = a.R + kRStride * p;
+    int i, j;
+    ray_pixel(a, R[11], tile, &i, &j);
+    float acc = 0.0f;
+    const bool valid = i < a.m1 && j < a.m2;
+    if (valid) {
+        const Ray r = make_ray(a, R, i, j);
+        int k0, k1;
+        ray_range(r, d1, d2, d3, &k0, &k1);
+        const float Z0 = (float)(k0 - c3), Z1 = (float)(k1 - c3);
+        if (R[9] != 0.0f) {
+            const char* v0 = reinterpret_cast<const char*>(a.zy);
+            acc = march<true, WIDE>(v0, v0 + 8 * (size_t)sy, sy, sxy, r, Z0, Z1);
         } else {
-            const float* vol = a.vol;             // index x + d1*(y + d2*z)
-            const int sy = d1, sz = d1 * d2;
-            for (int k = k0; k <= k1; ++k) {
-                const float Z = (float)(k - d3 / 2);
-                const float fx = bx + rx * Z, fy = by + ry * Z, fz = bz + rz * Z;
-                int a0, b0, e0;
-                float ta, tb, te;
-                if (!lin_axis(fx, d1, &a0, &ta) || !lin_axis(fy, d2, &b0, &tb) || !lin_axis(fz, d3, &e0, &te))
-                    continue;
-                const float* v = vol + (a0 + d1 * (b0 + d2 * e0));
-                const pair_t q00 = *reinterpret_cast<const pair_t*>(v);
-                const pair_t q10 = *reinterpret_cast<const pair_t*>(v + sy);
-                const pair_t q01 = *reinterpret_cast<const pair_t*>(v + sz);
-                const pair_t q11 = *reinterpret_cast<const pair_t*>(v + sz + sy);
-                const float x00 = q00.x + ta * (q00.y - q00.x), x10 = q10.x + ta * (q10.y - q10.x);
-                const float x01 = q01.x + ta * (q01.y - q01.x), x11 = q11.x + ta * (q11.y - q11.x);
-                const float y0 = x00 + tb * (x10 - x00), y1 = x01 + tb * (x11 - x01);
-                acc += y0 + te * (y1 - y0);
-            }
+            const char* v0 = reinterpret_cast<const char*>(a.zx);
+            acc = march<false, WIDE>(v0, v0 + 8 * (size_t)sx, sx, sxy, r, Z0, Z1);
         }
     }
     double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
@@ -312,8 +360,7 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
         if (a.meas) {
             const double m = (double)a.meas[o];
             const double b = (double)acc;
-            const double dlt = a.post_hoc ? fabs(fabs(b) - fabs(m)) : fabs(b - m);
-            q0 = dlt;
+            q0 = a.post_hoc ? fabs(fabs(b) - fabs(m)) : fabs(b - m);
             q1 = (b - m) * (b - m);
             q2 = fabs(m);
             q3 = m * m;
@@ -343,6 +390,7 @@ __device__ __forceinline__ void tile_sums(const ProjArgs& a, int p, int tile, in
         for (int k = 0; k < kRows; ++k) t += red[threadIdx.x][k];
         a.sums[((int64_t)p * 4 + threadIdx.x) * tiles + tile] = t;
     }
+    __syncthreads();   // red is reused by the next call of the same workgroup
 }
 
 // y-fastest copy of an x-fastest volume (dst[y + d2*(x + d1*z)] = src[x + d1*(y + d2*z)])
@@ -365,40 +413,6 @@ __global__ __launch_bounds__(256) void k_yfast(const float* __restrict__ src, fl
     }
 }
 
-// z-pair copy of a y-fastest volume: pair m of copy `odd` holds z = 2m + odd + s,
-// s = 0, 1, interleaved per voxel: dst[s + 2*(y + d2*(x + d1*m))]; z >= d3 is 0.
-__global__ __launch_bounds__(256) void k_zpair(const float* __restrict__ src, float* __restrict__ dst, int d1, int d2,
-                                               int d3, int odd, int64_t total)
-{
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int s = (int)(t & 1);
-        const int64_t yx = (t >> 1) % ((int64_t)d1 * d2), m = (t >> 1) / ((int64_t)d1 * d2);
-        const int64_t z = 2 * m + odd + s;
-        dst[t] = z < d3 ? src[yx + (int64_t)d1 * d2 * z] : 0.0f;
-    }
-}
-
-__device__ __forceinline__ int wrap_n(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
-
-// z-pair copies of the x-fastest crop: pair m of copy `odd` holds z = 2m + odd + s
-// at dst[s + 2 (x + d1 (y + d2 m))], straight from the FFT-order volume.
-__global__ __launch_bounds__(256) void k_crop_zpairs_x(const float* __restrict__ src, int n1, int n2, int n3,
-                                                       float* __restrict__ dst, int d1, int d2, int d3,
-                                                       int64_t per)
-{
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 2 * per;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t o = t % per, odd = t / per;
-        const int s = (int)(o & 1);
-        const int64_t xy = (o >> 1) % ((int64_t)d1 * d2), m = (o >> 1) / ((int64_t)d1 * d2);
-        const int x = (int)(xy % d1), y = (int)(xy / d1);
-        const int z = (int)(2 * m + odd + s);
-        dst[t] = z < d3 ? src[(int64_t)wrap_n(x - d1 / 2, n1) +
-                              (int64_t)n1 * (wrap_n(y - d2 / 2, n2) + (int64_t)n2 * wrap_n(z - d3 / 2, n3))]
-                        : 0.0f;
-    }
-}
-
 // One wave per (projection, sum): lane sums over a stride of 64, then the wave,
 // in a fixed order.
 __global__ __launch_bounds__(256) void k_project_reduce(const double* sums, int P, int nblk, double* out)
@@ -411,40 +425,69 @@ __global__ __launch_bounds__(256) void k_project_reduce(const double* sums, int 
     if (l == 0) out[w] = s;
 }
 
-// The z-pair copies (as k_zpair) straight from the FFT-order iterate: model
-// voxel (x, y, z) is src[wrap(x - d1/2) + n1 (wrap(y - d2/2) + n2 wrap(z - d3/2))]
-// (the centred crop of fftshift).  A workgroup takes a 32 x 32 (x, y) tile of
-// pair m: it stages slices 2m, 2m+1, 2m+2 through LDS (coalesced along x) and
-// writes pair m of both copies (coalesced along y).
-__global__ __launch_bounds__(256) void k_crop_zpairs(const float* __restrict__ src, int n1, int n2, int n3,
-                                                     float* __restrict__ dst, int d1, int d2, int d3,
-                                                     int64_t zodd)
+// z-duplicated forms of the model (see the top of this file): element
+// (x, y, z) = {vol(x, y, z), vol(x, y, z + 1)}, zero where x = d1, y = d2 or
+// z + 1 = d3.  The model is read either from a plain column-major d1 x d2 x d3
+// volume (n = d, no shift) or as the centred crop of an FFT-order n1 x n2 x n3
+// volume (voxel (x, y, z) at (wrap(x - d1/2), wrap(y - d2/2), wrap(z - d3/2))).
+__device__ __forceinline__ int wrap_n(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+struct ModelSrc {
+    const float* src;
+    int n1, n2, n3;   // source extents
+    int o1, o2, o3;   // shift of voxel (x, y, z): source (x - o1, y - o2, z - o3) wrapped
+    int d1, d2, d3;   // model extents
+    __device__ __forceinline__ float at(int x, int y, int z) const
+    {
+        if (x >= d1 || y >= d2 || z >= d3) return 0.0f;
+        return src[(int64_t)wrap_n(x - o1, n1) + (int64_t)n1 * (wrap_n(y - o2, n2) + (int64_t)n2 * wrap_n(z - o3, n3))];
+    }
+};
+
+// y-fastest form: a workgroup transposes a 32 x 32 (x, y) tile of slices z and
+// z + 1 through LDS (read along x, written along y with the pairs interleaved).
+__global__ __launch_bounds__(256) void k_zdup_y(ModelSrc m, float* __restrict__ dst)
 {
-    __shared__ float t[3][32][33];
-    const int bx = blockIdx.x * 32, by = blockIdx.y * 32, m = blockIdx.z;
+    __shared__ float t[2][32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32, z = blockIdx.z;
     const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int z = 2 * m + q;
-        for (int r = ty; r < 32; r += 8) {
-            const int x = bx + tx, y = by + r;
-            float v = 0.0f;
-            if (x < d1 && y < d2 && z < d3)
-                v = src[(int64_t)wrap_n(x - d1 / 2, n1) +
-                        (int64_t)n1 * (wrap_n(y - d2 / 2, n2) + (int64_t)n2 * wrap_n(z - d3 / 2, n3))];
-            t[q][r][tx] = v;
-        }
-    }
+    for (int q = 0; q < 2; ++q)
+        for (int r = ty; r < 32; r += 8) t[q][r][tx] = m.at(bx + tx, by + r, z + q);
     __syncthreads();
+    const int X1 = m.d1 + 1, Y2 = m.d2 + 1;
     const int l = threadIdx.x % 64, yy = l >> 1, sp = l & 1;
     const int y = by + yy;
     for (int xr = threadIdx.x / 64; xr < 32; xr += 4) {
         const int x = bx + xr;
-        if (x >= d1 || y >= d2) continue;
-        const int64_t o = sp + 2 * (y + (int64_t)d2 * (x + (int64_t)d1 * m));
-        dst[o] = t[sp][yy][xr];             // even copy: z = 2m + sp
-        dst[zodd + o] = t[1 + sp][yy][xr];  // odd copy: z = 2m + 1 + sp
+        if (x >= X1 || y >= Y2) continue;
+        dst[sp + 2 * ((int64_t)y + (int64_t)Y2 * (x + (int64_t)X1 * z))] = t[sp][yy][xr];
     }
+}
+
+// x-fastest form: one thread per element, reading along x.
+__global__ __launch_bounds__(256) void k_zdup_x(ModelSrc m, float2* __restrict__ dst, int64_t total)
+{
+    const int X1 = m.d1 + 1, Y1 = m.d2 + 1;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % X1);
+        const int64_t yz = e / X1;
+        const int y = (int)(yz % Y1), z = (int)(yz / Y1);
+        dst[e] = make_float2(m.at(x, y, z), m.at(x, y, z + 1));
+    }
+}
+
+void zdup_launch(const ModelSrc& m, float* d_dst, bool yfast, hipStream_t st)
+{
+    if (yfast) {
+        dim3 grid((m.d1 + 1 + 31) / 32, (m.d2 + 1 + 31) / 32, m.d3);
+        k_zdup_y<<<grid, 256, 0, st>>>(m, d_dst);
+    } else {
+        const int64_t total = (int64_t)(m.d1 + 1) * (m.d2 + 1) * m.d3;
+        const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
+        k_zdup_x<<<(unsigned)blocks, 256, 0, st>>>(m, reinterpret_cast<float2*>(d_dst), total);
+    }
+    GF_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -495,49 +538,80 @@ void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipS
     GF_LAUNCH_CHECK();
 }
 
-int64_t zpair_floats(int d1, int d2, int d3) { return 2 * (int64_t)d1 * d2 * (d3 / 2 + 1); }
 
-void volume_zpairs(const float* d_vol_y, float* d_dst, int d1, int d2, int d3, hipStream_t st)
+int64_t zdup_floats(int d1, int d2, int d3) { return 2 * (int64_t)(d1 + 1) * (d2 + 1) * d3; }
+
+void volume_zdup(const float* d_vol, int d1, int d2, int d3, float* d_dst, bool yfast, hipStream_t st)
 {
-    const int64_t total = zpair_floats(d1, d2, d3);
-    for (int odd = 0; odd < 2; ++odd) {
-        const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
-        k_zpair<<<(unsigned)blocks, 256, 0, st>>>(d_vol_y, d_dst + odd * total, d1, d2, d3, odd, total);
-        GF_LAUNCH_CHECK();
+    zdup_launch(ModelSrc{d_vol, d1, d2, d3, 0, 0, 0, d1, d2, d3}, d_dst, yfast, st);
+}
+
+void crop_zdup(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, bool yfast,
+               hipStream_t st)
+{
+    zdup_launch(ModelSrc{d_src, n1, n2, n3, d1 / 2, d2 / 2, d3 / 2, d1, d2, d3}, d_dst, yfast, st);
+}
+
+ProjForms project_forms(const float* Rm, int P, int d2)
+{
+    ProjForms f;
+    bool all_exact = P > 0;
+    for (int p = 0; p < P; ++p) {
+        const float* r = Rm + (size_t)kRStride * p;
+        all_exact = all_exact && r[10] != 0.0f;
+        f.zy = f.zy || r[9] != 0.0f;
+        f.zx = f.zx || r[9] == 0.0f;
     }
+    f.four_ray = all_exact && d2 % 4 == 0;
+    if (f.four_ray) f.zy = f.zx = false;
+    return f;
 }
 
-void crop_zpairs_x(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st)
-{
-    const int64_t per = zpair_floats(d1, d2, d3);
-    const int64_t blocks = std::min<int64_t>((2 * per + 255) / 256, 1 << 20);
-    k_crop_zpairs_x<<<(unsigned)blocks, 256, 0, st>>>(d_src, n1, n2, n3, d_dst, d1, d2, d3, per);
-    GF_LAUNCH_CHECK();
-}
+namespace {
 
-void crop_zpairs(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3, hipStream_t st)
+void launch_ray(const ProjArgs& a, int tiles, hipStream_t st)
 {
-    dim3 grid((d1 + 31) / 32, (d2 + 31) / 32, d3 / 2 + 1);
-    k_crop_zpairs<<<grid, 256, 0, st>>>(d_src, n1, n2, n3, d_dst, d1, d2, d3, zpair_floats(d1, d2, d3));
-    GF_LAUNCH_CHECK();
-}
-
-void project(const ProjArgs& a, hipStream_t st)
-{
-    GF_CHECK(a.P >= 1, GENFIRE_ERR_ARG, "projector needs at least one projection");
-    GF_CHECK(a.d1 >= 2 && a.d2 >= 2 && a.d3 >= 2, GENFIRE_ERR_ARG, "projector needs every volume dimension >= 2");
-    GF_CHECK(a.m1 <= a.d1 && a.m2 <= a.d2, GENFIRE_ERR_ARG, "measured projection larger than the volume");
-    GF_CHECK((int64_t)a.d1 * a.d2 * a.d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "projector volume above 2^31 voxels");
-    const int tiles = project_blocks(a.m1, a.m2);
-    const int per = (a.P + kXcd - 1) / kXcd, group = std::min(kProjGroup, per);
+    const int per = (a.P + kXcd - 1) / kXcd, group = std::min(kRayGroup, per);
     const int groups = (per + group - 1) / group;
     const int64_t blocks = per == 1 ? (int64_t)kXcd * a.P * ((tiles + kXcd - 1) / kXcd)
                                     : (int64_t)kXcd * groups * group * tiles;
     // a dispatch counts its x extent in work items, in 32 bits
     GF_CHECK(blocks * kLanes * kRows < (1ll << 32), GENFIRE_ERR_UNSUPPORTED,
              "projector launch above 2^32 work items: too many projections of this size");
-    k_project<<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
+    // 32-bit offsets from 24-bit products: the form below 4 GiB and the plane
+    // size (d1 + 1)(d2 + 1) below 2^24
+    const bool wide = 4 * zdup_floats(a.d1, a.d2, a.d3) > (int64_t)UINT32_MAX ||
+                      (int64_t)(a.d1 + 1) * (a.d2 + 1) >= (1ll << 24);
+    if (wide) k_project<true><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
+    else k_project<false><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
     GF_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void project(const ProjArgs& a, hipStream_t st)
+{
+    GF_CHECK(a.P >= 1, GENFIRE_ERR_ARG, "projector needs at least one projection");
+    GF_CHECK(a.d1 >= 2 && a.d2 >= 2 && a.d3 >= 2, GENFIRE_ERR_ARG, "projector needs every volume dimension >= 2");
+    GF_CHECK(a.m1 >= 1 && a.m2 >= 1 && a.m1 <= a.d1 && a.m2 <= a.d2, GENFIRE_ERR_ARG,
+             "measured projection empty or larger than the volume");
+    GF_CHECK((int64_t)a.d1 * a.d2 * a.d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "projector volume above 2^31 voxels");
+    const int tiles = project_blocks(a.m1, a.m2);
+    if (a.four_ray) {
+        GF_CHECK(a.vol_y && a.d2 % 4 == 0, GENFIRE_ERR_ARG, "four-ray projector form needs the y-fastest model");
+        const int per = (a.P + kXcd - 1) / kXcd, group = std::min(kProjGroup, per);
+        const int groups = (per + group - 1) / group;
+        const int64_t blocks = per == 1 ? (int64_t)kXcd * a.P * ((tiles + kXcd - 1) / kXcd)
+                                        : (int64_t)kXcd * groups * group * tiles;
+        // a dispatch counts its x extent in work items, in 32 bits
+        GF_CHECK(blocks * kLanes * kRows < (1ll << 32), GENFIRE_ERR_UNSUPPORTED,
+                 "projector launch above 2^32 work items: too many projections of this size");
+        k_project_ytilt<<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
+        GF_LAUNCH_CHECK();
+        return;
+    }
+    GF_CHECK(a.zy || a.zx, GENFIRE_ERR_ARG, "projector needs a z-duplicated model form");
+    launch_ray(a, tiles, st);
 }
 
 void project_reduce(const double* d_sums, int P, int nblk, double* d_out4, hipStream_t st)

@@ -419,3 +419,63 @@ void go_project(const float* vol, int d1, int d2, int d3, const double R[9], dou
             out[i + (int64_t)d1 * j] = acc;
         }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Type-2 NUFFT interpolation (the CPU gridding route, nufft2d2 at            */
+/* fillInFourierGrid_DFT.m:249-254), for the CPU baseline                     */
+/* ------------------------------------------------------------------------- */
+
+/* F_j = sum over the w x w nearest nodes (m1, m2) of the FP64 fine grid G
+ * (M1 x M2 row-major, interleaved complex, G[m1][m2]) of
+ * G[m1 mod M1][m2 mod M2] phi(x_j - m1) phi(y_j - m2), x_j = un_j M1,
+ * y_j = vn_j M2, with the exponential-of-semicircle kernel
+ * phi(t) = exp(beta (sqrt(1 - (2t/w)^2) - 1)) on |t| < w/2 (Barnett, Magland
+ * and af Klinteberg, SIAM J. Sci. Comput. 41, 2019).  The caller has divided
+ * the modes by the kernel's Fourier transform and taken the fine grid's FFT. */
+static double es_kernel(double t, int w, double beta)
+{
+    const double z = 2.0 * t / w;
+    if (z * z >= 1.0) return 0.0;
+    return exp(beta * (sqrt(1.0 - z * z) - 1.0));
+}
+
+void go_nufft_interp(const double* G, int M1, int M2, int w, double beta, int64_t m, const float* un,
+                     const float* vn, double* F)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < m; ++j) {
+        const double x = (double)un[j] * M1, y = (double)vn[j] * M2;
+        const int i0 = (int)ceil(x - 0.5 * w), j0 = (int)ceil(y - 0.5 * w);
+        double kx[32], ky[32];
+        for (int q = 0; q < w; ++q) {
+            kx[q] = es_kernel(x - (i0 + q), w, beta);
+            ky[q] = es_kernel(y - (j0 + q), w, beta);
+        }
+        double fr = 0.0, fi = 0.0;
+        for (int q = 0; q < w; ++q) {
+            int r1 = (i0 + q) % M1;
+            if (r1 < 0) r1 += M1;
+            const double* row = G + 2 * (int64_t)r1 * M2;
+            double gr = 0.0, gi = 0.0;
+            for (int s = 0; s < w; ++s) {
+                int c = (j0 + s) % M2;
+                if (c < 0) c += M2;
+                gr += row[2 * c] * ky[s];
+                gi += row[2 * c + 1] * ky[s];
+            }
+            fr += gr * kx[q];
+            fi += gi * kx[q];
+        }
+        F[2 * j] = fr;
+        F[2 * j + 1] = fi;
+    }
+}
+
+/* OpenMP threads of the oracle's parallel loops (the CPU baseline's core count). */
+void go_set_threads(int n);
+#ifdef _OPENMP
+#include <omp.h>
+void go_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+#else
+void go_set_threads(int n) { (void)n; }
+#endif

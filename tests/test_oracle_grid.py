@@ -242,3 +242,32 @@ def test_centro_grid_is_hermitian_off_self_paired_planes(O):
         a = mk[c + k1, 1:, 1:]
         b = mk[c - k1, 1:, 1:][::-1, ::-1]
         np.testing.assert_array_equal(a, np.conj(b))
+
+
+@pytest.mark.parametrize("d1,d2", [(64, 64), (65, 48), (128, 96)])
+def test_cpu_nufft_matches_direct_dft(O, d1, d2):
+    """The oracle's type-2 NUFFT (the CPU route of fillInFourierGrid_DFT.m:249-254,
+    used by the CPU baseline) against the FP64 direct sum at random in-plane
+    points, to nufft2d2's requested tolerance 1e-6; and at integer frequencies
+    against the FFT of the projection (a textbook identity)."""
+    rng = np.random.default_rng(d1 * 1000 + d2)
+    P = rng.uniform(0.0, 1.0, (d1, d2)).astype(np.float32)
+    un = rng.uniform(-0.5, 0.5, 2000).astype(np.float32)
+    vn = rng.uniform(-0.5, 0.5, 2000).astype(np.float32)
+    a = O.nufft2d2(P, un, vn)
+    b = O.dft64(P, un, vn)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-6
+    # u = p/d1, v = q/d2: F = sum P[a,b] exp(-2 pi i ((a - c1) p/d1 + (b - c2) q/d2)) = shifted fft2
+    p = np.arange(-(d1 // 2), d1 - d1 // 2)
+    q = np.arange(-(d2 // 2), d2 - d2 // 2)
+    pp, qq = np.meshgrid(p, q, indexing="ij")
+    ui = (pp / d1).astype(np.float32).ravel()
+    vi = (qq / d2).astype(np.float32).ravel()
+    exact = (np.fft.fft2(np.roll(P.astype(np.float64), (-(d1 // 2), -(d2 // 2)), axis=(0, 1)))
+             [pp % d1, qq % d2]).ravel()
+    # the FP32 coordinates p/d1 are not exactly the integer frequencies; compare with the direct sum there
+    direct = O.dft64(P, ui, vi)
+    fine = O.nufft2d2(P, ui, vi)
+    assert np.linalg.norm(fine - direct) / np.linalg.norm(direct) < 1e-6
+    if d1 & (d1 - 1) == 0 and d2 & (d2 - 1) == 0:   # p / d exact in FP32
+        assert np.linalg.norm(direct - exact) / np.linalg.norm(exact) < 1e-12
