@@ -35,7 +35,7 @@ EXPORTS = [
     "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_get_Rfree", "genfire_job_submit", "genfire_job_poll",
     "genfire_job_wait", "genfire_job_timing", "genfire_job_set_profiling",
     "genfire_job_kernel_stats", "genfire_job_graph_replays", "genfire_project", "genfire_rfactor", "genfire_fftn",
-    "genfire_project_general", "genfire_rfactor_general",
+    "genfire_project_general", "genfire_rfactor_general", "genfire_rfactor_noalloc",
     "genfire_server_config_default", "genfire_server_create", "genfire_server_destroy",
     "genfire_server_submit", "genfire_server_status", "genfire_server_wait", "genfire_server_drain",
     "genfire_server_result", "genfire_server_release", "genfire_server_get_stats",
@@ -160,6 +160,7 @@ def load():
         "genfire_project_general": ([vp, pf, i32, i32, i32, f64, f64, f64, pd, pd, pd, pf], ctypes.c_int),
         "genfire_rfactor_general": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pd, pd, pd, pf],
                                     ctypes.c_int),
+        "genfire_rfactor_noalloc": ([vp, vp, i32, i32, i32, vp, i32, i32, i32, pd, pd, pd, vp], ctypes.c_int),
         "genfire_server_config_default": ([P(ServerConfig)], ctypes.c_int),
         "genfire_server_create": ([vp, P(ServerConfig), P(vp)], ctypes.c_int),
         "genfire_server_destroy": ([vp], ctypes.c_int),
@@ -277,6 +278,24 @@ class Context:
                                           _ptr(ra, ctypes.c_double), _ptr(sim, ctypes.c_float)),
                    "genfire_rfactor")
         return float(rt[0]), ra, sim.reshape(p1, p2, P, order="F")
+
+
+    def rfactor_noalloc(self, d_model: int, model_shape, d_projections: int, proj_shape, angles: np.ndarray,
+                        d_simu: int = 0):
+        """genfire_rfactor_noalloc on device buffers (integer device addresses,
+        column-major model and projections, d_simu 0 for none) -> (Rtotal, Rarr)."""
+        d1, d2, d3 = (int(v) for v in model_shape)
+        p1, p2, P = (int(v) for v in proj_shape)
+        ang = np.ascontiguousarray(np.asarray(angles, np.float64).reshape(P, 3).ravel(order="F"))
+        rt = np.zeros(1)
+        ra = np.zeros(P)
+        self.check(self.L.genfire_rfactor_noalloc(self.h, ctypes.c_void_p(int(d_model)), d1, d2, d3,
+                                                  ctypes.c_void_p(int(d_projections)), p1, p2, P,
+                                                  _ptr(ang, ctypes.c_double), _ptr(rt, ctypes.c_double),
+                                                  _ptr(ra, ctypes.c_double),
+                                                  ctypes.c_void_p(int(d_simu)) if d_simu else None),
+                   "genfire_rfactor_noalloc")
+        return float(rt[0]), ra
 
 
 class Job:

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "gf_job.h"
 
@@ -12,6 +13,40 @@ using namespace gf;
 // ===========================================================================
 // C ABI
 // ===========================================================================
+// Device scratch of genfire_rfactor_noalloc (the projector's R matrices, tile
+// partial sums and model forms): one array per GPU, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel or a copy: it
+// uses the device address hipGetSymbolAddress returns for the current GPU,
+// taken once per device when a context is created.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kNoAllocMaxProjs = 1024;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_bind_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+// Serialises the no-allocation calls of the process: they share each GPU's
+// scratch and the host staging arrays below.
+std::mutex g_noalloc_mu;
+float g_noalloc_R[(size_t)kNoAllocMaxProjs * kRStride];
+double g_noalloc_sums[(size_t)kNoAllocMaxProjs * 4];
+
+// With `device` current: its address of `scratch`, looked up once.
+int bind_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return GENFIRE_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_bind_mu);
+    if (g_scratch_ptr[device]) return GENFIRE_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GENFIRE_ERR_HIP;
+    g_scratch_ptr[device] = p;
+    return GENFIRE_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int genfire_version(void) { return 100; }
@@ -52,6 +87,8 @@ int genfire_ctx_create(int32_t device, genfire_ctx** out)
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GENFIRE_ERR_HIP;
     if (device < 0 || device >= n) return GENFIRE_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return GENFIRE_ERR_HIP;
+    const int rc = bind_scratch(device);
+    if (rc != GENFIRE_OK) return rc;
     auto* c = new (std::nothrow) genfire_ctx();
     if (!c) return GENFIRE_ERR_NOMEM;
     c->device = device;
@@ -536,6 +573,77 @@ int genfire_rfactor_general(genfire_ctx* ctx, const float* model, int32_t d1, in
     for (int i = 0; i < 3; ++i) { vec[i] = vector1[i]; vec[3 + i] = vector2[i]; vec[6 + i] = vector3[i]; }
     return guarded(ctx, [&] {
         rfactor_all(ctx, model, d1, d2, d3, projections, p1, p2, P, angles, vec, Rtotal, Rarr, simu);
+    });
+}
+
+int genfire_rfactor_noalloc(genfire_ctx* ctx, const float* d_model, int32_t d1, int32_t d2, int32_t d3,
+                            const float* d_projections, int32_t p1, int32_t p2, int32_t P, const double* angles,
+                            double* Rtotal, double* Rarr, float* d_simuProjs)
+{
+    if (!ctx || !d_model || !d_projections || !angles || P < 1) return GENFIRE_ERR_ARG;
+    return guarded(ctx, [&] {
+        use_device(ctx);
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        GF_CHECK(P <= kNoAllocMaxProjs, GENFIRE_ERR_UNSUPPORTED,
+                 "GENFIRE: genfire_rfactor_noalloc takes at most " + std::to_string(kNoAllocMaxProjs) + " projections");
+        GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: model dimensions must be >= 2");
+        GF_CHECK(p1 >= 1 && p2 >= 1, GENFIRE_ERR_ARG, "GENFIRE: empty projections");
+        GF_CHECK(p1 <= d1 && p2 <= d2, GENFIRE_ERR_ARG, "GENFIRE: projection larger than the model");
+        GF_CHECK((int64_t)d1 * d2 * d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "GENFIRE: model above 2^31 voxels");
+        char* base = static_cast<char*>(g_scratch_ptr[ctx->device]);
+        GF_CHECK(base, GENFIRE_ERR_STATE, "GENFIRE: no scratch address for this device");
+        proj_matrices_into(angles, P, nullptr, g_noalloc_R);
+        const ProjForms f = project_forms(g_noalloc_R, P, d2);
+        // scratch layout, 256-byte aligned pieces: R matrices, tile partial sums,
+        // per-projection sums, then the model forms the projections read
+        const int nblk = project_blocks(p1, p2);
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t b_R = up(sizeof(float) * (size_t)kRStride * P);
+        const size_t b_s = up(sizeof(double) * (size_t)P * 4 * nblk);
+        const size_t b_s4 = up(sizeof(double) * (size_t)P * 4);
+        const size_t b_vy = f.four_ray ? up(sizeof(float) * (size_t)d1 * d2 * d3) : 0;
+        const size_t b_zy = f.zy ? up(sizeof(float) * (size_t)zdup_floats(d1, d2, d3)) : 0;
+        const size_t b_zx = f.zx ? up(sizeof(float) * (size_t)zdup_floats(d1, d2, d3)) : 0;
+        const size_t need = b_R + b_s + b_s4 + b_vy + b_zy + b_zx;
+        GF_CHECK(need <= kNoAllocScratchBytes, GENFIRE_ERR_UNSUPPORTED,
+                 "GENFIRE: genfire_rfactor_noalloc needs " + std::to_string(need) + " bytes of scratch for this model "
+                 "and these projections; its limit is " + std::to_string(kNoAllocScratchBytes));
+        float* dR = reinterpret_cast<float*>(base);
+        double* ds = reinterpret_cast<double*>(base + b_R);
+        double* ds4 = reinterpret_cast<double*>(base + b_R + b_s);
+        char* forms = base + b_R + b_s + b_s4;
+        float* dvy = f.four_ray ? reinterpret_cast<float*>(forms) : nullptr;
+        float* dzy = f.zy ? reinterpret_cast<float*>(forms + b_vy) : nullptr;
+        float* dzx = f.zx ? reinterpret_cast<float*>(forms + b_vy + b_zy) : nullptr;
+        hipStream_t st = nullptr;   // the null stream: nothing to create
+        GF_HIP(hipMemcpyAsync(dR, g_noalloc_R, sizeof(float) * (size_t)kRStride * P, hipMemcpyHostToDevice, st));
+        if (dvy) volume_yfast(d_model, dvy, d1, d2, d3, st);
+        if (dzy) volume_zdup(d_model, d1, d2, d3, dzy, true, st);
+        if (dzx) volume_zdup(d_model, d1, d2, d3, dzx, false, st);
+        ProjArgs pa{};
+        pa.four_ray = f.four_ray ? 1 : 0;
+        pa.vol_y = dvy;
+        pa.zy = dzy;
+        pa.zx = dzx;
+        pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
+        pa.R = dR;
+        pa.P = P;
+        pa.out = d_simuProjs;
+        pa.meas = d_projections;
+        pa.m1 = p1; pa.m2 = p2;
+        pa.sums = ds;
+        pa.post_hoc = 1;
+        project(pa, st);
+        project_reduce(ds, P, nblk, ds4, st);
+        GF_HIP(hipMemcpyAsync(g_noalloc_sums, ds4, sizeof(double) * (size_t)P * 4, hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
+        double mean = 0.0;
+        for (int p = 0; p < P; ++p) {
+            const double rr = g_noalloc_sums[4 * p] / g_noalloc_sums[4 * p + 2];
+            if (Rarr) Rarr[p] = rr;
+            mean += rr;
+        }
+        if (Rtotal) *Rtotal = mean / P;
     });
 }
 

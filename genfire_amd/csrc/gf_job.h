@@ -192,6 +192,8 @@ int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P);
 // calculate3Dprojection_RealSpaceinterp.m:13-15, or with vectors (v1, v2, v3,
 // 9 doubles) the quaternion product of ..._general.m:3-7.
 std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const double* vectors = nullptr);
+// The same matrices written into Rm (P x kRStride floats), allocating nothing.
+void proj_matrices_into(const double* angles, int P, const double* vectors, float* Rm);
 
 // runGridding: fillInFourierGrid_DFT + CentroSymmetricity (griddingMethod 2) or
 // fillInFourierGrid + hermitianSymmetrize (griddingMethod 1), and recIFFT.

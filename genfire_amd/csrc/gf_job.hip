@@ -147,26 +147,31 @@ void kstat_end(genfire_job* j, int k, std::pair<hipEvent_t, hipEvent_t>& ev)
 
 // ZYZ matrices of the projector (calculate3Dprojection_RealSpaceinterp.m:13-15),
 // kRStride floats per projection with the y-fastest flag at [9], the exact-y
-// flag at [10] and the lane-axis flag at [11].
-std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const double* vectors)
+// flag at [10] and the lane-axis flag at [11], into Rm (P x kRStride floats).
+void proj_matrices_into(const double* angles, int P, const double* vectors, float* Rm)
 {
-    std::vector<float> Rm((size_t)kRStride * P, 0.0f);
-    *any_y = false;
     for (int p = 0; p < P; ++p) {
         double R[9];
         if (vectors)   // src/calculate3Dprojection_RealSpaceinterp_general.m:3-7
             rotation_f64(vectors, vectors + 3, vectors + 6, angles[p], angles[P + p], angles[2 * P + p], R);
         else
             zyz_matrix(angles[p], angles[P + p], angles[2 * P + p], R);
-        float* r = Rm.data() + (size_t)kRStride * p;
-        for (int q = 0; q < 9; ++q) r[q] = (float)R[q];
+        float* r = Rm + (size_t)kRStride * p;
+        for (int q = 0; q < kRStride; ++q) r[q] = q < 9 ? (float)R[q] : 0.0f;
         if (project_y_exact(r)) {
             r[9] = 1.0f; r[10] = 1.0f; r[11] = 1.0f;
         } else {
             project_orientation(r);
         }
-        *any_y = *any_y || r[9] != 0.0f;
     }
+}
+
+std::vector<float> proj_matrices(const double* angles, int P, bool* any_y, const double* vectors)
+{
+    std::vector<float> Rm((size_t)kRStride * P, 0.0f);
+    proj_matrices_into(angles, P, vectors, Rm.data());
+    *any_y = false;
+    for (int p = 0; p < P; ++p) *any_y = *any_y || Rm[(size_t)kRStride * p + 9] != 0.0f;
     return Rm;
 }
 

@@ -208,6 +208,23 @@ int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2
                     const float* projections, int32_t p1, int32_t p2, int32_t num_projs,
                     const double* angles, double* Rtotal, double* Rarr, float* simuProjs);
 
+/* genfire_rfactor on device-resident buffers, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc).  d_model (d1 x d2 x d3
+ * float), d_projections (p1 x p2 x P float) and d_simuProjs (p1 x p2 x P float,
+ * or NULL) are device pointers on the context's GPU; angles (P x 3 double),
+ * Rtotal and Rarr[P] are host memory.  The projector's matrices, partial sums
+ * and model forms live in one static device array per GPU of 64 MiB, held until
+ * the process ends: a call needs about 8 (d1+1)(d2+1) d3 bytes per model form
+ * (one or two forms; 4 d1 d2 d3 for a tilt series about y with d2 % 4 == 0)
+ * plus 8 KiB per 256 x 256 projection, and returns GENFIRE_ERR_UNSUPPORTED above
+ * 64 MiB or above 1024 projections.  Calls share that array and its last
+ * contents, so the library serialises them with a process-wide lock; the work
+ * runs on the null stream and the call returns when it is done. */
+int genfire_rfactor_noalloc(genfire_ctx* ctx, const float* d_model, int32_t d1, int32_t d2, int32_t d3,
+                            const float* d_projections, int32_t p1, int32_t p2, int32_t num_projs,
+                            const double* angles, double* Rtotal, double* Rarr, float* d_simuProjs);
+
 /* calculate3Dprojection_RealSpaceinterp_general(Vol, phi, theta, psi, vector1,
  * vector2, vector3) (src/calculate3Dprojection_RealSpaceinterp_general.m):
  * R = Q(vector1, phi) Q(vector2, theta) Q(vector3, psi) (MatrixQuaternionRot),
