@@ -35,7 +35,7 @@ EXPORTS = [
     "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_get_Rfree", "genfire_job_submit", "genfire_job_poll",
     "genfire_job_wait", "genfire_job_timing", "genfire_job_set_profiling",
     "genfire_job_kernel_stats", "genfire_job_graph_replays", "genfire_project", "genfire_rfactor", "genfire_fftn",
-    "genfire_project_general", "genfire_rfactor_general", "genfire_rfactor_noalloc",
+    "genfire_project_general", "genfire_rfactor_general", "genfire_rfactor_noalloc", "genfire_job_set_snapshots",
     "genfire_server_config_default", "genfire_server_create", "genfire_server_destroy",
     "genfire_server_submit", "genfire_server_status", "genfire_server_wait", "genfire_server_drain",
     "genfire_server_result", "genfire_server_release", "genfire_server_get_stats",
@@ -99,6 +99,10 @@ class ServerStats(ctypes.Structure):
     ]
 
 
+# void (*)(void* user, int32_t iteration, const float* reconstruction)
+SNAPSHOT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_float))
+
+
 class GenfireError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS.get(status, status)}: {message}")
@@ -143,6 +147,7 @@ def load():
         "genfire_job_get_recIFFT": ([vp, pf], ctypes.c_int),
         "genfire_job_set_measuredK": ([vp, pf, pu8], ctypes.c_int),
         "genfire_job_reconstruct": ([vp], ctypes.c_int),
+        "genfire_job_set_snapshots": ([vp, i32, SNAPSHOT_FN, vp], ctypes.c_int),
         "genfire_job_get_reconstruction": ([vp, pf], ctypes.c_int),
         "genfire_job_get_errK": ([vp, pf], ctypes.c_int),
         "genfire_job_get_errR": ([vp, pf, pd, pd], ctypes.c_int),
@@ -409,6 +414,24 @@ class Job:
 
     def reconstruct(self):
         self._c(self.L.genfire_job_reconstruct(self.h), "genfire_job_reconstruct")
+
+    def set_snapshots(self, save_loopLength: int, fn=None):
+        """save_temp (reconstruct.m:155-159): fn(iteration, reconstruction) every
+        save_loopLength iterations of reconstruct(), reconstruction a
+        Dim1 x Dim2 x Dim1 copy; fn None (or a length <= 0) turns them off."""
+        if fn is None or save_loopLength <= 0:
+            self._snap_cb = None
+            self._c(self.L.genfire_job_set_snapshots(self.h, 0, SNAPSHOT_FN(), None), "genfire_job_set_snapshots")
+            return
+        shape, n = (self.dim1, self.dim2, self.dim1), self.dim1 * self.dim2 * self.dim1
+
+        def cb(_user, it, ptr):
+            vol = np.ctypeslib.as_array(ptr, shape=(n,)).copy().reshape(shape, order="F")
+            fn(int(it), vol)
+
+        self._snap_cb = SNAPSHOT_FN(cb)   # kept alive while the job may call it
+        self._c(self.L.genfire_job_set_snapshots(self.h, int(save_loopLength), self._snap_cb, None),
+                "genfire_job_set_snapshots")
 
     def reconstruction(self) -> np.ndarray:
         out = np.zeros(self.dim1 * self.dim2 * self.dim1, np.float32)

@@ -292,6 +292,16 @@ int genfire_job_reconstruct(genfire_job* j)
     });
 }
 
+int genfire_job_set_snapshots(genfire_job* j, int32_t save_loopLength, genfire_snapshot_fn fn, void* user)
+{
+    if (!j) return GENFIRE_ERR_ARG;
+    const bool on = save_loopLength > 0 && fn != nullptr;
+    j->snap_L = on ? save_loopLength : 0;
+    j->snap_fn = on ? fn : nullptr;
+    j->snap_user = on ? user : nullptr;
+    return GENFIRE_OK;
+}
+
 int genfire_job_get_reconstruction(genfire_job* j, float* out)
 {
     if (!j || !out) return GENFIRE_ERR_ARG;

@@ -125,6 +125,13 @@ struct genfire_job {
     std::vector<float> nu_inv_h;         // host source of nu_inv, the NUFFT deconvolution factors
     gf::ProjForms rforms;         // the model forms the monitor's projections read
     std::vector<float> errK_h, errR_h;
+    // save_temp snapshots (reconstruct.m:155-159): every snap_L iterations the
+    // best P_R(x) since the last schedule stage goes to snap_fn
+    int snap_L = 0;
+    genfire_snapshot_fn snap_fn = nullptr;
+    void* snap_user = nullptr;
+    gf::DevBuf cand[2], best_box, best_err, best_flag;
+    std::vector<float> snap_h;
     std::vector<double> Rarr_h, Rarr2_h;
     bool reconstructed = false;
     gf::ReconState rs;

@@ -578,13 +578,66 @@ void rfree_select(genfire_job* j)
     j->have_mask = true;
 }
 
+// save_temp snapshots (reconstruct.m:149-159): the candidate P_R(x) of
+// iteration it is written by the pass that forms it into cand[it % 2] (FFT
+// order, box rows); once errK(it) is reduced it replaces the best box when
+// errK(it) < bestErr (reset to +inf at every enforcement stage, :123).
+bool snap_on(const genfire_job* j) { return j->snap_L > 0 && j->snap_fn != nullptr; }
+float* snap_cand(const genfire_job* j, int it) { return const_cast<float*>(j->cand[it & 1].as<float>()); }
+
+__global__ void k_best_reset(double* best_err) { *best_err = INFINITY; }
+
+__global__ void k_best_decide(const float* errK, int slot, double* best_err, int* flag)
+{
+    const double e = (double)errK[slot];
+    const int better = e < *best_err;
+    if (better) *best_err = e;
+    *flag = better;
+}
+
+// best_box = crop of cand when *flag (the centred Dim box of the FFT-order grid)
+__global__ void k_best_crop(const int* flag, const float* __restrict__ cand, int n1, int n2, int n3,
+                            float* __restrict__ box, int d1, int d2, int d3)
+{
+    if (!*flag) return;
+    const int64_t total = (int64_t)d1 * d2 * d3;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(t % d1), b = (int)((t / d1) % d2), c = (int)(t / ((int64_t)d1 * d2));
+        int s1 = a - d1 / 2, s2 = b - d2 / 2, s3 = c - d3 / 2;
+        s1 += s1 < 0 ? n1 : 0;
+        s2 += s2 < 0 ? n2 : 0;
+        s3 += s3 < 0 ? n3 : 0;
+        box[t] = cand[(int64_t)s1 + (int64_t)n1 * (s2 + (int64_t)n2 * s3)];
+    }
+}
+
+void snap_update(genfire_job* j, int it)
+{
+    hipStream_t st = j->run_st;
+    k_best_decide<<<1, 1, 0, st>>>(j->errK.as<float>(), it - 1, j->best_err.as<double>(), j->best_flag.as<int>());
+    GF_LAUNCH_CHECK();
+    const int64_t total = box_size(j);
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+    k_best_crop<<<blocks, 256, 0, st>>>(j->best_flag.as<int>(), snap_cand(j, it), j->n1, j->n2, j->n1,
+                                        j->best_box.as<float>(), j->dim1, j->dim2, j->dim1);
+    GF_LAUNCH_CHECK();
+    if (it % j->snap_L == 0) {
+        j->snap_h.resize((size_t)total);
+        GF_HIP(hipMemcpyAsync(j->snap_h.data(), j->best_box.p, sizeof(float) * total, hipMemcpyDeviceToHost, st));
+        GF_HIP(hipStreamSynchronize(st));
+        j->snap_fn(j->snap_user, it, j->snap_h.data());
+    }
+}
+
 HalfIterArgs p5_args(const genfire_job* j, int it)
 {
     const ReconState& r = j->rs;
     HalfIterArgs ha = r.ha;
     const bool mon_it = r.monitor && (it % r.L == 0);
     ha.pf_out = mon_it ? const_cast<float*>(j->pfbuf.as<float>()) : nullptr;
-    ha.best = (it + 1 == r.N) ? const_cast<float*>(j->best.as<float>()) : nullptr;
+    ha.best = snap_on(j) ? snap_cand(j, it + 1)
+                         : (it + 1 == r.N) ? const_cast<float*>(j->best.as<float>()) : nullptr;
     return ha;
 }
 
@@ -810,7 +863,7 @@ void recon_pre(genfire_job* j, int it)
     // the iteration before.
     if (r.half && it == 1) {
         HalfIterArgs ha = r.ha;
-        ha.best = (r.N == 1) ? j->best.as<float>() : nullptr;
+        ha.best = snap_on(j) ? snap_cand(j, 1) : (r.N == 1) ? j->best.as<float>() : nullptr;
         ha.pf_out = nullptr;
         half_first(ha, st);
     }
@@ -859,7 +912,7 @@ void recon_body(genfire_job* j, int it)
     } else {
         IterArgs ia = r.ia;
         const bool mon_it = r.monitor && (it % r.L == 0);
-        ia.best = (it == r.N) ? j->best.as<float>() : nullptr;
+        ia.best = snap_on(j) ? snap_cand(j, it) : (it == r.N) ? j->best.as<float>() : nullptr;
         ia.pf_out = (mon_it && r.dr) ? j->pfbuf.as<float>() : nullptr;
         iter_first(ia, st);
         kstat_begin(j, GENFIRE_KSTAT_FFT_PASS, &e_pass);
@@ -915,7 +968,10 @@ void recon_end(genfire_job* j)
     GF_HIP(hipEventRecord(r.loop.b, st));
     // reconstruction = My_stripzero(fftshift(best), [Dim1 Dim2 Dim1]) (:192)
     j->recon.alloc(sizeof(float) * box_size(j));
-    crop_centre(j->best.p, 0, j->n1, j->n2, j->n1, j->recon.as<float>(), j->dim1, j->dim2, j->dim1, st);
+    if (snap_on(j))   // the best box kept for the snapshots: P_R(x) of the last iteration (a stage at N)
+        GF_HIP(hipMemcpyAsync(j->recon.p, j->best_box.p, sizeof(float) * box_size(j), hipMemcpyDeviceToDevice, st));
+    else
+        crop_centre(j->best.p, 0, j->n1, j->n2, j->n1, j->recon.as<float>(), j->dim1, j->dim2, j->dim1, st);
     j->errK_h.assign(N, 0.0f);
     GF_HIP(hipMemcpyAsync(j->errK_h.data(), j->errK.p, sizeof(float) * N, hipMemcpyDeviceToHost, st));
     j->Rfree_h.assign((size_t)j->rf_bins * N, -1.0f);
@@ -964,8 +1020,16 @@ void do_reconstruct(genfire_job* j)
     r.drop_graph();
     r.graph_launches = 0;
     const char* ge = std::getenv("GENFIRE_GRAPHS");
-    const bool graphs = !j->prof && !(ge && std::string(ge) == "0");
+    const bool snap = snap_on(j);
+    const bool graphs = !j->prof && !snap && !(ge && std::string(ge) == "0");
     hipStream_t st = j->run_st;
+    if (snap) {
+        j->cand[0].alloc(sizeof(float) * full_size(j));
+        j->cand[1].alloc(sizeof(float) * full_size(j));
+        j->best_box.alloc(sizeof(float) * box_size(j));
+        j->best_err.alloc(sizeof(double));
+        j->best_flag.alloc(sizeof(int));
+    }
     std::pair<hipEvent_t, hipEvent_t> e_it;
     for (int it = 1; it <= r.N; ++it) {
         const bool stage = r.cur < r.sched.size() && it == r.sched[r.cur];
@@ -985,11 +1049,16 @@ void do_reconstruct(genfire_job* j)
             GF_HIP(hipGraphLaunch(r.gexec, st));
             ++r.graph_launches;
         } else {
+            if (snap && stage) {
+                k_best_reset<<<1, 1, 0, st>>>(j->best_err.as<double>());
+                GF_LAUNCH_CHECK();
+            }
             recon_pre(j, it);
             recon_body(j, it);
             if (!r.half)
                 iter_errk_reduce_next(j->err_part.as<double>(), r.nparts, r.den, j->errK.as<float>(),
                                       j->errctr.as<int>(), st);
+            if (snap) snap_update(j, it);
         }
         kstat_end(j, GENFIRE_KSTAT_ITERATION, e_it);
         if (j->rs.monitor && (it % j->rs.L == 0) && it / j->rs.L - 1 < j->rs.nR) {

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -73,6 +74,9 @@ struct genfire_server {
     std::deque<std::function<void()>> tasks;
     int64_t used = 0, peak_used = 0, next_id = 1;
     bool stop = false, mem_blocked = false;
+    // GENFIRE_SERVER_FAIL_AT=k (tests of the failure path): the scheduler's k-th
+    // group iteration raises a HIP-class error after its launches are queued, once
+    int64_t fail_at = 0;
     std::thread sched;
     std::vector<std::thread> workers;
 
@@ -230,7 +234,11 @@ void genfire_server::finish(genfire_request* r)
     }
 }
 
-// One iteration for every group whose stream has room; true if anything was launched.
+// One iteration for every group whose stream has room; true if anything was
+// launched.  A group's requests advance (it, FINISHING) only after every launch
+// and event record of their iteration succeeded; on any error the group's
+// stream is drained and its requests fail with the error, so no request stays
+// ITERATING and none is iterated past numIterations.
 bool genfire_server::advance_groups()
 {
     std::map<std::tuple<int, int, int, int>, std::vector<genfire_request*>> ready;
@@ -241,27 +249,29 @@ bool genfire_server::advance_groups()
     }
     bool launched = false;
     for (auto& kv : ready) {
-        Group& g = groups[kv.first];
-        if (!g.st) GF_HIP(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
-        while (!g.inflight.empty() && hipEventQuery(g.inflight.front()) == hipSuccess) {
-            g.pool.push_back(g.inflight.front());
-            g.inflight.pop_front();
-        }
-        if ((int)g.inflight.size() >= cfg.lookahead) continue;
         std::vector<genfire_request*>& rs = kv.second;
-        std::vector<genfire_job*> js;
-        std::vector<int> its;
-        for (genfire_request* r : rs) {
-            genfire_job* j = r->job.get();
-            if (!r->joined) {
-                GF_HIP(hipStreamWaitEvent(g.st, r->ev_ready, 0));
-                j->run_st = g.st;
-                r->joined = true;
-            }
-            js.push_back(j);
-            its.push_back(r->it);
-        }
+        Group& g = groups[kv.first];
+        int code = GENFIRE_OK;
+        std::string msg;
         try {
+            if (!g.st) GF_HIP(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
+            while (!g.inflight.empty() && hipEventQuery(g.inflight.front()) == hipSuccess) {
+                g.pool.push_back(g.inflight.front());
+                g.inflight.pop_front();
+            }
+            if ((int)g.inflight.size() >= cfg.lookahead) continue;
+            std::vector<genfire_job*> js;
+            std::vector<int> its;
+            for (genfire_request* r : rs) {
+                genfire_job* j = r->job.get();
+                if (!r->joined) {
+                    GF_HIP(hipStreamWaitEvent(g.st, r->ev_ready, 0));
+                    j->run_st = g.st;
+                    r->joined = true;
+                }
+                js.push_back(j);
+                its.push_back(r->it);
+            }
             for (size_t i = 0; i < js.size(); ++i) recon_pre(js[i], its[i]);
             if (js[0]->rs.half) {
                 for (size_t c = 0; c < js.size(); c += (size_t)cfg.max_batch) {
@@ -272,29 +282,53 @@ bool genfire_server::advance_groups()
                 for (size_t i = 0; i < js.size(); ++i) recon_body(js[i], its[i]);
             }
             for (size_t i = 0; i < js.size(); ++i) recon_post(js[i], its[i]);
-        } catch (const Error& e) {
-            for (genfire_request* r : rs) fail(r, e.code, e.msg);
-            continue;
-        }
-        hipEvent_t ev;
-        if (!g.pool.empty()) {
-            ev = g.pool.back();
-            g.pool.pop_back();
-        } else {
-            GF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        }
-        GF_HIP(hipEventRecord(ev, g.st));
-        g.inflight.push_back(ev);
-        launched = true;
-        std::lock_guard<std::mutex> lk(mu);
-        ++batched_launches;
-        batched_jobs += (int64_t)rs.size();
-        for (genfire_request* r : rs) {
-            if (++r->it > r->job->rs.N) {
-                GF_HIP(hipEventRecord(r->ev_done, g.st));
+            if (fail_at > 0 && batched_launches + 1 == fail_at) {
+                const int64_t k = fail_at;
+                fail_at = 0;   // once
+                raise(GENFIRE_ERR_HIP, "server: failure injected at scheduler iteration " + std::to_string(k));
+            }
+            std::vector<genfire_request*> done;
+            for (genfire_request* r : rs)
+                if (r->it + 1 > r->job->rs.N) {
+                    GF_HIP(hipEventRecord(r->ev_done, g.st));
+                    done.push_back(r);
+                }
+            hipEvent_t ev;
+            if (!g.pool.empty()) {
+                ev = g.pool.back();
+                g.pool.pop_back();
+            } else {
+                GF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            }
+            const hipError_t er = hipEventRecord(ev, g.st);
+            if (er != hipSuccess) {
+                g.pool.push_back(ev);
+                GF_HIP(er);
+            }
+            g.inflight.push_back(ev);
+            launched = true;
+            std::lock_guard<std::mutex> lk(mu);
+            ++batched_launches;
+            batched_jobs += (int64_t)rs.size();
+            for (genfire_request* r : rs) ++r->it;
+            for (genfire_request* r : done) {
                 r->state = GENFIRE_REQ_FINISHING;
                 post_locked([this, r] { finish(r); });
             }
+        } catch (const Error& e) {
+            code = e.code;
+            msg = e.msg;
+        } catch (const std::bad_alloc&) {
+            code = GENFIRE_ERR_NOMEM;
+            msg = "host allocation failed";
+        } catch (const std::exception& e) {
+            code = GENFIRE_ERR_ARG;
+            msg = e.what();
+        }
+        if (code != GENFIRE_OK) {
+            // launches already queued for these jobs finish before their memory is freed
+            if (g.st) (void)hipStreamSynchronize(g.st);
+            for (genfire_request* r : rs) fail(r, code, msg);
         }
     }
     return launched;
@@ -312,7 +346,7 @@ void genfire_server::sched_loop()
         bool launched = false;
         try {
             launched = advance_groups();
-        } catch (const Error&) {
+        } catch (const std::exception&) {   // only the grouping itself can throw here (host allocation)
             launched = false;
         }
         std::unique_lock<std::mutex> lk(mu);
@@ -365,6 +399,7 @@ int genfire_server_create(genfire_ctx* ctx, const genfire_server_config* cfg, ge
             GF_HIP(hipMemGetInfo(&fr, &tot));
             c.hbm_budget_bytes = (int64_t)((double)fr * 0.9);
         }
+        if (const char* fa = std::getenv("GENFIRE_SERVER_FAIL_AT")) s->fail_at = std::atoll(fa);
         genfire_server* sp = s.get();
         for (int i = 0; i < c.prep_threads + 1; ++i) sp->workers.emplace_back([sp] { sp->worker_loop(); });
         sp->sched = std::thread([sp] { sp->sched_loop(); });

@@ -304,6 +304,19 @@ class GENFIRE_Reconstructor:
             mk = self._measuredK if self._measuredK is not None else job.measuredK()
             job.set_measuredK(mk, self.measuredK_mask)
             self._measuredK_dirty = False
+        if self.save_temp == 1:
+            # reconstruct.m:155-159: temp_reconstruction = My_stripzero(fftshift(best)) saved
+            # to [saveFilename num2str(iterationNum) '.mat'] every save_loopLength iterations
+            import scipy.io
+            d = os.path.dirname(self.saveFilename)
+            if d:
+                os.makedirs(d, exist_ok=True)
+
+            def save(it, vol):
+                scipy.io.savemat(f"{self.saveFilename}{it}.mat", {"temp_reconstruction": vol})
+            job.set_snapshots(int(self.save_loopLength), save)
+        else:
+            job.set_snapshots(0)
         job.reconstruct()
         self.reconstruction = job.reconstruction()
         self.errK = job.errK()

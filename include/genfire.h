@@ -167,6 +167,18 @@ int genfire_job_get_errR(genfire_job* job, float* errR, double* Rarr, double* Ra
  * -1 where a shell has no measured point.  Requires calculate_Rfree. */
 int genfire_job_get_Rfree(genfire_job* job, float* Rfree);
 
+/* save_temp (reconstruct.m:155-159): with save_loopLength > 0 and fn set, every
+ * save_loopLength-th iteration of genfire_job_reconstruct calls
+ * fn(user, iteration, reconstruction) with the reconstruction the reference
+ * would save there: My_stripzero(fftshift(best)) (Dim1 x Dim2 x Dim1 float,
+ * host memory valid during the call), best being P_R(x) of the iteration with
+ * the lowest errK since the last enforcement stage (:149-153).  The reference
+ * writes it to [saveFilename num2str(iteration) '.mat']; the caller's fn does
+ * what it wants with it.  save_loopLength <= 0 or fn NULL turns snapshots off.
+ * With snapshots on, every iteration launches directly (no graph replay). */
+typedef void (*genfire_snapshot_fn)(void* user, int32_t iteration, const float* reconstruction);
+int genfire_job_set_snapshots(genfire_job* job, int32_t save_loopLength, genfire_snapshot_fn fn, void* user);
+
 /* Asynchronous form for serving: grid + reconstruct enqueued on the job's
  * worker; poll returns 1 in *done when finished; wait blocks. */
 int genfire_job_submit(genfire_job* job);

@@ -679,6 +679,7 @@ class ReconResult:
     Rarr2_record: np.ndarray | None = None
     last_iterate: np.ndarray | None = None
     Rfree: np.ndarray | None = None
+    snapshots: list | None = None
 
 
 _U64 = np.uint64
@@ -727,7 +728,8 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
                 dim2: int, num_iterations: int, mode: int = 3, positivity: bool = True,
                 support_on: bool = True, algorithm: str = "genfire", precision: str = "fp32",
                 initial_object: np.ndarray | None = None, monitor_R: bool = False,
-                loop_length: int = 10, projections=None, angles=None, rfree=None) -> ReconResult:
+                loop_length: int = 10, projections=None, angles=None, rfree=None,
+                save_every: int = 0) -> ReconResult:
     """reconstruct.m:28-194 (GENFIRE, algorithm='genfire').
 
     algorithm='dr' is the library's Douglas-Rachford variant composed of the same two
@@ -741,6 +743,8 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
     precision: 'fp32' (MATLAB single, as when the support constraint is on) or 'fp64'.
     rfree: None or (num_bins, percent, seed): withhold points per shell
     (rfree_select, :46-69) and record Rfree_complex (:137-147) in ``Rfree``.
+    save_every: save_loopLength of save_temp (:155-159); ``snapshots`` then holds
+    (iteration, My_stripzero(fftshift(best))) for every multiple of it.
     """
     cdt = np.complex64 if precision == "fp32" else np.complex128
     rdt = np.float32 if precision == "fp32" else np.float64
@@ -782,6 +786,7 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
     vol_ind = volumn_index(shape, (dim1, dim2, dim1))
     best_err = 1e30
     recon = None
+    snapshots = []
     c_sh = None
     c_val = None
     for it in range(1, num_iterations + 1):                                          # :117
@@ -818,6 +823,8 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
         if errK[it - 1] < best_err:                                                   # :149-153
             best_err = errK[it - 1]
             recon = r.copy()
+        if save_every > 0 and it % save_every == 0:                                   # :155-159
+            snapshots.append((it, np.asarray(stripzero(np.fft.fftshift(recon), (dim1, dim2, dim1)), np.float32)))
         kf = kf.copy()
         kf[c_sh] = c_val                                                              # :163
         k = kf.reshape(shape, order="F")
@@ -837,4 +844,4 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
                        errR=errR if monitor_R else None,
                        Rarr_record=Rrec if monitor_R else None,
                        Rarr2_record=R2rec if monitor_R else None,
-                       last_iterate=x, Rfree=Rfree if rf is not None else None)
+                       last_iterate=x, Rfree=Rfree if rf is not None else None, snapshots=snapshots)

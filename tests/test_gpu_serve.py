@@ -159,3 +159,35 @@ def test_invalid_request_fails_at_submission(ctx, mix_inputs):
     st = srv.stats()
     assert st["submitted"] == 0
     srv.close()
+
+
+def test_failed_iteration_fails_its_group(ctx, mix_inputs, monkeypatch):
+    """An error inside the scheduler's iteration of a group (forced from the
+    host: GENFIRE_SERVER_FAIL_AT makes its third group iteration raise after the
+    launches are queued) fails every request of that group with the error: the
+    waits return, nothing stays ITERATING, and the server goes on serving."""
+    print("[serve] injected scheduler failure", flush=True)
+    monkeypatch.setenv("GENFIRE_SERVER_FAIL_AT", "3")
+    cfg = _lib.default_server_config()
+    cfg.max_active = 4
+    srv = _lib.Server(ctx, cfg)
+    monkeypatch.delenv("GENFIRE_SERVER_FAIL_AT")
+    projs, ang, sup, p = mix_inputs[0]
+    ids = [srv.submit(p, projs, ang, sup) for _ in range(2)]
+    srv.drain()
+    states = [srv.status(i) for i in ids]
+    assert _lib.REQ_FAILED in states, states
+    assert all(s in (_lib.REQ_FAILED, _lib.REQ_DONE) for s in states), states
+    with pytest.raises(_lib.GenfireError, match="failure injected"):
+        srv.result(ids[states.index(_lib.REQ_FAILED)])
+    st = srv.stats()
+    assert st["failed"] >= 1 and st["failed"] + st["completed"] == 2, st
+    # the server keeps serving: a later request completes and equals the job alone
+    rid = srv.submit(p, projs, ang, sup)
+    srv.wait(rid)
+    assert srv.status(rid) == _lib.REQ_DONE
+    rec, ek, er, _ = srv.result(rid)
+    rec0, ek0, er0 = _alone(ctx, projs, ang, sup, p)
+    np.testing.assert_array_equal(rec, rec0)
+    np.testing.assert_array_equal(ek, ek0)
+    srv.close()

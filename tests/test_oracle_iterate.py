@@ -280,3 +280,32 @@ def test_rfactor_monitor_records_scaled_iterate(O):
     np.testing.assert_allclose(r.Rarr_record, abs(1 - c), atol=1e-5)
     np.testing.assert_allclose(r.Rarr2_record, abs(1 - c), atol=1e-5)
     np.testing.assert_allclose(r.errR, abs(1 - c), atol=1e-5)
+
+
+@pytest.mark.parametrize("alg,mode", [("genfire", 3), ("dr", 1)])
+def test_snapshots_follow_best_since_stage(O, alg, mode):
+    """save_temp (reconstruct.m:149-159): the snapshot of iteration t is the
+    iterate whose errK is the first minimum since the last enforcement stage
+    (bestErr reset at :123, replaced only on a strictly lower errK); the last
+    snapshot is the returned reconstruction."""
+    obj, M = _obj(seed=7)
+    meas = np.random.default_rng(9).random(M.shape) < 0.45
+    N = 17
+    r = O.reconstruct(np.where(meas, M, 0), meas, obj > 0, 8, 8, N, mode=mode, algorithm=alg, precision="fp64",
+                      save_every=1)
+    snaps = dict(r.snapshots)
+    assert sorted(snaps) == list(range(1, N + 1))
+    stages = set(O.cutoff_schedule(N, len(O.delay_indicators(mode))).tolist())
+    start = 1
+    for t in range(1, N + 1):
+        if t in stages:
+            start = t
+        seg = r.errK[start - 1:t]
+        b = start + int(np.argmin(seg))          # first occurrence of the minimum
+        np.testing.assert_array_equal(snaps[t], snaps[b])
+    np.testing.assert_array_equal(snaps[N], r.reconstruction)
+    r3 = O.reconstruct(np.where(meas, M, 0), meas, obj > 0, 8, 8, N, mode=mode, algorithm=alg, precision="fp64",
+                       save_every=3)
+    assert [t for t, _ in r3.snapshots] == [3, 6, 9, 12, 15]
+    for t, v in r3.snapshots:
+        np.testing.assert_array_equal(v, snaps[t])
