@@ -345,5 +345,46 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row,
 }
 
 
+// Orders the LDS accesses of one wave: a pencil whose T lanes lie in one wave
+// (T divides 64) exchanges its elements through LDS with no workgroup barrier.
+// A wave's LDS instructions execute in order; the fences keep the compiler from
+// moving its accesses across this point.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// reg_fft for pencils that each lie in one wave: the stage exchanges are
+// ordered per wave, with no workgroup barrier.
+template <int N, bool INV>
+__device__ __forceinline__ void reg_fft_wave(float2 (&v)[RegPlan<N>::E], float2* row, int j, int p, const float2* twt)
+{
+    using PL = RegPlan<N>;
+    constexpr int E = PL::E, R0 = PL::R0, R1 = PL::R1, R2 = PL::R2, R3 = PL::R3;
+    static_assert(64 % (N / E) == 0, "a pencil's lanes must lie in one wave");
+    reg_stage<N, E, R0, 1, INV>(v, j, twt);
+    reg_write<N, E, R0, 1>(row, v, j, p);
+    wave_sync();
+    reg_read<N, E, R1>(row, v, j, p);
+    wave_sync();
+    reg_stage<N, E, R1, R0, INV>(v, j, twt);
+    if constexpr (R2 > 1) {
+        reg_write<N, E, R1, R0>(row, v, j, p);
+        wave_sync();
+        reg_read<N, E, R2>(row, v, j, p);
+        wave_sync();
+        reg_stage<N, E, R2, R0 * R1, INV>(v, j, twt + (R1 - 1) * R0);
+    }
+    if constexpr (R3 > 1) {
+        reg_write<N, E, R2, R0 * R1>(row, v, j, p);
+        wave_sync();
+        reg_read<N, E, R3>(row, v, j, p);
+        wave_sync();
+        reg_stage<N, E, R3, R0 * R1 * R2, INV>(v, j, twt + (R1 - 1) * R0 + (R2 - 1) * R0 * R1);
+    }
+}
+
 }  // namespace fftc
 }  // namespace gf

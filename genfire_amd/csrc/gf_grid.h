@@ -189,17 +189,22 @@ struct HalfIterArgs {
     int dmode;
     float2* csr_d;
     const int64_t* csr_mate;
-    // Set on the y-inverse launch of an iteration only: its first workgroup
-    // reduces the z pass's errK partials (err_n of them, fixed order) into
-    // errk[*errctr] = sum / err_den and advances *errctr; with dherm its first
-    // workgroups also replace D on the kx = 0 and n1/2 planes by its Hermitian
-    // part.  Null / 0 on every other launch.
+    // errK of the iteration, in two fixed-order stages: on the y-inverse launch
+    // (err_stage 1) workgroup c < kErrChunks (or c mod the grid) sums chunk c of
+    // the z pass's err_n partials into err_chunk[c]; on the launch of the
+    // iteration's last x pass (err_stage 2) workgroup 0 sets
+    // errk[*errctr] = sum(err_chunk) / err_den and advances *errctr.  With dherm
+    // the y-inverse launch's first workgroups also replace D on the kx = 0 and
+    // n1/2 planes by its Hermitian part.  0 / null on every other launch.
     float* errk;
     int* errctr;
+    double* err_chunk;
     double err_den;
     int64_t err_n;
+    int err_stage;
     int dherm;
 };
+constexpr int kErrChunks = 64;
 // Rfree[s + nbins * (*ctr)] = sum of terms of shell s (entries order[shell_off[s]
 // .. shell_off[s+1]), fixed order) / den[s], or -1 for an empty shell; *ctr is
 // the iteration index the errK reduction advances afterwards.

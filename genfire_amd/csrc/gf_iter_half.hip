@@ -171,7 +171,8 @@ __device__ __forceinline__ int phase_tid()
     return t;
 }
 
-// Forward M-point FFT of the tile's rows (v in input slots), the R2C split
+// Forward M-point FFT of the tile's rows (v in input slots; a row's lanes lie in
+// one wave and the twiddle tables are filled and ordered by the caller), the R2C split
 //   X[k] = (Z[k] + conj Z[M-k])/2 + W_N^k (Z[k] - conj Z[M-k])/(2i), k = 0..M,
 // and the transposed store to layout A.
 template <int N>
@@ -185,7 +186,7 @@ __device__ __forceinline__ void r2c_store(const HalfIterArgs& a, float2* lds, fl
     const int t = phase_tid(), j = t % T, p = t / T;
     const int n2 = a.n2, h = M + 1;
     float2* rowl = lds + p * M;
-    reg_fft<M, false>(v, rowl, j, p, twM);
+    reg_fft_wave<M, false>(v, rowl, j, p, twM);
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
@@ -271,6 +272,7 @@ __global__ __launch_bounds__(x_threads<N / 2>()) void k_half_first(HalfBatch hb,
     float2 v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) v[in_slot<M>(s)] = p1_value(a, xv[s], s2[s], row + 2 * (j + T * s), in[s] != 0);
+    __syncthreads();   // the twiddle tables
     r2c_store<N>(a, lds, v, i2_0, i3, twlM, twlN);
 }
 
@@ -300,8 +302,9 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
             const int y = j + T * i + r * (N / R0);
             v[i * R0 + r] = (z_in && inb(y, N, a.by)) ? src[y] : make_float2(0.0f, 0.0f);
         }
+    __syncthreads();   // the twiddle table; the pencils' transforms are wave-local
     float2* rowl = lds + p * N;
-    reg_fft<N, false>(v, rowl, j, p, twl);
+    reg_fft_wave<N, false>(v, rowl, j, p, twl);
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
@@ -332,20 +335,27 @@ template <int N> constexpr bool zhalf_ok() { return RegPlan<N>::R0 >= 4 && reg_l
 // [0, N/4) u [3N/4, N) when r < R/4 or r >= 3R/4
 template <int R> __device__ __forceinline__ constexpr bool half_slot(int r) { return r < R / 4 || r >= 3 * R / 4; }
 
+// P3: the z pencils of layout B, contiguous.  A workgroup takes kZP pencils (one
+// kx, kZP consecutive ky); the T lanes of a pencil lie in one wave (T divides 64),
+// so each pencil's transforms, its list entries (errK terms, replacement, DR
+// delta) and its LDS row are handled by its own lanes with wave-ordered LDS
+// exchanges: the only workgroup barrier orders the shared twiddle table.  The
+// errK partials are one per wave, err_part[blockIdx.x * W + w].
 template <int N, bool HALF>
 __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch hb, const float2* __restrict__ tw)
 {
     const HalfIterArgs& a = hb.job[blockIdx.y];   // the job of this launch row
     using PL = RegPlan<N>;
     constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
-    constexpr int NT = zfused_threads<N>();
+    constexpr int NT = zfused_threads<N>(), W = NT / 64;
+    static_assert(64 % T == 0 && NT % 64 == 0, "a pencil's lanes lie in one wave");
     extern __shared__ float2 lds[];
     const int t = threadIdx.x, j = t % T, p = t / T;
     const int n2 = a.n2, n3 = N;
     const int tpr = n2 / kZP;
     const int kx = blockIdx.x / tpr, ky_0 = (blockIdx.x % tpr) * kZP;
-    // LDS: the tile, one errK slot per wave, the twiddle table
-    float2* twl = lds + kZP * N + NT / 64;
+    // LDS: the tile, then the twiddle table
+    float2* twl = lds + kZP * N;
     reg_tw_fill<N>(twl, tw, t, NT);
     float2* pen = a.B + (int64_t)n3 * ((ky_0 + p) + (int64_t)n2 * kx);
     float2 v[E];
@@ -356,12 +366,12 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             const int z = j + T * i + r * (N / R0);
             v[i * R0 + r] = (HALF ? half_slot<R0>(r) : inb(z, N, a.bz)) ? pen[z] : make_float2(0.0f, 0.0f);
         }
-    // The tile's list range and each lane's first entry are loaded before the
-    // forward transform, which hides their latency (a tile has fewer entries
+    // The pencil's list range and each lane's first entry are loaded before the
+    // forward transform, which hides their latency (a pencil has fewer entries
     // than lanes in most of the grid).
-    const int64_t r0 = (int64_t)ky_0 + (int64_t)n2 * kx;
-    const int beg = (int)a.csr_off[r0], end = (int)a.csr_off[r0 + kZP];   // < 2^31 entries
-    const int e0 = beg + t;
+    const int64_t rp = (int64_t)(ky_0 + p) + (int64_t)n2 * kx;
+    const int beg = (int)a.csr_off[rp], end = (int)a.csr_off[rp + 1];   // < 2^31 entries
+    const int e0 = beg + j;
     uint32_t m0 = 0;
     float2 mg0 = make_float2(0.0f, 0.0f), V0 = mg0, d0 = mg0;
     if (e0 < end) {
@@ -370,48 +380,47 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
         V0 = a.csr_val[e0];
         if (a.dmode) d0 = a.csr_d[e0];
     }
+    __syncthreads();   // the twiddle table
     float2* rowl = lds + p * N;
-    reg_fft<N, false>(v, rowl, j, p, twl);
+    reg_fft_wave<N, false>(v, rowl, j, p, twl);
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
         for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
-    __syncthreads();
+    wave_sync();
     // DR on the box: the transform of the box part minus D' of the last
     // iteration is the transform of the iterate's reflection.  The Rfree terms
     // read it from LDS; otherwise the errK loop subtracts in registers.
     if (a.dmode && a.rf_off) {
-        for (int e = e0; e < end; e += NT) {
+        for (int e = e0; e < end; e += T) {
             const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
-            const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
-            float2* slot = lds + pl * N + swz(z, pl);
+            float2* slot = rowl + swz((int)(m & 0x7FFu), p);
             float2 dv = d0;   // by value: a select of two float2 lvalues selects addresses
             if (e != e0) dv = a.csr_d[e];
             const float2 kv = *slot;
             *slot = make_float2(kv.x - dv.x, kv.y - dv.y);
         }
-        __syncthreads();
+        wave_sync();
     }
     // Rfree terms (reconstruct.m:137-147) on the transform before replacement
     if (a.rf_off) {
-        const int rb = (int)a.rf_off[r0], re = (int)a.rf_off[r0 + kZP];
-        for (int e = rb + t; e < re; e += NT) {
+        const int rb = (int)a.rf_off[rp], re = (int)a.rf_off[rp + 1];
+        for (int e = rb + j; e < re; e += T) {
             const uint32_t m = a.rf_meta[e];
-            const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
-            const float2 kv = lds[pl * N + swz(z, pl)];
+            const float2 kv = rowl[swz((int)(m & 0x7FFu), p)];
             const float2 V = a.rf_val[e];
             const float dx = kv.x - V.x, dy = kv.y - V.y;
             a.rf_terms[e] = sqrtf(dx * dx + dy * dy);
         }
-        __syncthreads();
+        wave_sync();
     }
     // errK numerator (:135) and replacement (:163) from the half-spectrum lists;
-    // the numerator is reduced here (LDS past the tile) so that it is not live
-    // through the inverse transform
+    // the numerator is reduced here, per wave, so that it is not live through the
+    // inverse transform
     float2 dn0 = make_float2(0.0f, 0.0f);   // the new delta of the lane's first entry
     {
         float err = 0.0f;   // at most a few entries per lane
-        for (int e = e0; e < end; e += NT) {
+        for (int e = e0; e < end; e += T) {
             uint32_t m = m0;
             float2 mg = mg0, V = V0;
             if (e != e0) {
@@ -419,8 +428,7 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
                 mg = a.csr_mag[e];
                 V = a.csr_val[e];
             }
-            const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
-            float2* slot = lds + pl * N + swz(z, pl);
+            float2* slot = rowl + swz((int)(m & 0x7FFu), p);
             float2 kv = *slot;
             if (a.dmode && !a.rf_off) {
                 float2 dv = d0;
@@ -443,35 +451,27 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
         }
         double de = (double)err;
         for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off, 64);
-        double* red = reinterpret_cast<double*>(lds + kZP * N);
-        if ((t & 63) == 0) red[t >> 6] = de;
-        __syncthreads();
-        if (t == 0) {
-            double sum = 0.0;
-            for (int w = 0; w < NT / 64; ++w) sum += red[w];
-            a.err_part[blockIdx.x] = sum;
-        }
+        if ((t & 63) == 0) a.err_part[(int64_t)blockIdx.x * W + (t >> 6)] = de;
     }
     // DR on the box: the inverse transform takes the sparse D
     if (a.dmode) {
-        for (int i = t; i < kZP * N; i += NT) lds[i] = make_float2(0.0f, 0.0f);
-        __syncthreads();
-        for (int e = e0; e < end; e += NT) {
+        for (int i = j; i < N; i += T) rowl[i] = make_float2(0.0f, 0.0f);
+        wave_sync();
+        for (int e = e0; e < end; e += T) {
             const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
-            const int z = (int)(m & 0x7FFu), pl = (int)((m >> 11) & 0x7FFu) - ky_0;
             float2 dv = dn0;
             if (e != e0) dv = a.csr_d[e];
-            lds[pl * N + swz(z, pl)] = dv;
+            rowl[swz((int)(m & 0x7FFu), p)] = dv;
         }
-        __syncthreads();
     }
+    wave_sync();
     // the inverse transform takes fresh lane indices and its own address of the
     // twiddle table (opaque_zero) so that nothing of the forward one stays live
     const int t2 = phase_tid(), j2 = t2 % T, p2 = t2 / T;
     float2* rowl2 = lds + p2 * N;
     reg_read<N, E, R0>(rowl2, v, j2, p2);
-    __syncthreads();
-    reg_fft<N, true>(v, rowl2, j2, p2, lds + (kZP * N + NT / 64 + opaque_zero()));
+    wave_sync();
+    reg_fft_wave<N, true>(v, rowl2, j2, p2, lds + (kZP * N + opaque_zero()));
     const float sc = 1.0f / (float)N;
     float2* pen2 = a.B + (int64_t)n3 * ((ky_0 + p2) + (int64_t)n2 * kx);
 #pragma unroll
@@ -486,23 +486,37 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
 // ---------------------------------------------------------------------------
 // P4: y inverse, B -> A.  Tile: one kx, 16 consecutive z (strided read).
 // ---------------------------------------------------------------------------
-// errK of the iteration from the z pass's per-tile partials: lane sums over a
-// fixed stride, wave sums, then the waves in order (one workgroup).
-__device__ void errk_reduce(const HalfIterArgs& a, double* red)
+// errK of the iteration, stage 1 (y-inverse launch): chunk c of the z pass's
+// partials summed into err_chunk[c], lane sums over a fixed stride, wave sums,
+// then the waves in order.
+__device__ void errk_chunks(const HalfIterArgs& a, double* red)
 {
     const int t = threadIdx.x, nt = blockDim.x;
-    double s = 0.0;
-    for (int64_t i = t; i < a.err_n; i += nt) s += a.err_part[i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((t & 63) == 0) red[t >> 6] = s;
-    __syncthreads();
-    if (t == 0) {
-        double sum = 0.0;
-        for (int w = 0; w < nt / 64; ++w) sum += red[w];
-        a.errk[*a.errctr] = (float)(sum / a.err_den);
-        *a.errctr += 1;
+    const int64_t len = (a.err_n + kErrChunks - 1) / kErrChunks;
+    for (int c = blockIdx.x; c < kErrChunks; c += gridDim.x) {
+        const int64_t b = (int64_t)c * len, e = min(a.err_n, b + len);
+        double s = 0.0;
+        for (int64_t i = b + t; i < e; i += nt) s += a.err_part[i];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if ((t & 63) == 0) red[t >> 6] = s;
+        __syncthreads();
+        if (t == 0) {
+            double sum = 0.0;
+            for (int w = 0; w < nt / 64; ++w) sum += red[w];
+            a.err_chunk[c] = sum;
+        }
+        __syncthreads();   // red is the tile's LDS
     }
-    __syncthreads();   // red is the tile's LDS
+}
+
+// errK of the iteration, stage 2 (the last x pass's launch, one lane).
+__device__ __forceinline__ void errk_final(const HalfIterArgs& a)
+{
+    if (a.err_stage != 2 || blockIdx.x != 0 || threadIdx.x != 0) return;
+    double sum = 0.0;
+    for (int c = 0; c < kErrChunks; ++c) sum += a.err_chunk[c];
+    a.errk[*a.errctr] = (float)(sum / a.err_den);
+    *a.errctr += 1;
 }
 
 // D on the kx = 0 and n1/2 planes -> its Hermitian part (the mate of entry e on
@@ -535,9 +549,9 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     using PL = RegPlan<N>;
     constexpr int E = PL::E, T = N / E, R0 = PL::R0, RL = reg_last_radix<N>();
     extern __shared__ float2 lds[];
-    // the iteration's errK and D's Hermitian part: both need the whole z pass,
-    // which this launch follows, and nothing before the next z pass reads them
-    if (a.errk && blockIdx.x == 0) errk_reduce(a, reinterpret_cast<double*>(lds));
+    // the iteration's errK chunks and D's Hermitian part: both need the whole z
+    // pass, which this launch follows, and nothing before the next z pass reads them
+    if (a.err_stage == 1) errk_chunks(a, reinterpret_cast<double*>(lds));
     if (a.dherm) {
         const int nb = min(16, (int)gridDim.x);
         if ((int)blockIdx.x < nb) dherm_part(a, (int)blockIdx.x, nb);
@@ -564,8 +578,8 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     float2* rowl = lds + p2 * N;
     float2 v[E];
     reg_read<N, E, R0>(rowl, v, j2, p2);
-    __syncthreads();
-    reg_fft<N, true>(v, rowl, j2, p2, twl);
+    wave_sync();
+    reg_fft_wave<N, true>(v, rowl, j2, p2, twl);
     float2* dst = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p2));
     const float sc = 1.0f / (float)N;
     if (!inb(i3_0 + p2, n3, a.bz)) return;
@@ -622,8 +636,8 @@ __device__ __forceinline__ void load_c2r(const HalfIterArgs& a, float2* lds, flo
             const float2 od = cscale(cmul(csub(xk, xm), conjf2(twN[e])), 0.5f);
             v[i * R0 + r] = make_float2(ev.x - od.y, ev.y + od.x);   // ev + i*od
         }
-    __syncthreads();
-    reg_fft<M, true>(v, rowl, j, p, twM);
+    wave_sync();   // the row's lanes lie in one wave
+    reg_fft_wave<M, true>(v, rowl, j, p, twM);
 }
 
 // The iterate update of the real pair at ad from pf = real(ifftn(k')) (:165):
@@ -671,6 +685,7 @@ __global__ __launch_bounds__(x_threads<N / 2>()) void k_half_last(HalfBatch hb, 
     extern __shared__ float2 lds[];
     const int n2 = a.n2;
     int i3, i2_0;
+    errk_final(a);
     if (!x_tile<M>(a, cy, i3, i2_0)) return;
     float2* twlM = lds + xrows<M>() * (M + 1);
     float2* twlN = twlM + reg_tw_size<M>();
@@ -724,6 +739,7 @@ __global__ __launch_bounds__(x_threads<N / 2>()) void k_half_last_first(HalfBatc
     extern __shared__ float2 lds[];
     const int n2 = a.n2;
     int i3, i2_0;
+    errk_final(a);
     if (!x_tile<M>(a, cy, i3, i2_0)) return;
     float2* twlM = lds + xrows<M>() * (M + 1);
     float2* twlN = twlM + reg_tw_size<M>();
@@ -1065,9 +1081,8 @@ template <int N> void launch_y(const HalfBatch& b, int inverse, hipStream_t st)
 template <int N> void launch_z(const HalfBatch& b, hipStream_t st)
 {
     const HalfIterArgs& a = b.job[0];
-    // tile + one double per wave for the errK partial + twiddle table
-    const size_t lds = (size_t)kZP * N * sizeof(float2) + sizeof(double) * (zfused_threads<N>() / 64) +
-                       (size_t)reg_tw_size<N>() * sizeof(float2);
+    // tile + twiddle table
+    const size_t lds = (size_t)kZP * N * sizeof(float2) + (size_t)reg_tw_size<N>() * sizeof(float2);
     const dim3 grid((a.n1 / 2 + 1) * (a.n2 / kZP), b.count);
     bool half = zhalf_ok<N>();
     for (int i = 0; i < b.count; ++i) half = half && b.job[i].bz == N / 2;
@@ -1096,7 +1111,7 @@ bool half_supported(int n1, int n2, int n3)
     return n1 % 2 == 0 && n1 >= 128 && reg(n1 / 2) && reg(n2) && reg(n3);
 }
 
-int64_t half_partials(int n1, int n2, int n3) { (void)n3; return (int64_t)(n1 / 2 + 1) * (n2 / kZP); }
+
 
 #define GF_HALF_SWITCH(n, CALL)                                               \
     switch (n) {                                                              \
@@ -1114,6 +1129,22 @@ int64_t half_partials(int n1, int n2, int n3) { (void)n3; return (int64_t)(n1 / 
         case 1024: CALL(1024); break;                                         \
         default: raise(GENFIRE_ERR_UNSUPPORTED, "half-spectrum length not supported"); \
     }
+
+
+
+namespace {
+template <int N> int z_waves() { return zfused_threads<N>() / 64; }
+}  // namespace
+
+// errK partials of the z pass: one per wave of each workgroup
+int64_t half_partials(int n1, int n2, int n3)
+{
+    int w = 0;
+#define C(NN) w = z_waves<NN>()
+    GF_HALF_SWITCH(n3, C)
+#undef C
+    return (int64_t)(n1 / 2 + 1) * (n2 / kZP) * w;
+}
 
 // x lengths n1: the complex transform runs at n1/2
 #define GF_HALF_SWITCH_X(n, CALL)                                             \

@@ -113,7 +113,7 @@ struct genfire_job {
     bool gridded = false, have_mask = false;
 
     // iteration
-    gf::DevBuf x, K, S, best, pfbuf, err_part, errK, errctr, den_tmp, den, recon;
+    gf::DevBuf x, K, S, best, pfbuf, err_part, err_chunk, errK, errctr, den_tmp, den, recon;
     gf::DevBuf csr_cnt, csr_off, csr_coff, csr_meta, csr_val, csr_mag, csr_d, csr_mate;
     int64_t nmeas = 0;
     gf::DevBuf vol, vol_y, zy, zx, Rmat, rsums, rout;

@@ -638,6 +638,12 @@ HalfIterArgs p5_args(const genfire_job* j, int it)
     ha.pf_out = mon_it ? const_cast<float*>(j->pfbuf.as<float>()) : nullptr;
     ha.best = snap_on(j) ? snap_cand(j, it + 1)
                          : (it + 1 == r.N) ? const_cast<float*>(j->best.as<float>()) : nullptr;
+    // the iteration's errK from the y-inverse launch's chunk sums
+    ha.errk = const_cast<float*>(j->errK.as<float>());
+    ha.errctr = const_cast<int*>(j->errctr.as<int>());
+    ha.err_chunk = const_cast<double*>(j->err_chunk.as<double>());
+    ha.err_den = r.den;
+    ha.err_stage = 2;
     return ha;
 }
 
@@ -679,6 +685,7 @@ void recon_begin(genfire_job* j)
     GF_CHECK(n1 <= 2048 && n3 <= 2048, GENFIRE_ERR_UNSUPPORTED, "GENFIRE: grid edge above 2048");
     r.nparts = half ? half_partials(n1, n2, n3) : iter_fused_partials(ia);
     j->err_part.alloc(sizeof(double) * r.nparts);
+    j->err_chunk.alloc(sizeof(double) * kErrChunks);
     j->errK.alloc(sizeof(float) * std::max(N, 1));
     j->errctr.alloc(sizeof(int));
     GF_HIP(hipMemsetAsync(j->errctr.p, 0, sizeof(int), st));
@@ -884,10 +891,9 @@ HalfIterArgs y_inv_args(const genfire_job* j)
 {
     const ReconState& r = j->rs;
     HalfIterArgs a = r.ha;
-    a.errk = const_cast<float*>(j->errK.as<float>());
-    a.errctr = const_cast<int*>(j->errctr.as<int>());
-    a.err_den = r.den;
+    a.err_chunk = const_cast<double*>(j->err_chunk.as<double>());
     a.err_n = r.nparts;
+    a.err_stage = 1;
     a.dherm = a.dmode;
     return a;
 }
