@@ -249,7 +249,7 @@ def serve_tilt(ctx, p, w, projs, ang, sup, njobs, precision, max_active):
         "jobs": njobs, "max_active": max_active, "recon_per_hour": njobs / span * 3600.0,
         "makespan_s": span, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
         "batched_launches": st["batched_launches"], "batched_jobs": st["batched_jobs"],
-        "peak_footprint_gb": st["peak_footprint_bytes"] / 1e9,
+        "graph_replays": st["graph_replays"], "peak_footprint_gb": st["peak_footprint_bytes"] / 1e9,
         "rfactor_final": float(errR[0][-1]) if len(errR[0]) else None,
         "max_active_seen": st["max_active_seen"],
         "phases_ms": [{k: round(v, 1) for k, v in r[3].items()} for r in res],
@@ -309,7 +309,8 @@ def serve_mix(ctx, nreq, rate, seed, max_active):
         "requests": nreq, "arrival_rate_per_s": rate, "span_s": span, "completed": st["completed"],
         "failed": st["failed"], "max_active": max_active, "max_active_seen": st["max_active_seen"],
         "batched_launches": st["batched_launches"], "batched_jobs": st["batched_jobs"],
-        "peak_footprint_gb": st["peak_footprint_bytes"] / 1e9, "sizes": sizes,
+        "graph_replays": st["graph_replays"], "peak_footprint_gb": st["peak_footprint_bytes"] / 1e9,
+        "sizes": sizes,
         "data": "synthetic (one seeded blob phantom per size, per-request tilts, counts and Poisson noise)",
         "config": {"workload": "serving mix", "sizes": list(phantom.MIX_SIZES), "projections": [31, 91],
                    "iterations": [50, 200], "algorithm": "dr", "oversampling": 2.0, "seed": seed},
@@ -528,7 +529,7 @@ def main():
         m = serve_mix(ctx, args.mix_summary, args.mix_rate, 7, args.max_active)
         out["serving_mix"] = {k: m[k] for k in ("value", "unit", "p50_ms", "p99_ms", "mean_ms", "requests",
                                                 "arrival_rate_per_s", "completed", "failed", "max_active_seen",
-                                                "batched_launches", "batched_jobs", "sizes")}
+                                                "batched_launches", "batched_jobs", "graph_replays", "sizes")}
     if prec != _lib.PREC_FP32:
         # the same job with FP32 gridding, untimed: volume and R-factor deltas
         p.precision = _lib.PREC_FP32

@@ -89,6 +89,7 @@ class ServerStats(ctypes.Structure):
         ("batched_launches", ctypes.c_int64),
         ("batched_jobs", ctypes.c_int64),
         ("peak_footprint_bytes", ctypes.c_int64),
+        ("graph_replays", ctypes.c_int64),
         ("hbm_budget_bytes", ctypes.c_int64),
         ("max_active_seen", ctypes.c_int32),
         ("pad_", ctypes.c_int32),

@@ -219,6 +219,12 @@ void recon_post(genfire_job* j, int it);
 void recon_end(genfire_job* j);
 void do_reconstruct(genfire_job* j);
 
+// True when iteration `it` of the job launches the same kernels with the same
+// arguments as its neighbours (no enforcement stage, no R-factor evaluation, not
+// the first or the last two iterations, no snapshots): such iterations may
+// replay one captured graph.
+bool steady_iteration(const genfire_job* j, int it);
+
 // The passes of iteration its[i] of every job (half-spectrum path, one shape)
 // as batched launches on st.
 void half_body_batch(genfire_job* const* jobs, const int* its, int count, hipStream_t st);

@@ -1015,6 +1015,14 @@ void recon_end(genfire_job* j)
     r.begun = false;
 }
 
+bool steady_iteration(const genfire_job* j, int it)
+{
+    const ReconState& r = j->rs;
+    const bool stage = r.cur < r.sched.size() && it == r.sched[r.cur];
+    const bool mon_it = r.monitor && (it % r.L == 0);
+    return !snap_on(j) && it > 1 && it + 1 < r.N && !stage && !mon_it;
+}
+
 // Steady-state iterations replay one captured HIP graph (the transform passes
 // and the errK reduction, whose output slot is the device-side iteration
 // index) unless per-kernel profiling is on or GENFIRE_GRAPHS=0.
@@ -1039,8 +1047,7 @@ void do_reconstruct(genfire_job* j)
     std::pair<hipEvent_t, hipEvent_t> e_it;
     for (int it = 1; it <= r.N; ++it) {
         const bool stage = r.cur < r.sched.size() && it == r.sched[r.cur];
-        const bool mon_it = r.monitor && (it % r.L == 0);
-        const bool steady = graphs && it > 1 && it + 1 < r.N && !stage && !mon_it;
+        const bool steady = graphs && steady_iteration(j, it);
         kstat_begin(j, GENFIRE_KSTAT_ITERATION, &e_it);
         if (steady) {
             if (!r.gexec) {

@@ -83,13 +83,27 @@ struct genfire_server {
     struct Group {
         hipStream_t st = nullptr;
         std::deque<hipEvent_t> inflight, pool;
+        // the batched steady-state iteration of the requests gids, captured once
+        // and replayed while the group's membership stays the same
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        std::vector<int64_t> gids;
+        void drop_graph()
+        {
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+            gids.clear();
+        }
     };
     std::map<std::tuple<int, int, int, int>, Group> groups;   // touched by the scheduler thread only
 
     // statistics
     std::vector<double> lat;
     double t_first = 0, t_last = 0;
-    int64_t submitted = 0, completed = 0, failed = 0, batched_launches = 0, batched_jobs = 0;
+    int64_t submitted = 0, completed = 0, failed = 0, batched_launches = 0, batched_jobs = 0, graph_replays = 0;
+    bool graphs = true;   // GENFIRE_GRAPHS=0 launches every iteration directly
     int max_active_seen = 0;
 
     // queues a worker task; the caller holds mu
@@ -272,16 +286,51 @@ bool genfire_server::advance_groups()
                 js.push_back(j);
                 its.push_back(r->it);
             }
-            for (size_t i = 0; i < js.size(); ++i) recon_pre(js[i], its[i]);
-            if (js[0]->rs.half) {
-                for (size_t c = 0; c < js.size(); c += (size_t)cfg.max_batch) {
-                    const int n = (int)std::min(js.size() - c, (size_t)cfg.max_batch);
-                    half_body_batch(js.data() + c, its.data() + c, n, g.st);
+            // the group's transform passes (the errK reduction included); for a
+            // steady iteration of every member, exactly what the graph holds
+            auto body = [&] {
+                if (js[0]->rs.half) {
+                    for (size_t c = 0; c < js.size(); c += (size_t)cfg.max_batch) {
+                        const int n = (int)std::min(js.size() - c, (size_t)cfg.max_batch);
+                        half_body_batch(js.data() + c, its.data() + c, n, g.st);
+                    }
+                } else {
+                    for (size_t i = 0; i < js.size(); ++i) {
+                        recon_body(js[i], its[i]);
+                        recon_post(js[i], its[i]);   // errK only: no R-factor in a steady iteration
+                    }
                 }
+            };
+            bool steady = graphs;
+            for (size_t i = 0; i < js.size() && steady; ++i) steady = steady_iteration(js[i], its[i]);
+            if (steady) {
+                std::vector<int64_t> ids;
+                for (genfire_request* r : rs) ids.push_back(r->id);
+                if (!g.gexec || g.gids != ids) {
+                    if (g.gexec) {   // launches of the old graph finish before it goes
+                        GF_HIP(hipStreamSynchronize(g.st));
+                        g.drop_graph();
+                    }
+                    GF_HIP(hipStreamBeginCapture(g.st, hipStreamCaptureModeThreadLocal));
+                    try {
+                        body();
+                    } catch (...) {
+                        hipGraph_t discard = nullptr;
+                        (void)hipStreamEndCapture(g.st, &discard);
+                        if (discard) (void)hipGraphDestroy(discard);
+                        throw;
+                    }
+                    GF_HIP(hipStreamEndCapture(g.st, &g.graph));
+                    GF_HIP(hipGraphInstantiate(&g.gexec, g.graph, nullptr, nullptr, 0));
+                    g.gids = ids;
+                }
+                GF_HIP(hipGraphLaunch(g.gexec, g.st));
             } else {
-                for (size_t i = 0; i < js.size(); ++i) recon_body(js[i], its[i]);
+                for (size_t i = 0; i < js.size(); ++i) recon_pre(js[i], its[i]);
+                body();
+                if (js[0]->rs.half)
+                    for (size_t i = 0; i < js.size(); ++i) recon_post(js[i], its[i]);   // the R-factor monitor
             }
-            for (size_t i = 0; i < js.size(); ++i) recon_post(js[i], its[i]);
             if (fail_at > 0 && batched_launches + 1 == fail_at) {
                 const int64_t k = fail_at;
                 fail_at = 0;   // once
@@ -310,6 +359,7 @@ bool genfire_server::advance_groups()
             std::lock_guard<std::mutex> lk(mu);
             ++batched_launches;
             batched_jobs += (int64_t)rs.size();
+            if (steady) ++graph_replays;
             for (genfire_request* r : rs) ++r->it;
             for (genfire_request* r : done) {
                 r->state = GENFIRE_REQ_FINISHING;
@@ -328,6 +378,7 @@ bool genfire_server::advance_groups()
         if (code != GENFIRE_OK) {
             // launches already queued for these jobs finish before their memory is freed
             if (g.st) (void)hipStreamSynchronize(g.st);
+            g.drop_graph();
             for (genfire_request* r : rs) fail(r, code, msg);
         }
     }
@@ -400,6 +451,7 @@ int genfire_server_create(genfire_ctx* ctx, const genfire_server_config* cfg, ge
             c.hbm_budget_bytes = (int64_t)((double)fr * 0.9);
         }
         if (const char* fa = std::getenv("GENFIRE_SERVER_FAIL_AT")) s->fail_at = std::atoll(fa);
+        if (const char* ge = std::getenv("GENFIRE_GRAPHS")) s->graphs = std::string(ge) != "0";
         genfire_server* sp = s.get();
         for (int i = 0; i < c.prep_threads + 1; ++i) sp->workers.emplace_back([sp] { sp->worker_loop(); });
         sp->sched = std::thread([sp] { sp->sched_loop(); });
@@ -425,6 +477,7 @@ int genfire_server_destroy(genfire_server* s)
         if (kv.second.st) (void)hipStreamSynchronize(kv.second.st);
         for (hipEvent_t e : kv.second.inflight) (void)hipEventDestroy(e);
         for (hipEvent_t e : kv.second.pool) (void)hipEventDestroy(e);
+        kv.second.drop_graph();
         if (kv.second.st) (void)hipStreamDestroy(kv.second.st);
     }
     delete s;
@@ -543,6 +596,7 @@ int genfire_server_get_stats(genfire_server* s, genfire_server_stats* out)
     out->max_active_seen = s->max_active_seen;
     out->batched_launches = s->batched_launches;
     out->batched_jobs = s->batched_jobs;
+    out->graph_replays = s->graph_replays;
     out->peak_footprint_bytes = s->peak_used;
     out->hbm_budget_bytes = s->cfg.hbm_budget_bytes;
     if (!s->lat.empty()) {

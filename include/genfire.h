@@ -290,6 +290,8 @@ typedef struct {
     int64_t batched_launches;      /* scheduler iterations launched (all groups)       */
     int64_t batched_jobs;          /* job-iterations they carried                      */
     int64_t peak_footprint_bytes;  /* largest sum of admitted footprints               */
+    int64_t graph_replays;         /* scheduler iterations that replayed a captured HIP graph of
+                                      the group's steady-state iteration (GENFIRE_GRAPHS=0: none) */
     int64_t hbm_budget_bytes;
     int32_t max_active_seen;
     int32_t pad_;

@@ -100,7 +100,33 @@ def test_results_same_alone_and_in_mix(ctx, mix_inputs):
     assert st["max_active_seen"] <= 4
     # the three 128^3 jobs iterate together at least part of the time
     assert st["batched_jobs"] > st["batched_launches"]
+    # steady-state iterations replay the group's captured graph
+    assert 0 < st["graph_replays"] < st["batched_launches"], st
     assert st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()
+
+
+def test_direct_launches_equal_graph_replays(ctx, mix_inputs, monkeypatch):
+    """GENFIRE_GRAPHS=0 launches every scheduler iteration directly; the results
+    equal those of the jobs alone (which replay graphs), bit for bit."""
+    print("[serve] graphs off", flush=True)
+    sub = mix_inputs[:3] + mix_inputs[3:4]
+    alone = [_alone(ctx, *m) for m in sub]
+    monkeypatch.setenv("GENFIRE_GRAPHS", "0")
+    cfg = _lib.default_server_config()
+    cfg.max_active = 4
+    srv = _lib.Server(ctx, cfg)
+    monkeypatch.delenv("GENFIRE_GRAPHS")
+    ids = [srv.submit(p, projs, ang, sup) for projs, ang, sup, p in sub]
+    srv.drain()
+    for rid, (rec0, ek0, er0) in zip(ids, alone):
+        assert srv.status(rid) == _lib.REQ_DONE
+        rec, ek, er, _ = srv.result(rid)
+        np.testing.assert_array_equal(rec, rec0)
+        np.testing.assert_array_equal(ek, ek0)
+        np.testing.assert_array_equal(er, er0)
+    st = srv.stats()
+    assert st["graph_replays"] == 0 and st["batched_jobs"] > st["batched_launches"], st
     srv.close()
 
 
