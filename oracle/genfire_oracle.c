@@ -228,7 +228,7 @@ int64_t go_select(int n1, int n2, int centro, const float R[9], double cutoff,
  * within `ulps` ulps of n/2.  Reports where an unpinned 1-ulp difference in
  * norm() or mldivide could flip an index decision. */
 void go_tie_census(int n1, int n2, int centro, const float R[9], double cutoff,
-                   int ulps, int64_t out[2])
+                   int ulps, int64_t out[4])
 {
     plane_t pl;
     plane_setup(R, &pl);
@@ -242,7 +242,8 @@ void go_tie_census(int n1, int n2, int centro, const float R[9], double cutoff,
         ulo = nextafterf(ulo, 0.0f); uhi = nextafterf(uhi, 1e30f);
         vlo = nextafterf(vlo, 0.0f); vhi = nextafterf(vhi, 1e30f);
     }
-    int64_t nd = 0, nuv = 0;
+    int64_t nd = 0, nuv = 0, nkept = 0, nexact = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nd, nuv, nkept, nexact)
     for (int i3 = 0; i3 < len[2]; ++i3)
         for (int i2 = 0; i2 < len[1]; ++i2)
             for (int i1 = 0; i1 < len[0]; ++i1) {
@@ -256,8 +257,10 @@ void go_tie_census(int n1, int n2, int centro, const float R[9], double cutoff,
                 lu3_solve(pl.lu, pl.piv, cp);
                 float au = fabsf(cp[0]), av = fabsf(cp[1]);
                 if ((au >= ulo && au <= uhi) || (av >= vlo && av <= vhi)) ++nuv;
+                if (au <= h1 && av <= h2) ++nkept;
+                if (au == h1 || av == h2) ++nexact;
             }
-    out[0] = nd; out[1] = nuv;
+    out[0] = nd; out[1] = nuv; out[2] = nkept; out[3] = nexact;
 }
 
 /* ------------------------------------------------------------------------- */

@@ -266,10 +266,15 @@ def select(n1: int, n2: int, centro: bool, R: np.ndarray, cutoff: float):
 
 
 def tie_census(n1: int, n2: int, centro: bool, R: np.ndarray, cutoff: float, ulps: int = 1):
-    out = np.zeros(2, np.int64)
+    """Near-ties of the two FP32 index decisions of fillInFourierGrid_DFT.m
+    (:178 D < cutoff, :190 |u| <= n1/2 and |v| <= n2/2) for one projection:
+    (grid points whose D lies within `ulps` ulps of the cutoff, points with
+    D < cutoff whose |u| or |v| lies within `ulps` ulps of n/2, points kept,
+    points with D < cutoff whose |u| or |v| equals n/2 exactly)."""
+    out = np.zeros(4, np.int64)
     Rf = np.ascontiguousarray(np.asarray(R, np.float32).ravel(order="F"))
     lib().go_tie_census(n1, n2, int(bool(centro)), Rf, float(cutoff), int(ulps), out)
-    return int(out[0]), int(out[1])
+    return int(out[0]), int(out[1]), int(out[2]), int(out[3])
 
 
 def lu_solve(R: np.ndarray, b) -> np.ndarray:

@@ -148,10 +148,19 @@ def test_kept_count_scales_with_slab(O):
         assert abs(len(ind) / (2 * c * area_half) - 1) < 0.1
 
 
-def test_tie_census_runs(O):
+def test_tie_census_closed_form(O):
+    """theta = 0 with cutoff 1: D = |k3| exactly, so the planes k3 = +-1 sit on the
+    cutoff (2 x 65 x 129 points); the kept set is the k3 = 0 plane (65 x 129) with
+    u = k1, v = k2, whose edge |u| = 64 or |v| = 64 holds 129 + 2 x 65 - 2 points."""
+    R, _ = O.rotation(0, 0, 0)
+    nd, nuv, nkept, nexact = O.tie_census(128, 128, True, R, 1.0, ulps=1)
+    assert (nd, nuv, nkept) == (2 * 65 * 129, 129 + 2 * 65 - 2, 65 * 129)
+    assert nexact == nuv   # u and v are the integers k1, k2 there
+    # the kept count is the selection's
     R, _ = O.rotation(3, -33, 2, v3=(0, 0, 1))
-    nd, nuv = O.tie_census(64, 64, True, R, 0.3, ulps=2)
-    assert nd >= 0 and nuv >= 0
+    nd, nuv, nkept, nexact = O.tie_census(64, 64, True, R, 0.3, ulps=2)
+    assert nkept == len(O.select(64, 64, True, R, 0.3)[0])
+    assert 0 <= nd and 0 <= nexact <= nuv <= nkept
 
 
 # ---------------------------------------------------------------- DFT values
@@ -271,3 +280,19 @@ def test_cpu_nufft_matches_direct_dft(O, d1, d2):
     assert np.linalg.norm(fine - direct) / np.linalg.norm(direct) < 1e-6
     if d1 & (d1 - 1) == 0 and d2 & (d2 - 1) == 0:   # p / d exact in FP32
         assert np.linalg.norm(direct - exact) / np.linalg.norm(exact) < 1e-12
+
+
+@pytest.mark.parametrize("d1,d2", [(32, 32), (48, 40), (64, 64)])
+def test_cpu_nufft_width16_is_fp64_grade(O, d1, d2):
+    """With kernel width 16 (sigma 2) the oracle's NUFFT agrees with the FP64
+    direct sum (fillInFourierGrid_DFT.m:236-248) to 1e-12: the FP64 reference
+    that reruns the gridding at the 1024^3 grids, where the direct sum is out of
+    reach on the CPU."""
+    rng = np.random.default_rng(7 + d1)
+    P = rng.uniform(0.0, 1.0, (d1, d2)).astype(np.float32)
+    un = rng.uniform(-0.5, 0.5, 3000).astype(np.float32)
+    vn = rng.uniform(-0.5, 0.5, 3000).astype(np.float32)
+    b = O.dft64(P, un, vn)
+    assert np.linalg.norm(O.nufft2d2(P, un, vn, w=16) - b) / np.linalg.norm(b) < 1e-12
+    # a width-8 kernel is not: the check above has teeth
+    assert np.linalg.norm(O.nufft2d2(P, un, vn, w=8) - b) / np.linalg.norm(b) > 1e-9
