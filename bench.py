@@ -190,6 +190,153 @@ def rfactor_fp64_delta(ctx, model, projs, ang, idx):
     return abs(r_gpu - r_64), r_gpu
 
 
+def fp64_regrid(job, projs, width=16):
+    """The gridding rerun on the CPU in FP64 at every point of the job's master
+    list (fillInFourierGrid_DFT.m:236-291): values by the oracle's type-2 NUFFT
+    with a width-16 kernel (agrees with the FP64 direct sum to 1e-12; the direct
+    sum is out of reach at these sizes), then the weighted average and
+    CentroSymmetricity in FP64.  Returns (points, FP64 values, measuredK as the
+    complex64 grid a job takes)."""
+    from oracle import oracle as O
+    pts = job.points()
+    proj = pts["proj"]
+    order = np.argsort(proj, kind="stable")
+    bounds = np.searchsorted(proj[order], np.arange(projs.shape[2] + 1))
+    v64 = np.empty(len(proj), np.complex128)
+    for p in range(projs.shape[2]):
+        g = order[bounds[p]:bounds[p + 1]]
+        if len(g):
+            v64[g] = O.nufft2d2(projs[:, :, p], pts["un"][g], pts["vn"][g], w=width)
+    total = int(np.prod(job.half_shape))
+    FS = O.weighted_average(pts["ind"], pts["dist"], v64, total)
+    mk = O.centro_symmetricity(FS, job.n1, job.n1 // 2, job.n2, dtype=np.complex64)
+    return pts, v64, mk
+
+
+def crop_mask(pts, job, crop):
+    """Points of the master list inside the central crop^3 of k-space."""
+    l1, l2, _ = job.half_shape
+    i1 = pts["ind"] % l1
+    i2 = (pts["ind"] // l1) % l2
+    i3 = pts["ind"] // (l1 * l2)
+    k = [i1 - job.n1 // 2, i2 - job.n2 // 2, i3 - job.n1 // 2]
+    return (np.abs(k[0]) < crop // 2) & (np.abs(k[1]) < crop // 2) & (np.abs(k[2]) < crop // 2)
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).astype(np.complex128)) / np.linalg.norm(b.astype(np.complex128)))
+
+
+# The larger configurations, one reconstruction each beside the headline line:
+# (workload, gridding precisions timed, FP16 projection storage, k-space crop of
+# the value error)
+EXTRA_CONFIGS = {
+    "tilt512": (("split_bf16", "fp16", "fp32"), False, 0),
+    "tilt512x121": (("split_bf16",), True, 128),
+}
+
+
+def extra_config(ctx, name, steps=1):
+    """One configuration end to end: each listed gridding precision timed over
+    `steps` reconstructions after one warm-up (grid + 200/300 DR iterations +
+    R-factor monitor, inputs resident in HBM), then the FP64 regridding check:
+    relative error of the gridded values over every point (and in the k-space
+    crop), and of the final volume and R-factor against the same FP32
+    iterations started from the FP64-gridded measuredK."""
+    from genfire_amd import _lib, phantom
+    precs_named, fp16_inputs, crop = EXTRA_CONFIGS[name]
+    precs = {"fp32": _lib.PREC_FP32, "split_bf16": _lib.PREC_SPLIT_BF16, "fp16": _lib.PREC_FP16}
+    w = phantom.WORKLOADS[name]
+    t0 = time.perf_counter()
+    vol, projs, ang, sup = phantom.make_inputs(w, phantom.gpu_project_many(ctx))
+    del vol
+    log(f"{name}: inputs in {time.perf_counter() - t0:.1f} s")
+    p = _lib.default_params()
+    p.griddingMethod = 2
+    p.numIterations = w.iterations
+    p.oversamplingRatio = w.oversampling
+    p.interpolationCutoffDistance = w.cutoff
+    p.constraintEnforcementMode = w.mode
+    p.algorithm = _lib.ALG_DR if w.algorithm == "dr" else _lib.ALG_GENFIRE
+    p.monitor_R = w.monitor_R
+    p.monitorR_loopLength = w.monitor_every
+    for i, vv in enumerate(w.vectors):
+        for j in range(3):
+            getattr(p, f"vector{i + 1}")[j] = vv[j]
+    if fp16_inputs:
+        scale = np.maximum(np.abs(projs).reshape(-1, w.num_projs, order="F").max(axis=0), 1e-30).astype(np.float32)
+        projs16 = (projs / scale[None, None, :]).astype(np.float16)
+        projs = (projs16.astype(np.float32) * scale[None, None, :]).astype(np.float32)
+    out = {"workload": name, "volume": w.dim, "projections": w.num_projs, "grid": None, "iterations": w.iterations,
+           "algorithm": w.algorithm, "rfactor_every": w.monitor_every,
+           "projection_storage": "fp16 + per-projection scale" if fp16_inputs else "fp32", "forms": {}}
+    recs = {}
+    vals = {}
+    first = None
+    for sp in precs_named:
+        p.precision = precs[sp]
+        job = _lib.Job(ctx, p, w.dim, w.dim, w.num_projs)
+        if fp16_inputs:
+            job.set_inputs_f16(projs16, scale, ang, sup)
+        else:
+            job.set_inputs(projs, ang, sup)
+        lat = []
+        for s in range(1 + steps):
+            t = time.perf_counter()
+            job.grid()
+            job.reconstruct()
+            if s:
+                lat.append(time.perf_counter() - t)
+        tm = job.timing()
+        errR = job.errR()[0]
+        recs[sp] = (job.reconstruction(), float(errR[-1]) if len(errR) else None)
+        ms = float(np.mean(lat)) * 1e3
+        out["grid"] = job.n1
+        out["forms"][sp] = {"recon_per_hour": 3600.0 / (ms * 1e-3), "ms_per_recon": ms,
+                            "gridded_kpts_per_s": job.num_points() / (tm["grid_ms"] * 1e-3),
+                            "stage_ms": {"grid": tm["grid_ms"], "iterations": tm["iterate_ms"],
+                                         "rfactor": tm["rfactor_ms"]},
+                            "values": values_route(argparse.Namespace(gridding="dft"), sp),
+                            "graph_replays": job.graph_replays(), "rfactor_final": recs[sp][1]}
+        log(f"{name} {sp}: {ms:.0f} ms per reconstruction, {tm}")
+        vals[sp] = job.points()["val"]
+        if first is None:
+            first = job          # the FP64 regridding reads its master list
+        else:
+            job.close()
+    try:
+        t = time.perf_counter()
+        pts, v64, mk = fp64_regrid(first, projs)
+        log(f"{name}: FP64 regridding of {len(v64)} points in {time.perf_counter() - t:.0f} s")
+        keep = crop_mask(pts, first, crop) if crop else None
+        first.close()
+        p.precision = _lib.PREC_FP32
+        ref = _lib.Job(ctx, p, w.dim, w.dim, w.num_projs)
+        ref.set_inputs(projs, ang, sup)
+        ref.set_measuredK(mk)
+        del mk
+        ref.reconstruct()
+        rec64 = ref.reconstruction()
+        errR64 = ref.errR()[0]
+        ref.close()
+        chk = {"reference": "gridding rerun on the CPU in FP64 at every master-list point (type-2 NUFFT, "
+                            "width-16 kernel, within 1e-12 of the direct sum; FP64 weighted average and "
+                            "centro-symmetry), then the same FP32 GPU iterations from that measuredK",
+               "points": int(len(v64))}
+        for sp, v32 in vals.items():
+            chk[f"grid_value_rel_err_{sp}"] = rel(v32, v64)
+            if crop:
+                chk[f"grid_value_rel_err_{sp}_crop{crop}"] = rel(v32[keep], v64[keep])
+        for sp, (rec, rf) in recs.items():
+            chk[f"volume_rel_err_{sp}"] = float(np.linalg.norm(rec - rec64) / np.linalg.norm(rec64))
+            if rf is not None and len(errR64):
+                chk[f"rfactor_delta_{sp}"] = abs(rf - float(errR64[-1]))
+        out["fp64_check"] = chk
+    except Exception as e:  # reported, never the target
+        out["fp64_check"] = {"error": str(e)}
+    return out
+
+
 def box_rows(n: int, b: int, ts: int) -> int:
     """Rows of the ts-row tiles along an axis of length n that meet the centred
     box of extent b (FFT order: [0, b - b//2) and [n - b//2, n))."""
@@ -359,6 +506,8 @@ def main():
                     help="report the FP64 gridded-value error on the central crop^3 of k-space only")
     ap.add_argument("--single-axis", action="store_true",
                     help="tilt series about y only (phi = psi = 0) instead of the workload's three Euler angles")
+    ap.add_argument("--extra-configs", default="tilt512,tilt512x121",
+                    help="comma-separated larger configurations run once each beside the line ('' for none)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
@@ -571,6 +720,10 @@ def main():
         except Exception as e:  # the baseline is reported, never the target
             out["cpu_baseline"] = out.get("cpu_baseline") or {"error": str(e)}
             out["fp64_check_error"] = str(e)
+    extras = [c for c in args.extra_configs.split(",") if c]
+    if extras:
+        job.close()
+        out["configs_extra"] = {c: extra_config(ctx, c) for c in extras}
     print(json.dumps(out))
 
 

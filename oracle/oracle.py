@@ -367,10 +367,10 @@ def weighted_average(master_ind, master_dist, master_val, total: int):
     w[~index_exact] = np.float32(1.0) / dist[~index_exact]  # :275, single reciprocal
     w = w.astype(np.float64)
     vals = np.asarray(master_val, np.complex128)
-    FS = np.zeros(total, np.complex128)
-    np.add.at(FS, master_ind, vals * w)                     # :285
-    sw = np.zeros(total, np.float64)
-    np.add.at(sw, master_ind, w)                            # :286
+    # accumarray in list order: bincount adds its weights in input order
+    FS = (np.bincount(master_ind, weights=vals.real * w, minlength=total)
+          + 1j * np.bincount(master_ind, weights=vals.imag * w, minlength=total))   # :285
+    sw = np.bincount(master_ind, weights=w, minlength=total)                        # :286
     interp = sw > 0                                         # :287
     FS[interp] = FS[interp] / sw[interp]                    # :289
     ex_ind = master_ind[index_exact]
