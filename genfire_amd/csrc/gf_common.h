@@ -55,6 +55,13 @@ struct DevBuf {
         }
         bytes = n;
     }
+    // at least n bytes: keeps a larger buffer (no free and allocate when a
+    // buffer alternates between two uses of different sizes)
+    void ensure(size_t n) {
+        if (p && bytes >= n) return;
+        release();
+        alloc(n);
+    }
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;

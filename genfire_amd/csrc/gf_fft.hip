@@ -33,32 +33,47 @@ struct Tile {
     int valid;
 };
 
+// Row c of the rows a pass visits along an outer axis of length n: every row
+// (b == 0), or only the b rows of the centred box in FFT order, [0, b - b/2)
+// and [n - b/2, n).
+__host__ __device__ __forceinline__ int64_t box_row(int64_t c, int n, int b)
+{
+    return (b == 0 || c < b - b / 2) ? c : c + (n - b);
+}
+__host__ __device__ constexpr int rows_of(int n, int b) { return b == 0 ? n : b; }
+
 // Tile t of TP pencils along `axis`: axis 0 pencils are contiguous rows; axes 1
-// and 2 take TP consecutive x-positions at one (z) or (y) coordinate.
-__device__ __forceinline__ Tile tile_of(int axis, int n1, int n2, int n3, int TP, int64_t t)
+// and 2 take TP consecutive x-positions at one (z) or (y) coordinate.  yb, zb
+// restrict the y and z rows visited to the centred box (0: all); for axis 0 the
+// host only restricts y when TP divides both box segments, so a tile's pencils
+// stay consecutive.
+__device__ __forceinline__ Tile tile_of(int axis, int n1, int n2, int n3, int TP, int64_t t, int yb = 0,
+                                        int zb = 0)
 {
     Tile g;
     if (axis == 0) {
-        const int64_t np = (int64_t)n2 * n3;
+        const int ny = rows_of(n2, yb);
+        const int64_t np = (int64_t)ny * rows_of(n3, zb);
         const int64_t first = t * TP;
-        g.base = first * n1; g.ps = n1; g.es = 1;
+        const int64_t y = box_row(first % ny, n2, yb), z = box_row(first / ny, n3, zb);
+        g.base = (y + (int64_t)n2 * z) * n1; g.ps = n1; g.es = 1;
         g.valid = (int)min((int64_t)TP, np - first);
     } else {
         const int tpr = (n1 + TP - 1) / TP;
         const int64_t outer = t / tpr;
         const int i1 = (int)(t % tpr) * TP;
         g.ps = 1;
-        if (axis == 1) { g.base = i1 + (int64_t)n1 * n2 * outer; g.es = n1; }
-        else { g.base = i1 + (int64_t)n1 * outer; g.es = (int64_t)n1 * n2; }
+        if (axis == 1) { g.base = i1 + (int64_t)n1 * n2 * box_row(outer, n3, zb); g.es = n1; }
+        else { g.base = i1 + (int64_t)n1 * box_row(outer, n2, yb); g.es = (int64_t)n1 * n2; }
         g.valid = min(TP, n1 - i1);
     }
     return g;
 }
 
-__host__ __device__ constexpr int64_t ntiles(int axis, int n1, int n2, int n3, int TP)
+__host__ __device__ constexpr int64_t ntiles(int axis, int n1, int n2, int n3, int TP, int yb = 0, int zb = 0)
 {
-    return axis == 0 ? ((int64_t)n2 * n3 + TP - 1) / TP
-                     : (int64_t)((n1 + TP - 1) / TP) * (axis == 1 ? n3 : n2);
+    return axis == 0 ? ((int64_t)rows_of(n2, yb) * rows_of(n3, zb) + TP - 1) / TP
+                     : (int64_t)((n1 + TP - 1) / TP) * (axis == 1 ? rows_of(n3, zb) : rows_of(n2, yb));
 }
 
 enum Mode { kPlain = 0, kFirst = 1, kFused = 2, kLast = 3 };
@@ -69,6 +84,7 @@ struct PassArgs {
     const float2* src;   // plain pass input when not data (out of place), else null
     const float2* tw;    // exp(-2 pi i m / N), m < N
     int axis;
+    int yb, zb;          // plain passes: visit only the y / z rows of the centred box (0: all)
 };
 
 __device__ __forceinline__ float pr_apply(float v, uint8_t s, int positivity, int support)
@@ -205,7 +221,7 @@ __global__ __launch_bounds__(kFT) void k_fft(PassArgs a)
     constexpr int TP = tile_pencils(N);
     extern __shared__ float2 lds[];
     const IterArgs& it = a.it;
-    const Tile g = tile_of(a.axis, it.n1, it.n2, it.n3, TP, blockIdx.x);
+    const Tile g = tile_of(a.axis, it.n1, it.n2, it.n3, TP, blockIdx.x, a.yb, a.zb);
     float2* __restrict__ K = (MODE == kPlain) ? a.data : it.K;
 
     for (int idx = threadIdx.x; idx < TP * N; idx += kFT) {
@@ -248,7 +264,7 @@ __global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
     const int t = threadIdx.x;
     const int j = a.axis == 0 ? t % T : t / TP;
     const int p = a.axis == 0 ? t / T : t % TP;
-    const Tile g = tile_of(a.axis, it.n1, it.n2, it.n3, TP, blockIdx.x);
+    const Tile g = tile_of(a.axis, it.n1, it.n2, it.n3, TP, blockIdx.x, a.yb, a.zb);
     const bool valid = p < g.valid;
     float2* __restrict__ K = (MODE == kPlain) ? a.data : it.K;
     float2* row = lds + p * N;
@@ -305,13 +321,30 @@ struct LaunchShape {
     size_t lds;
 };
 
-template <int N> LaunchShape shape_n(int axis, int n1, int n2, int n3)
+template <int N> constexpr int pass_pencils()
+{
+    if constexpr (RegPlan<N>::ok) return RegPlan<N>::TP;
+    else return tile_pencils(N);
+}
+
+template <int N> LaunchShape shape_n(int axis, int n1, int n2, int n3, int yb = 0, int zb = 0)
 {
     if constexpr (RegPlan<N>::ok) {
-        return {ntiles(axis, n1, n2, n3, RegPlan<N>::TP), reg_threads<N>(),
+        return {ntiles(axis, n1, n2, n3, RegPlan<N>::TP, yb, zb), reg_threads<N>(),
                 ((size_t)RegPlan<N>::TP * N + reg_tw_size<N>()) * sizeof(float2)};
     } else {
-        return {ntiles(axis, n1, n2, n3, tile_pencils(N)), kFT, (size_t)tile_pencils(N) * N * sizeof(float2)};
+        return {ntiles(axis, n1, n2, n3, tile_pencils(N), yb, zb), kFT, (size_t)tile_pencils(N) * N * sizeof(float2)};
+    }
+}
+
+int pass_pencils_of(int n)
+{
+    switch (n) {
+#define GF_CASE(NN) case NN: return pass_pencils<NN>();
+        GF_FFT_SIZES(GF_CASE)
+#undef GF_CASE
+        default:
+            raise(GENFIRE_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not supported");
     }
 }
 
@@ -329,7 +362,7 @@ LaunchShape shape_of(int n, int axis, int n1, int n2, int n3)
 template <int N, int MODE, bool INV>
 void launch_n(const PassArgs& a, hipStream_t st)
 {
-    const LaunchShape sh = shape_n<N>(a.axis, a.it.n1, a.it.n2, a.it.n3);
+    const LaunchShape sh = shape_n<N>(a.axis, a.it.n1, a.it.n2, a.it.n3, a.yb, a.zb);
     if constexpr (RegPlan<N>::ok) {
         if (sh.lds > 65536)
             GF_HIP(hipFuncSetAttribute((const void*)k_fft_reg<N, MODE, INV>,
@@ -541,6 +574,32 @@ void fft_axis_from(const float2* src, float2* d, int n1, int n2, int n3, int axi
     a.axis = axis;
     const int n = axis_len(a.it, axis);
     GF_CHECK(n > 1, GENFIRE_ERR_ARG, "out-of-place pass along an axis of length 1");
+    a.tw = fft_twiddles(n);
+    if (inverse) launch<kPlain, true>(n, a, st);
+    else launch<kPlain, false>(n, a, st);
+}
+
+void fft_axis_rows(float2* d, int n1, int n2, int n3, int axis, int inverse, int yb, int zb, hipStream_t st)
+{
+    GF_CHECK(yb >= 0 && yb <= n2 && zb >= 0 && zb <= n3 && (axis != 1 || yb == 0) && (axis != 2 || zb == 0),
+             GENFIRE_ERR_ARG, "fft_axis_rows: box rows along the transformed axis or beyond the grid");
+    PassArgs a{};
+    a.it.n1 = n1; a.it.n2 = n2; a.it.n3 = n3;
+    a.data = d;
+    a.axis = axis;
+    const int n = axis_len(a.it, axis);
+    if (n == 1) return;
+    yb = yb == n2 ? 0 : yb;
+    zb = zb == n3 ? 0 : zb;
+    if (axis == 0 && (yb > 0 || zb > 0)) {
+        // a tile's TP pencils must stay consecutive in memory: TP divides the
+        // rows per z plane and both y segments of the box; else every row
+        const int tp = pass_pencils_of(n);
+        const bool ok = (yb > 0 ? ((yb - yb / 2) % tp == 0 && (yb / 2) % tp == 0) : n2 % tp == 0);
+        if (!ok) yb = zb = 0;
+    }
+    a.yb = yb;
+    a.zb = zb;
     a.tw = fft_twiddles(n);
     if (inverse) launch<kPlain, true>(n, a, st);
     else launch<kPlain, false>(n, a, st);

@@ -26,6 +26,10 @@ struct PointTile {
     int32_t count;
     int32_t proj;
 };
+// The tiles of every projection: proj_off[P + 1] point offsets, tile_off[P + 1]
+// the first tile of each projection (tile_off[P] = ntiles).
+void grid_make_tiles(const int64_t* d_proj_off, const int64_t* d_tile_off, int P, int tile_pts, PointTile* d_tiles,
+                     int ntiles, hipStream_t st);
 // FP32 direct DFT of every point (fillInFourierGrid_DFT.m:236-248).
 void grid_values_fp32(const float* d_proj, int dim1, int dim2, const float* d_un, const float* d_vn,
                       const PointTile* d_tiles, int ntiles, float2* d_F, hipStream_t st);
@@ -115,6 +119,9 @@ const float2* fft_twiddles(int n);
 void fft_axis(float2* d, int n1, int n2, int n3, int axis, int inverse, hipStream_t st);
 // the same pass reading src and writing d (src and d must not overlap)
 void fft_axis_from(const float2* src, float2* d, int n1, int n2, int n3, int axis, int inverse, hipStream_t st);
+// In place along `axis`, visiting only the y rows (yb > 0) and z rows (zb > 0)
+// of the centred box in FFT order; the other rows are left as they are.
+void fft_axis_rows(float2* d, int n1, int n2, int n3, int axis, int inverse, int yb, int zb, hipStream_t st);
 void fft3(float2* d, int n1, int n2, int n3, int inverse, hipStream_t st);
 
 // ---- iteration passes (gf_fft.hip) ----

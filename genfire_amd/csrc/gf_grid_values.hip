@@ -100,6 +100,25 @@ __global__ __launch_bounds__(kVT) void k_dft_fp32(const float* __restrict__ proj
 // the reference's list order without atomics):
 //   exact (D < 1e-7): value overwrite, flag set (last projection wins, :291);
 //   else w = 1/D in single (:275), accumulated in FP64.
+__global__ void k_make_tiles(const int64_t* __restrict__ proj_off, const int64_t* __restrict__ tile_off, int P,
+                             int tile_pts, PointTile* __restrict__ tiles, int ntiles)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    int lo = 0, hi = P - 1;   // the projection p with tile_off[p] <= t < tile_off[p + 1]
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (tile_off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t b = proj_off[lo] + (int64_t)(t - tile_off[lo]) * tile_pts;
+    PointTile tl;
+    tl.begin = b;
+    tl.count = (int32_t)min((int64_t)tile_pts, proj_off[lo + 1] - b);
+    tl.proj = lo;
+    tiles[t] = tl;
+}
+
 __global__ void k_scatter(const uint32_t* __restrict__ ind, const float* __restrict__ dist,
                           const float2* __restrict__ F, int64_t begin, int64_t count,
                           double2* __restrict__ acc, double* __restrict__ sw,
@@ -116,13 +135,18 @@ __global__ void k_scatter(const uint32_t* __restrict__ ind, const float* __restr
         exact[h] = f;
         hits[h] = (hh + 1u) | kExactBit;
     } else {
+        // acc and sw are not cleared: a voxel's first point (no hits yet) writes
+        // them.  After an exact point they may hold stale sums, which the
+        // finalize ignores (the exact value wins).
         const float w = 1.0f / d;
         const double wd = (double)w;
-        double2 a = acc[h];
+        const bool first = hh == 0u;
+        double2 a = first ? make_double2(0.0, 0.0) : acc[h];
+        const double s = first ? 0.0 : sw[h];
         a.x += wd * (double)f.x;
         a.y += wd * (double)f.y;
         acc[h] = a;
-        sw[h] += wd;
+        sw[h] = s + wd;
         hits[h] = hh + 1u;
     }
 }
@@ -133,33 +157,38 @@ __device__ __forceinline__ int signed_freq(int s, int n) { return s < (n + 1) / 
 // (GENFIRE_Reconstructor.m:182); CentroSymmetricity.m: k1 <= 0 from FS(k),
 // k1 > 0 from conj(FS(-k)) (all four parity branches reduce to this rule).
 // Written directly in FFT order: voxel s holds frequency k = s (mod n).
-// One workgroup per (row segment of 256, y, z) of the FFT-order grid: no
-// division; a voxel with no hits (most of the grid) reads only its hit count.
+// One workgroup per kFinRows rows (y) of one z plane of the FFT-order grid, a
+// thread per x position (strided by 256): no division; a voxel with no hits
+// (most of the grid) reads only its hit count.
+constexpr int kFinRows = 8;
 __global__ __launch_bounds__(256) void k_finalize(HalfGrid hg, int n1, int n2, int centro,
                                                   const double2* __restrict__ acc, const double* __restrict__ sw,
                                                   const uint32_t* __restrict__ hits, const float2* __restrict__ exact,
                                                   float2* __restrict__ M)
 {
-    const int s1 = blockIdx.x * 256 + threadIdx.x, s2 = blockIdx.y, s3 = blockIdx.z;
-    if (s1 >= n1) return;
-    int k1 = signed_freq(s1, n1), k2 = signed_freq(s2, n2), k3 = signed_freq(s3, n1);
-    bool cj = false;
-    if (centro && k1 > 0) { k1 = -k1; k2 = -k2; k3 = -k3; cj = true; }
-    const int64_t h = (int64_t)(k1 - hg.lo[0]) +
-                      (int64_t)hg.len[0] * ((k2 - hg.lo[1]) + (int64_t)hg.len[1] * (k3 - hg.lo[2]));
-    float2 v = make_float2(0.0f, 0.0f);
-    const uint32_t hh = hits[h];
-    if (hh & kExactBit) {
-        v = exact[h];
-    } else if (hh != 0) {   // hits > 0: positive weights were summed
-        const double w = sw[h];
-        if (w > 0.0) {
-            const double2 a = acc[h];
-            v = make_float2((float)(a.x / w), (float)(a.y / w));
+    const int s3 = blockIdx.y;
+    const int y0 = blockIdx.x * kFinRows, y1 = min(n2, y0 + kFinRows);
+    for (int s2 = y0; s2 < y1; ++s2)
+        for (int s1 = threadIdx.x; s1 < n1; s1 += 256) {
+            int k1 = signed_freq(s1, n1), k2 = signed_freq(s2, n2), k3 = signed_freq(s3, n1);
+            bool cj = false;
+            if (centro && k1 > 0) { k1 = -k1; k2 = -k2; k3 = -k3; cj = true; }
+            const int64_t h = (int64_t)(k1 - hg.lo[0]) +
+                              (int64_t)hg.len[0] * ((k2 - hg.lo[1]) + (int64_t)hg.len[1] * (k3 - hg.lo[2]));
+            float2 v = make_float2(0.0f, 0.0f);
+            const uint32_t hh = hits[h];
+            if (hh & kExactBit) {
+                v = exact[h];
+            } else if (hh != 0) {   // hits > 0: positive weights were summed
+                const double w = sw[h];
+                if (w > 0.0) {
+                    const double2 a = acc[h];
+                    v = make_float2((float)(a.x / w), (float)(a.y / w));
+                }
+            }
+            if (cj) v.y = -v.y;
+            M[(int64_t)s1 + (int64_t)n1 * (s2 + (int64_t)n2 * s3)] = v;
         }
-    }
-    if (cj) v.y = -v.y;
-    M[(int64_t)s1 + (int64_t)n1 * (s2 + (int64_t)n2 * s3)] = v;
 }
 
 __global__ void k_shift(const float2* __restrict__ in, float2* __restrict__ out, int n1, int n2,
@@ -262,6 +291,14 @@ void grid_values_fp32(const float* d_proj, int dim1, int dim2, const float* d_un
     GF_LAUNCH_CHECK();
 }
 
+void grid_make_tiles(const int64_t* d_proj_off, const int64_t* d_tile_off, int P, int tile_pts, PointTile* d_tiles,
+                     int ntiles, hipStream_t st)
+{
+    if (ntiles <= 0) return;
+    k_make_tiles<<<(ntiles + 255) / 256, 256, 0, st>>>(d_proj_off, d_tile_off, P, tile_pts, d_tiles, ntiles);
+    GF_LAUNCH_CHECK();
+}
+
 void grid_scatter(const uint32_t* d_ind, const float* d_dist, const float2* d_F, int64_t begin,
                   int64_t count, double2* d_acc, double* d_sw, uint32_t* d_hits, float2* d_exact,
                   hipStream_t st)
@@ -277,8 +314,8 @@ void grid_finalize(const HalfGrid& hg, int n1, int n2, int centro, const double2
                    const double* d_sw, const uint32_t* d_hits, const float2* d_exact, float2* d_M,
                    hipStream_t st)
 {
-    GF_CHECK(n2 <= 65535 && n1 <= 65535, GENFIRE_ERR_UNSUPPORTED, "grid edge above 65535");
-    const dim3 grid((n1 + 255) / 256, n2, n1);
+    GF_CHECK(n2 <= 65535 * kFinRows && n1 <= 65535, GENFIRE_ERR_UNSUPPORTED, "grid edge above 65535");
+    const dim3 grid((n2 + kFinRows - 1) / kFinRows, n1);
     k_finalize<<<grid, 256, 0, st>>>(hg, n1, n2, centro, d_acc, d_sw, d_hits, d_exact, d_M);
     GF_LAUNCH_CHECK();
 }

@@ -121,7 +121,8 @@ struct genfire_job {
     gf::DevBuf rf_off, rf_meta, rf_val, rf_terms, rf_order, rf_shell_off, rf_den, Rfree;
     int rf_bins = 0;
     std::vector<float> Rfree_h;
-    std::vector<gf::PointTile> tiles_h;  // host source of tiles (outlives its asynchronous copy)
+    std::vector<int64_t> tile_meta_h;    // projection offsets and first tiles (outlive their asynchronous copy)
+    gf::DevBuf tile_meta;
     std::vector<float> nu_inv_h;         // host source of nu_inv, the NUFFT deconvolution factors
     gf::ProjForms rforms;         // the model forms the monitor's projections read
     std::vector<float> errK_h, errR_h;
@@ -234,5 +235,8 @@ void rfree_step(genfire_job* j, hipStream_t st);
 
 // Frees what the iterations no longer need (point lists, hits, measuredK).
 void release_grid_extras(genfire_job* j);
+// The gridding's FP64 accumulators and value operands, which a job keeps for
+// its next gridding (part of release_grid_extras).
+void release_grid_scratch(genfire_job* j);
 
 }  // namespace gf

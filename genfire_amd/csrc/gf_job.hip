@@ -209,7 +209,7 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
     const double cutoff = j->prm.interpolationCutoffDistance;
     grid_index_count(j->planes.as<PlaneConst>(), P, j->hg, cutoff, n1, n2, nblk,
                      j->blk_counts.as<uint32_t>(), st);
-    j->scan_tmp.alloc(grid_index_scan_scratch(nb_total));
+    j->scan_tmp.ensure(grid_index_scan_scratch(nb_total));
     grid_index_scan(j->blk_counts.as<uint32_t>(), nb_total, j->blk_offsets.as<int64_t>(),
                     j->total.as<int64_t>(), j->scan_tmp.as<int64_t>(), st);
     std::vector<int64_t> offs(nb_total);
@@ -242,15 +242,22 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
     const int nu1 = nufft_fine_len(j->dim1), nu2 = nufft_fine_len(j->dim2);
     const bool use_nufft = prec == GENFIRE_PREC_FP32 && !use_valu && gks != "mfma" && nu1 > 0 && nu2 > 0;
     const int tile_pts = use_valu ? 256 : mfma_tile_points();
-    // kept in the job: the asynchronous copy below may read it after this scope
-    std::vector<PointTile>& tl = j->tiles_h;
-    tl.clear();
+    // tiles of tile_pts points within one projection, generated on the device
+    // from the projection offsets and the first tile of each projection (kept
+    // in the job: the asynchronous copy may read them after this scope)
+    std::vector<int64_t>& tm = j->tile_meta_h;
+    tm.assign(2 * (size_t)(P + 1), 0);
+    for (int p = 0; p <= P; ++p) tm[p] = j->proj_off[p];
     for (int p = 0; p < P; ++p)
-        for (int64_t b = j->proj_off[p]; b < j->proj_off[p + 1]; b += tile_pts)
-            tl.push_back({b, (int32_t)std::min<int64_t>(tile_pts, j->proj_off[p + 1] - b), p});
-    if (!tl.empty()) {
-        j->tiles.alloc(sizeof(PointTile) * tl.size());
-        GF_HIP(hipMemcpyAsync(j->tiles.p, tl.data(), sizeof(PointTile) * tl.size(), hipMemcpyHostToDevice, st));
+        tm[P + 1 + p + 1] = tm[P + 1 + p] + (j->proj_off[p + 1] - j->proj_off[p] + tile_pts - 1) / tile_pts;
+    const int64_t ntiles = tm[2 * (size_t)P + 1];
+    GF_CHECK(ntiles < ((int64_t)1 << 31), GENFIRE_ERR_UNSUPPORTED, "gridding: too many point tiles");
+    if (ntiles > 0) {
+        j->tile_meta.alloc(sizeof(int64_t) * tm.size());
+        GF_HIP(hipMemcpyAsync(j->tile_meta.p, tm.data(), sizeof(int64_t) * tm.size(), hipMemcpyHostToDevice, st));
+        j->tiles.alloc(sizeof(PointTile) * ntiles);
+        grid_make_tiles(j->tile_meta.as<int64_t>(), j->tile_meta.as<int64_t>() + P + 1, P, tile_pts,
+                        j->tiles.as<PointTile>(), (int)ntiles, st);
         if (prec != GENFIRE_PREC_FP32) {
             const size_t ne = grid_prep_lp_elems(j->dim1, j->dim2, P);
             j->Phi.alloc(sizeof(uint16_t) * ne);
@@ -259,13 +266,17 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
             grid_prep_lp(prec, j->proj.as<float>(), j->dim1, j->dim2, P, j->Phi.as<uint16_t>(),
                          j->Plo.as<uint16_t>(), j->Pscale.as<float>(), st);
         } else if (use_nufft) {
-            // kept in the job, as the tiles are
-            j->nu_inv_h = nufft_deconv(j->dim1, nu1);
-            const std::vector<float> inv2 = nufft_deconv(j->dim2, nu2);
-            j->nu_inv_h.insert(j->nu_inv_h.end(), inv2.begin(), inv2.end());
-            j->nu_inv.alloc(sizeof(float) * j->nu_inv_h.size());
-            GF_HIP(hipMemcpyAsync(j->nu_inv.p, j->nu_inv_h.data(), sizeof(float) * j->nu_inv_h.size(),
-                                  hipMemcpyHostToDevice, st));
+            // the deconvolution factors depend on the sizes only: computed and
+            // uploaded at the job's first gridding (kept in the job, as the
+            // tiles are, for the asynchronous copy)
+            if (!j->nu_inv.p) {
+                j->nu_inv_h = nufft_deconv(j->dim1, nu1);
+                const std::vector<float> inv2 = nufft_deconv(j->dim2, nu2);
+                j->nu_inv_h.insert(j->nu_inv_h.end(), inv2.begin(), inv2.end());
+                j->nu_inv.alloc(sizeof(float) * j->nu_inv_h.size());
+                GF_HIP(hipMemcpyAsync(j->nu_inv.p, j->nu_inv_h.data(), sizeof(float) * j->nu_inv_h.size(),
+                                      hipMemcpyHostToDevice, st));
+            }
             j->Pt.alloc(sizeof(float2) * (size_t)nu1 * nu2 * P);   // the padded spectra
         } else if (!use_valu) {
             j->Pt.alloc(sizeof(float) * grid_prep_floats(j->dim1, j->dim2, P));
@@ -280,17 +291,17 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
             fft_axis(G, nu1, nu2, P, 0, 0, st);
             fft_axis(G, nu1, nu2, P, 1, 0, st);
             nufft_interp(G, nu1, nu2, nufft_beta(j->dim1, nu1), nufft_beta(j->dim2, nu2), j->un.as<float>(),
-                         j->vn.as<float>(), j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
+                         j->vn.as<float>(), j->tiles.as<PointTile>(), (int)ntiles, j->F.as<float2>(), st);
         } else if (prec != GENFIRE_PREC_FP32)
             grid_values_mfma_lp(prec, j->Phi.as<uint16_t>(), j->Plo.as<uint16_t>(), j->Pscale.as<float>(), j->dim1,
                                 j->dim2, j->un.as<float>(), j->vn.as<float>(), j->tiles.as<PointTile>(),
-                                (int)tl.size(), j->F.as<float2>(), st);
+                                (int)ntiles, j->F.as<float2>(), st);
         else if (use_valu)
             grid_values_fp32(j->proj.as<float>(), j->dim1, j->dim2, j->un.as<float>(), j->vn.as<float>(),
-                             j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
+                             j->tiles.as<PointTile>(), (int)ntiles, j->F.as<float2>(), st);
         else
             grid_values_mfma_fp32(j->Pt.as<float>(), j->dim1, j->dim2, j->un.as<float>(), j->vn.as<float>(),
-                                  j->tiles.as<PointTile>(), (int)tl.size(), j->F.as<float2>(), st);
+                                  j->tiles.as<PointTile>(), (int)ntiles, j->F.as<float2>(), st);
         kstat_end(j, GENFIRE_KSTAT_GRID_VALUES, ev);
     }
 
@@ -300,9 +311,7 @@ void grid_dft_body(genfire_job* j, hipStream_t st)
     j->sw.alloc(sizeof(double) * ht);
     j->hits.alloc(sizeof(uint32_t) * ht);
     j->exact.alloc(sizeof(float2) * ht);
-    GF_HIP(hipMemsetAsync(j->acc.p, 0, j->acc.bytes, st));
-    GF_HIP(hipMemsetAsync(j->sw.p, 0, j->sw.bytes, st));
-    GF_HIP(hipMemsetAsync(j->hits.p, 0, j->hits.bytes, st));
+    GF_HIP(hipMemsetAsync(j->hits.p, 0, j->hits.bytes, st));   // acc and sw are written by each voxel's first point
     for (int p = 0; p < P; ++p)
         grid_scatter(j->ind.as<uint32_t>(), j->dist.as<float>(), j->F.as<float2>(), j->proj_off[p],
                      j->proj_off[p + 1] - j->proj_off[p], j->acc.as<double2>(), j->sw.as<double>(),
@@ -376,10 +385,12 @@ void do_grid(genfire_job* j)
 
     // recIFFT = crop(real(fftshift(ifftn(ifftshift(measuredK))))) (:304)
     if (j->prm.compute_recIFFT) {
-        j->K.alloc(sizeof(float2) * full_size(j));
-        fft_axis_from(j->M.as<float2>(), j->K.as<float2>(), n1, n2, n1, 0, 1, st);   // M stays as gridded
-        fft_axis(j->K.as<float2>(), n1, n2, n1, 1, 1, st);
-        fft_axis(j->K.as<float2>(), n1, n2, n1, 2, 1, st);
+        // z, then y on the rows with z in the Dim1 box, then x on the rows with y
+        // and z in the box: the crop reads nothing else
+        j->K.ensure(sizeof(float2) * full_size(j));   // the iteration's spectrum buffer, when it exists
+        fft_axis_from(j->M.as<float2>(), j->K.as<float2>(), n1, n2, n1, 2, 1, st);   // M stays as gridded
+        fft_axis_rows(j->K.as<float2>(), n1, n2, n1, 1, 1, 0, j->dim1, st);
+        fft_axis_rows(j->K.as<float2>(), n1, n2, n1, 0, 1, j->dim2, j->dim1, st);
         j->recifft.alloc(sizeof(float) * box_size(j));
         crop_centre(j->K.p, 1, n1, n2, n1, j->recifft.as<float>(), j->dim1, j->dim2, j->dim1, st);
     }
@@ -388,14 +399,9 @@ void do_grid(genfire_job* j)
     float ms = 0.0f;
     GF_HIP(hipEventElapsedTime(&ms, tm.a, tm.b));
     j->t_grid = ms;
-    // the FP64 accumulators and the GEMM operands are only needed for the values above
-    j->acc.release();
-    j->sw.release();
-    j->exact.release();
-    j->Pt.release();
-    j->nu_inv.release();
-    j->Phi.release();
-    j->Plo.release();
+    // The FP64 accumulators and the value operands stay allocated for the
+    // job's next gridding (a request server frees them before the iteration
+    // set-up, release_grid_scratch).
     j->gridded = true;
     j->reconstructed = false;
 }
@@ -675,7 +681,7 @@ void recon_begin(genfire_job* j)
     r.loop.init();
     GF_HIP(hipEventRecord(r.all.a, st));
     j->x.alloc(sizeof(float) * nf);
-    j->K.alloc(sizeof(float2) * (half ? std::max<int64_t>(2 * hn, nf) : nf));
+    j->K.ensure(sizeof(float2) * (half ? std::max<int64_t>(2 * hn, nf) : nf));
     j->S.alloc(nf);
     j->best.alloc(sizeof(float) * nf);
     if (monitor && (dr || half)) j->pfbuf.alloc(sizeof(float) * nf);
@@ -725,7 +731,7 @@ void recon_begin(genfire_job* j)
     const int64_t nseg = half ? npen * hcsr_chunks() : npen;   // counted segments
     j->csr_cnt.alloc(sizeof(uint32_t) * nseg);
     j->csr_off.alloc(sizeof(int64_t) * (npen + 1));
-    j->scan_tmp.alloc(grid_index_scan_scratch(nseg));
+    j->scan_tmp.ensure(grid_index_scan_scratch(nseg));
     if (half) {
         j->csr_coff.alloc(sizeof(int64_t) * nseg);
         hcsr_count(j->M.as<float2>(), mask, n1, n2, n3, j->csr_cnt.as<uint32_t>(), st);
@@ -1099,6 +1105,20 @@ void release_grid_extras(genfire_job* j)
     j->mask.release();
     j->csr_cnt.release();
     j->den_tmp.release();
+    release_grid_scratch(j);
+}
+
+void release_grid_scratch(genfire_job* j)
+{
+    j->acc.release();
+    j->sw.release();
+    j->exact.release();
+    j->Pt.release();
+    j->nu_inv.release();
+    j->Phi.release();
+    j->Plo.release();
+    j->Pscale.release();
+    j->tile_meta.release();
 }
 
 // ---------------------------------------------------------------------------

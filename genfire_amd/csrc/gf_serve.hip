@@ -184,6 +184,7 @@ void genfire_server::prepare(genfire_request* r)
         job_setup(j.get(), ctx, &r->prm, r->dim1, r->dim2, r->P);
         job_set_inputs(j.get(), r->proj.data(), r->ang.data(), r->has_sup ? r->sup.data() : nullptr, nullptr);
         do_grid(j.get());
+        release_grid_scratch(j.get());   // before the iteration buffers: the footprint is the larger phase
         j->run_st = j->st;
         recon_begin(j.get());
         release_grid_extras(j.get());
