@@ -488,8 +488,8 @@ def main():
                     help="gridding GEMM precision (FFT and constraint steps stay FP32)")
     ap.add_argument("--gridding", choices=["dft", "fft"], default="dft",
                     help="gridding method: DFT (griddingMethod 2, the headline path) or FFT (griddingMethod 1)")
-    ap.add_argument("--served-jobs", type=int, default=6,
-                    help="requests of the served form on the same workload (0: skip)")
+    ap.add_argument("--served-jobs", type=int, default=24,
+                    help="requests of the served form on the same workload, submitted at once (0: skip)")
     ap.add_argument("--served-precision", choices=["fp32", "split_bf16", "fp16"], default="split_bf16",
                     help="gridding precision of the second served line (the mixed-precision served form)")
     ap.add_argument("--serve-mix", type=int, default=0,
