@@ -403,10 +403,10 @@ def serve_tilt(ctx, p, w, projs, ang, sup, njobs, precision, max_active):
     }
 
 
-def serve_mix(ctx, nreq, rate, seed, max_active):
-    """The serving mix: a seeded Poisson stream of requests of uneven size
-    (phantom.serving_mix) submitted at their arrival times; reports jobs/hour
-    and p50/p99 time to reconstruction."""
+def mix_requests(ctx, nreq, rate, seed):
+    """The requests of the serving mix (phantom.serving_mix): a seeded Poisson
+    stream of uneven sizes, one blob phantom per size, per-request tilts, counts
+    and noise; returns (requests, [(params, request inputs)])."""
     from genfire_amd import _lib, phantom
     reqs = phantom.serving_mix(nreq, rate, seed)
     pm = phantom.gpu_project_many(ctx)
@@ -428,18 +428,27 @@ def serve_mix(ctx, nreq, rate, seed, max_active):
         prm.interpolationCutoffDistance = w.cutoff
         prm.constraintEnforcementMode = w.mode
         prm.algorithm = _lib.ALG_DR
-        for i, v in enumerate(w.vectors):
+        for i, vv in enumerate(w.vectors):
             for k in range(3):
-                getattr(prm, f"vector{i + 1}")[k] = v[k]
+                getattr(prm, f"vector{i + 1}")[k] = vv[k]
         inputs.append((prm, _lib.Server.request_inputs(pj, a, phantom.spherical_support(r.dim, w.support_radius))))
     log(f"serving mix: {nreq} requests prepared in {time.perf_counter() - t0:.1f} s")
+    return reqs, inputs
+
+
+def serve_mix(ctx, nreq, rate, seed, max_active, saturate=False, prepared=None):
+    """The serving mix submitted at its arrival times (saturate: all at once,
+    the server's capacity on the mix); reports jobs/hour and p50/p99 time to
+    reconstruction."""
+    from genfire_amd import _lib, phantom
+    reqs, inputs = prepared if prepared is not None else mix_requests(ctx, nreq, rate, seed)
     cfg = _lib.default_server_config()
     cfg.max_active = max_active
     srv = _lib.Server(ctx, cfg)
     t0 = time.perf_counter()
     for r, (prm, req) in zip(reqs, inputs):
         dt = r.arrival_s - (time.perf_counter() - t0)
-        if dt > 0:
+        if dt > 0 and not saturate:
             time.sleep(dt)
         srv.submit(prm, req)
     srv.drain()
@@ -453,8 +462,9 @@ def serve_mix(ctx, nreq, rate, seed, max_active):
         "metric": "serving mix: jobs/hour with p50/p99 time to reconstruction",
         "value": st["jobs_per_hour"], "unit": "jobs/hour", "n_gpus": 1, "higher_is_better": True,
         "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
-        "requests": nreq, "arrival_rate_per_s": rate, "span_s": span, "completed": st["completed"],
-        "failed": st["failed"], "max_active": max_active, "max_active_seen": st["max_active_seen"],
+        "requests": nreq, "arrival_rate_per_s": None if saturate else rate, "span_s": span,
+        "completed": st["completed"], "failed": st["failed"], "max_active": max_active,
+        "max_active_seen": st["max_active_seen"],
         "batched_launches": st["batched_launches"], "batched_jobs": st["batched_jobs"],
         "graph_replays": st["graph_replays"], "peak_footprint_gb": st["peak_footprint_bytes"] / 1e9,
         "sizes": sizes,
@@ -494,7 +504,7 @@ def main():
                     help="gridding precision of the second served line (the mixed-precision served form)")
     ap.add_argument("--serve-mix", type=int, default=0,
                     help="run only the serving mix with this many requests and print its line")
-    ap.add_argument("--mix-summary", type=int, default=16,
+    ap.add_argument("--mix-summary", type=int, default=48,
                     help="requests of the serving-mix summary appended to the line (0: skip)")
     ap.add_argument("--mix-rate", type=float, default=4.0)
     ap.add_argument("--max-active", type=int, default=4)
@@ -675,10 +685,14 @@ def main():
             out["served_mixed"]["rfactor_delta_vs_fp32"] = abs(served[1]["rfactor_final"] - served[0]["rfactor_final"])
         p.precision = prec
     if args.mix_summary > 0:
-        m = serve_mix(ctx, args.mix_summary, args.mix_rate, 7, args.max_active)
-        out["serving_mix"] = {k: m[k] for k in ("value", "unit", "p50_ms", "p99_ms", "mean_ms", "requests",
-                                                "arrival_rate_per_s", "completed", "failed", "max_active_seen",
-                                                "batched_launches", "batched_jobs", "graph_replays", "sizes")}
+        keys = ("value", "unit", "p50_ms", "p99_ms", "mean_ms", "requests", "arrival_rate_per_s", "completed",
+                "failed", "max_active_seen", "batched_launches", "batched_jobs", "graph_replays", "sizes")
+        prepared = mix_requests(ctx, args.mix_summary, args.mix_rate, 7)
+        m = serve_mix(ctx, args.mix_summary, args.mix_rate, 7, args.max_active, prepared=prepared)
+        out["serving_mix"] = {k: m[k] for k in keys}
+        # the same requests submitted at once: the server's capacity on the mix
+        m = serve_mix(ctx, args.mix_summary, args.mix_rate, 7, args.max_active, saturate=True, prepared=prepared)
+        out["serving_mix_saturated"] = {k: m[k] for k in keys}
     if prec != _lib.PREC_FP32:
         # the same job with FP32 gridding, untimed: volume and R-factor deltas
         p.precision = _lib.PREC_FP32
