@@ -518,6 +518,8 @@ def main():
                     help="tilt series about y only (phi = psi = 0) instead of the workload's three Euler angles")
     ap.add_argument("--extra-configs", default="tilt512,tilt512x121",
                     help="comma-separated larger configurations run once each beside the line ('' for none)")
+    ap.add_argument("--projector", choices=["linear", "cubic"], default="linear",
+                    help="the R-factor monitor's interpolant: trilinear, or interp3 'cubic' (cubic convolution)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (the HBM-counter calibration kernel)")
     args = ap.parse_args()
@@ -568,6 +570,7 @@ def main():
     precs = {"fp32": _lib.PREC_FP32, "split_bf16": _lib.PREC_SPLIT_BF16, "fp16": _lib.PREC_FP16}
     prec = precs[args.precision]
     p.precision = prec
+    p.projectorInterpolation = _lib.INTERP_CUBIC if args.projector == "cubic" else _lib.INTERP_LINEAR
     job = _lib.Job(ctx, p, w.dim, w.dim, w.num_projs)
     if args.fp16_inputs:
         # FP16 storage with a max-abs scale per projection; the FP32 values the
@@ -648,7 +651,8 @@ def main():
         "data": "synthetic (seeded Gaussian-blob phantom, trilinear projections, Poisson 1e4 counts)",
         "config": {"workload": w.name, "volume": w.dim, "projections": w.num_projs, "projection_size": w.dim,
                    "grid": n, "iterations": w.iterations, "algorithm": w.algorithm,
-                   "rfactor_every": w.monitor_every, "gridding": args.gridding, "gridding_precision": args.precision,
+                   "rfactor_every": w.monitor_every, "projector": args.projector,
+                   "gridding": args.gridding, "gridding_precision": args.precision,
                    "values": values_route(args, args.precision),
                    "cutoff": w.cutoff, "oversampling": w.oversampling,
                    "tilt_series": "three Euler angles" if w.three_euler else "single axis (y)",

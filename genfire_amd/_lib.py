@@ -23,6 +23,7 @@ STATUS = {
 }
 ALG_GENFIRE, ALG_DR = 0, 1
 PREC_FP32, PREC_SPLIT_BF16, PREC_FP16 = 0, 1, 2
+INTERP_LINEAR, INTERP_CUBIC = 0, 1
 KSTAT_GRID_VALUES, KSTAT_FFT_PASS, KSTAT_FFT_FUSED, KSTAT_ITERATION, KSTAT_PROJECT = range(5)
 
 # every symbol include/genfire.h declares
@@ -35,7 +36,7 @@ EXPORTS = [
     "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_get_Rfree", "genfire_job_submit", "genfire_job_poll",
     "genfire_job_wait", "genfire_job_timing", "genfire_job_set_profiling",
     "genfire_job_kernel_stats", "genfire_job_graph_replays", "genfire_project", "genfire_rfactor", "genfire_fftn",
-    "genfire_project_general", "genfire_rfactor_general", "genfire_rfactor_noalloc", "genfire_job_set_snapshots",
+    "genfire_project_general", "genfire_rfactor_general", "genfire_project_cubic", "genfire_rfactor_cubic", "genfire_rfactor_noalloc", "genfire_job_set_snapshots",
     "genfire_server_config_default", "genfire_server_create", "genfire_server_destroy",
     "genfire_server_submit", "genfire_server_status", "genfire_server_wait", "genfire_server_drain",
     "genfire_server_result", "genfire_server_release", "genfire_server_get_stats",
@@ -66,6 +67,7 @@ class GenfireParams(ctypes.Structure):
         ("RandomSeed", ctypes.c_int64),
         ("griddingMethod", ctypes.c_int32),
         ("allowMultipleGridMatches", ctypes.c_int32),
+        ("projectorInterpolation", ctypes.c_int32),
     ]
 
 
@@ -162,6 +164,8 @@ def load():
         "genfire_job_graph_replays": ([vp, pi64], ctypes.c_int),
         "genfire_project": ([vp, pf, i32, i32, i32, f64, f64, f64, pf], ctypes.c_int),
         "genfire_rfactor": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pf], ctypes.c_int),
+        "genfire_project_cubic": ([vp, pf, i32, i32, i32, f64, f64, f64, pf], ctypes.c_int),
+        "genfire_rfactor_cubic": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pf], ctypes.c_int),
         "genfire_fftn": ([vp, pf, i32, i32, i32, i32], ctypes.c_int),
         "genfire_project_general": ([vp, pf, i32, i32, i32, f64, f64, f64, pd, pd, pd, pf], ctypes.c_int),
         "genfire_rfactor_general": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pd, pd, pd, pf],
@@ -229,14 +233,15 @@ class Context:
                                        1 if inverse else 0), "genfire_fftn")
         return buf.view(np.complex64).reshape(a.shape, order="F")
 
-    def project(self, vol: np.ndarray, phi: float, theta: float, psi: float) -> np.ndarray:
-        """calculate3Dprojection_RealSpaceinterp (trilinear)."""
+    def project(self, vol: np.ndarray, phi: float, theta: float, psi: float, interp: str = "linear") -> np.ndarray:
+        """calculate3Dprojection_RealSpaceinterp: trilinear (genfire_project) or
+        interp3 'cubic' (genfire_project_cubic)."""
         d1, d2, d3 = vol.shape
         v = np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F"))
         out = np.zeros(d1 * d2, np.float32)
-        self.check(self.L.genfire_project(self.h, _ptr(v, ctypes.c_float), d1, d2, d3, float(phi),
-                                          float(theta), float(psi), _ptr(out, ctypes.c_float)),
-                   "genfire_project")
+        name = {"linear": "genfire_project", "cubic": "genfire_project_cubic"}[interp]
+        self.check(getattr(self.L, name)(self.h, _ptr(v, ctypes.c_float), d1, d2, d3, float(phi),
+                                         float(theta), float(psi), _ptr(out, ctypes.c_float)), name)
         return out.reshape(d1, d2, order="F")
 
     def project_general(self, vol: np.ndarray, phi: float, theta: float, psi: float, v1, v2, v3) -> np.ndarray:
@@ -268,8 +273,9 @@ class Context:
                                                   _ptr(sim, ctypes.c_float)), "genfire_rfactor_general")
         return float(rt[0]), ra, sim.reshape(p1, p2, P, order="F")
 
-    def rfactor(self, model: np.ndarray, projections: np.ndarray, angles: np.ndarray):
-        """Tian_calc_Rfactor_realspace(model, Projs, Angles) -> (Rtotal, Rarr, simuProjs)."""
+    def rfactor(self, model: np.ndarray, projections: np.ndarray, angles: np.ndarray, interp: str = "linear"):
+        """Tian_calc_Rfactor_realspace(model, Projs, Angles) -> (Rtotal, Rarr, simuProjs),
+        with the trilinear (genfire_rfactor) or cubic (genfire_rfactor_cubic) projector."""
         d1, d2, d3 = model.shape
         p1, p2, P = projections.shape
         v = np.ascontiguousarray(np.asarray(model, np.float32).ravel(order="F"))
@@ -278,11 +284,11 @@ class Context:
         rt = np.zeros(1)
         ra = np.zeros(P)
         sim = np.zeros(p1 * p2 * P, np.float32)
-        self.check(self.L.genfire_rfactor(self.h, _ptr(v, ctypes.c_float), d1, d2, d3,
-                                          _ptr(m, ctypes.c_float), p1, p2, P,
-                                          _ptr(ang, ctypes.c_double), _ptr(rt, ctypes.c_double),
-                                          _ptr(ra, ctypes.c_double), _ptr(sim, ctypes.c_float)),
-                   "genfire_rfactor")
+        name = {"linear": "genfire_rfactor", "cubic": "genfire_rfactor_cubic"}[interp]
+        self.check(getattr(self.L, name)(self.h, _ptr(v, ctypes.c_float), d1, d2, d3,
+                                         _ptr(m, ctypes.c_float), p1, p2, P,
+                                         _ptr(ang, ctypes.c_double), _ptr(rt, ctypes.c_double),
+                                         _ptr(ra, ctypes.c_double), _ptr(sim, ctypes.c_float)), name)
         return float(rt[0]), ra, sim.reshape(p1, p2, P, order="F")
 
 

@@ -76,6 +76,7 @@ int genfire_params_default(genfire_params* p)
     p->RandomSeed = 0;
     p->griddingMethod = 1;
     p->allowMultipleGridMatches = 1;
+    p->projectorInterpolation = GENFIRE_INTERP_LINEAR;
     return GENFIRE_OK;
 }
 
@@ -415,9 +416,13 @@ namespace {
 
 // The model's forms the projections read (project_forms), on the device.
 struct ModelForms {
-    DevBuf vy, zy, zx;
+    DevBuf vy, zy, zx, vc;
     void build(const ProjForms& f, const float* d_vol, int d1, int d2, int d3, hipStream_t st)
     {
+        if (f.cubic) {
+            vc.alloc(sizeof(float) * cubic_form_floats(d1, d2, d3));
+            volume_cubic_form(d_vol, d1, d2, d3, vc.as<float>(), st);
+        }
         if (f.four_ray) {
             vy.alloc(sizeof(float) * (size_t)d1 * d2 * d3);
             volume_yfast(d_vol, vy.as<float>(), d1, d2, d3, st);
@@ -437,6 +442,7 @@ struct ModelForms {
         pa->vol_y = vy.as<float>();
         pa->zy = zy.as<float>();
         pa->zx = zx.as<float>();
+        pa->vcubic = f.cubic ? vc.as<float>() : nullptr;
     }
 };
 
@@ -456,10 +462,12 @@ struct OwnStream {
 };
 
 void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, const double ang[3],
-                 const double* vectors, float* proj)
+                 const double* vectors, float* proj, bool cubic = false)
 {
     use_device(ctx);
     GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: volume dimensions must be >= 2");
+    GF_CHECK(!cubic || (d1 >= 4 && d2 >= 4 && d3 >= 4), GENFIRE_ERR_ARG,
+             "GENFIRE: interp3 'cubic' needs at least 4 nodes along every axis");
     GF_CHECK((int64_t)d1 * d2 * d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "GENFIRE: volume above 2^31 voxels");
     DevBuf v, r, o;
     ModelForms mf;
@@ -470,7 +478,7 @@ void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, con
     o.alloc(sizeof(float) * (size_t)d1 * d2);
     bool any_y = false;
     std::vector<float> Rf = proj_matrices(ang, 1, &any_y, vectors);
-    const ProjForms f = project_forms(Rf.data(), 1, d2);
+    const ProjForms f = project_forms(Rf.data(), 1, d2, cubic);
     r.alloc(sizeof(float) * Rf.size());
     GF_HIP(hipMemcpyAsync(v.p, vol, sizeof(float) * nv, hipMemcpyHostToDevice, st));
     GF_HIP(hipMemcpyAsync(r.p, Rf.data(), sizeof(float) * Rf.size(), hipMemcpyHostToDevice, st));
@@ -489,11 +497,13 @@ void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, con
 
 void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, const float* projections, int p1,
                  int p2, int P, const double* angles, const double* vectors, double* Rtotal, double* Rarr,
-                 float* simu)
+                 float* simu, bool cubic = false)
 {
     use_device(ctx);
     GF_CHECK(P >= 1, GENFIRE_ERR_ARG, "GENFIRE: no projections");
     GF_CHECK(d1 >= 2 && d2 >= 2 && d3 >= 2, GENFIRE_ERR_ARG, "GENFIRE: model dimensions must be >= 2");
+    GF_CHECK(!cubic || (d1 >= 4 && d2 >= 4 && d3 >= 4), GENFIRE_ERR_ARG,
+             "GENFIRE: interp3 'cubic' needs at least 4 nodes along every axis");
     GF_CHECK(p1 >= 1 && p2 >= 1, GENFIRE_ERR_ARG, "GENFIRE: empty projections");
     GF_CHECK(p1 <= d1 && p2 <= d2, GENFIRE_ERR_ARG, "GENFIRE: projection larger than the model");
     GF_CHECK((int64_t)d1 * d2 * d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "GENFIRE: model above 2^31 voxels");
@@ -507,7 +517,7 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
     o.alloc(sizeof(float) * np);
     bool any_y = false;
     std::vector<float> Rf = proj_matrices(angles, P, &any_y, vectors);
-    const ProjForms f = project_forms(Rf.data(), P, d2);
+    const ProjForms f = project_forms(Rf.data(), P, d2, cubic);
     r.alloc(sizeof(float) * Rf.size());
     GF_HIP(hipMemcpyAsync(v.p, model, sizeof(float) * nv, hipMemcpyHostToDevice, st));
     GF_HIP(hipMemcpyAsync(m.p, projections, sizeof(float) * np, hipMemcpyHostToDevice, st));
@@ -551,6 +561,14 @@ int genfire_project(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, 
     return guarded(ctx, [&] { project_one(ctx, vol, d1, d2, d3, ang, nullptr, proj); });
 }
 
+int genfire_project_cubic(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3, double phi,
+                          double theta, double psi, float* proj)
+{
+    if (!ctx || !vol || !proj) return GENFIRE_ERR_ARG;
+    const double ang[3] = {phi, theta, psi};
+    return guarded(ctx, [&] { project_one(ctx, vol, d1, d2, d3, ang, nullptr, proj, true); });
+}
+
 int genfire_project_general(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3, double phi,
                             double theta, double psi, const double vector1[3], const double vector2[3],
                             const double vector3[3], float* proj)
@@ -569,6 +587,16 @@ int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2
     if (!ctx || !model || !projections || !angles || P < 1) return GENFIRE_ERR_ARG;
     return guarded(ctx, [&] {
         rfactor_all(ctx, model, d1, d2, d3, projections, p1, p2, P, angles, nullptr, Rtotal, Rarr, simu);
+    });
+}
+
+int genfire_rfactor_cubic(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
+                          const float* projections, int32_t p1, int32_t p2, int32_t P, const double* angles,
+                          double* Rtotal, double* Rarr, float* simu)
+{
+    if (!ctx || !model || !projections || !angles || P < 1) return GENFIRE_ERR_ARG;
+    return guarded(ctx, [&] {
+        rfactor_all(ctx, model, d1, d2, d3, projections, p1, p2, P, angles, nullptr, Rtotal, Rarr, simu, true);
     });
 }
 

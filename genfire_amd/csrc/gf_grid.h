@@ -268,6 +268,8 @@ struct ProjArgs {
     const float* vol_y; // y-fastest model (index y + d2*(x + d1*z)) for the four-ray form, else nullable
     const float* zy;    // z-duplicated y-fastest model (crop_zdup / volume_zdup), nullable when unused
     const float* zx;    // z-duplicated x-fastest model, nullable when unused
+    const float* vcubic; // padded form of the cubic-convolution projector (crop_cubic_form /
+                         // volume_cubic_form); when set every projection reads it
     int four_ray;       // every projection is an exact tilt about y and d2 % 4 == 0: read vol_y only
     int d1, d2, d3;
     const float* R;    // P x kRStride, single
@@ -283,8 +285,17 @@ struct ProjForms {
     bool four_ray = false;   // vol_y only
     bool zy = false;         // the z-duplicated y-fastest form
     bool zx = false;         // the z-duplicated x-fastest form
+    bool cubic = false;      // the cubic-convolution projector's padded form (then no other form)
 };
-ProjForms project_forms(const float* Rm, int P, int d2);
+// cubic: the projections interpolate with interp3's 'cubic' (cubic convolution)
+ProjForms project_forms(const float* Rm, int P, int d2, bool cubic = false);
+// Cubic convolution (Keys, a = -1/2) with Keys' boundary nodes: the model padded
+// by one node on every side, (d1+2) x (d2+2) x (d3+2), y fastest, node (x, y, z)
+// at (y + 1) + (d2 + 2) ((x + 1) + (d1 + 2) (z + 1)).  Every extent must be >= 4.
+int64_t cubic_form_floats(int d1, int d2, int d3);
+void volume_cubic_form(const float* d_vol, int d1, int d2, int d3, float* d_dst, hipStream_t st);
+void crop_cubic_form(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3,
+                     hipStream_t st);
 int project_blocks(int m1, int m2);
 // sets R[9] (y-fastest forms) and R[11] (lanes along pixel j) of a projection
 void project_orientation(float* R);

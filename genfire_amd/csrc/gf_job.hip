@@ -429,12 +429,14 @@ void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
     }
     if (f.zy) crop_zdup(d_model_fft, j->n1, j->n2, j->n1, j->zy.as<float>(), j->dim1, j->dim2, j->dim1, true, st);
     if (f.zx) crop_zdup(d_model_fft, j->n1, j->n2, j->n1, j->zx.as<float>(), j->dim1, j->dim2, j->dim1, false, st);
+    if (f.cubic) crop_cubic_form(d_model_fft, j->n1, j->n2, j->n1, j->vcubic.as<float>(), j->dim1, j->dim2, j->dim1, st);
     const int nblk = project_blocks(j->dim1, j->dim2);
     ProjArgs pa{};
     pa.four_ray = f.four_ray ? 1 : 0;
     pa.vol_y = f.four_ray ? j->vol_y.as<float>() : nullptr;
     pa.zy = f.zy ? j->zy.as<float>() : nullptr;
     pa.zx = f.zx ? j->zx.as<float>() : nullptr;
+    pa.vcubic = f.cubic ? j->vcubic.as<float>() : nullptr;
     pa.d1 = j->dim1; pa.d2 = j->dim2; pa.d3 = j->dim1;
     pa.R = j->Rmat.as<float>();
     pa.P = P;
@@ -777,7 +779,7 @@ void recon_begin(genfire_job* j)
     if (r.nR > 0) {
         bool any_y = false;
         std::vector<float> Rm = proj_matrices(j->angles.data(), j->P, &any_y);
-        j->rforms = project_forms(Rm.data(), j->P, j->dim2);
+        j->rforms = project_forms(Rm.data(), j->P, j->dim2, j->prm.projectorInterpolation == GENFIRE_INTERP_CUBIC);
         j->Rmat.alloc(sizeof(float) * Rm.size());
         GF_HIP(hipMemcpyAsync(j->Rmat.p, Rm.data(), sizeof(float) * Rm.size(), hipMemcpyHostToDevice, st));
         const ProjForms& f = j->rforms;
@@ -787,6 +789,7 @@ void recon_begin(genfire_job* j)
         }
         if (f.zy) j->zy.alloc(sizeof(float) * zdup_floats(j->dim1, j->dim2, j->dim1));
         if (f.zx) j->zx.alloc(sizeof(float) * zdup_floats(j->dim1, j->dim2, j->dim1));
+        if (f.cubic) j->vcubic.alloc(sizeof(float) * cubic_form_floats(j->dim1, j->dim2, j->dim1));
         j->rsums.alloc(sizeof(double) * (size_t)j->P * 4 * project_blocks(j->dim1, j->dim2));
         j->rout.alloc(sizeof(double) * (size_t)r.nR * j->P * 4);
         GF_HIP(hipStreamSynchronize(st));
@@ -1153,6 +1156,10 @@ void validate_job(const genfire_params* p, int dim1, int dim2, int P)
              GENFIRE_ERR_ARG, "GENFIRE: unknown gridding precision");
     GF_CHECK(p->griddingMethod == 1 || p->griddingMethod == 2, GENFIRE_ERR_ARG,
              "GENFIRE: Unrecognized gridding method.");   // GENFIRE_Reconstructor.m:151-153
+    GF_CHECK(p->projectorInterpolation == GENFIRE_INTERP_LINEAR || p->projectorInterpolation == GENFIRE_INTERP_CUBIC,
+             GENFIRE_ERR_ARG, "GENFIRE: unknown projector interpolation");
+    GF_CHECK(p->projectorInterpolation != GENFIRE_INTERP_CUBIC || !p->monitor_R || (dim1 >= 4 && dim2 >= 4),
+             GENFIRE_ERR_ARG, "GENFIRE: interp3 'cubic' needs at least 4 nodes along every axis");
     int n1 = 0, n2 = 0;
     grid_edges(*p, dim1, dim2, &n1, &n2);
     if (p->griddingMethod == 1)

@@ -310,6 +310,54 @@ __device__ __forceinline__ float march(const char* __restrict__ v0, const char* 
     return acc;
 }
 
+// Cubic convolution (interp3 'cubic', calculate3Dprojection_RealSpaceinterp.m:
+// 45,51): Keys' kernel with a = -1/2 on the 4 x 4 x 4 nodes around the cell
+// min(floor(f), d - 2), read from the padded form (whose boundary nodes follow
+// Keys' rule), one 16-byte load per (x, z) node pair: the four y nodes.  The
+// weights of t in [0, 1] for the nodes i - 1 .. i + 2 are
+// ((-t + 2) t - 1) t / 2, ((3t - 5) t t + 2) / 2, ((-3t + 4) t + 1) t / 2,
+// (t - 1) t t / 2.  Offsets in 64 bits: the form passes 4 GiB at d = 1024.
+typedef float quad4_t __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ void keys_w(float t, float w[4])
+{
+    w[0] = ((-t + 2.0f) * t - 1.0f) * t * 0.5f;
+    w[1] = ((3.0f * t - 5.0f) * t * t + 2.0f) * 0.5f;
+    w[2] = ((-3.0f * t + 4.0f) * t + 1.0f) * t * 0.5f;
+    w[3] = (t - 1.0f) * t * t * 0.5f;
+}
+
+__device__ __forceinline__ float march_cubic(const float* __restrict__ form, int d1, int d2, int d3, const Ray& r,
+                                             float Z, float Z1)
+{
+    const int64_t py = d2 + 2, pxy = (int64_t)(d1 + 2) * py;
+    float acc = 0.0f;
+    for (; Z <= Z1; Z += 1.0f) {
+        const float fx = __builtin_fmaf(r.rx, Z, r.bx), fy = __builtin_fmaf(r.ry, Z, r.by);
+        const float fz = __builtin_fmaf(r.rz, Z, r.bz);
+        const int a0 = min((int)fx, d1 - 2), b0 = min((int)fy, d2 - 2), e0 = min((int)fz, d3 - 2);
+        float wx[4], wy[4], wz[4];
+        keys_w(fx - (float)a0, wx);
+        keys_w(fy - (float)b0, wy);
+        keys_w(fz - (float)e0, wz);
+        // padded node (a0 - 1, b0 - 1, e0 - 1) sits at (a0, b0, e0)
+        const float* base = form + ((int64_t)b0 + py * (int64_t)a0 + pxy * (int64_t)e0);
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float sc = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const quad4_t q = *reinterpret_cast<const quad4_t*>(base + py * a + pxy * c);
+                sc += wx[a] * (wy[0] * q.x + wy[1] * q.y + wy[2] * q.z + wy[3] * q.w);
+            }
+            s += wz[c] * sc;
+        }
+        acc += s;
+    }
+    return acc;
+}
+
 // Every projection of a launch that is not all exact tilts about y: one ray
 // per lane on the z-duplicated forms, the lanes of a wave along the pixel axis
 // R[11] picks (or 8 x 8 patches), and the block mapping of the four-ray form:
@@ -317,7 +365,7 @@ __device__ __forceinline__ float march(const char* __restrict__ v0, const char* 
 // tiles of groups of G neighbouring ones, so that the G workgroups of one tile
 // share the XCD's L2 fills.  With at most 8 projections each XCD takes an
 // eighth of every projection's tiles.
-template <bool WIDE>
+template <bool WIDE, bool CUBIC>
 __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles, int per, int group)
 {
     const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
@@ -345,7 +393,9 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
         int k0, k1;
         ray_range(r, d1, d2, d3, &k0, &k1);
         const float Z0 = (float)(k0 - c3), Z1 = (float)(k1 - c3);
-        if (R[9] != 0.0f) {
+        if constexpr (CUBIC) {
+            acc = march_cubic(a.vcubic, d1, d2, d3, r, Z0, Z1);
+        } else if (R[9] != 0.0f) {
             const char* v0 = reinterpret_cast<const char*>(a.zy);
             acc = march<true, WIDE>(v0, v0 + 8 * (size_t)sy, sy, sxy, r, Z0, Z1);
         } else {
@@ -477,6 +527,47 @@ __global__ __launch_bounds__(256) void k_zdup_x(ModelSrc m, float2* __restrict__
     }
 }
 
+// The cubic projector's padded form (see gf_grid.h): padded node (x, y, z),
+// each in [-1, d], is the model node, or on an axis where it lies outside Keys'
+// boundary combination 3 f(0) - 3 f(1) + f(2) (and its mirror at the top),
+// applied on every such axis: a sum of up to 27 model nodes.  One thread per
+// padded node, y fastest.
+__device__ __forceinline__ int keys_taps(int i, int n, int idx[3], float w[3])
+{
+    if (i < 0) { idx[0] = 0; idx[1] = 1; idx[2] = 2; w[0] = 3.0f; w[1] = -3.0f; w[2] = 1.0f; return 3; }
+    if (i >= n) { idx[0] = n - 1; idx[1] = n - 2; idx[2] = n - 3; w[0] = 3.0f; w[1] = -3.0f; w[2] = 1.0f; return 3; }
+    idx[0] = i; w[0] = 1.0f;
+    return 1;
+}
+
+__global__ __launch_bounds__(256) void k_cubic_form(ModelSrc m, float* __restrict__ dst, int64_t total)
+{
+    const int py = m.d2 + 2, px = m.d1 + 2;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(e % py) - 1;
+        const int64_t xz = e / py;
+        const int x = (int)(xz % px) - 1, z = (int)(xz / px) - 1;
+        int ix[3], iy[3], iz[3];
+        float wx[3], wy[3], wz[3];
+        const int nx = keys_taps(x, m.d1, ix, wx), ny = keys_taps(y, m.d2, iy, wy), nz = keys_taps(z, m.d3, iz, wz);
+        float v = 0.0f;
+        for (int c = 0; c < nz; ++c)
+            for (int b = 0; b < ny; ++b)
+                for (int a = 0; a < nx; ++a) v += wz[c] * wy[b] * wx[a] * m.at(ix[a], iy[b], iz[c]);
+        dst[e] = v;
+    }
+}
+
+void cubic_launch(const ModelSrc& m, float* d_dst, hipStream_t st)
+{
+    GF_CHECK(m.d1 >= 4 && m.d2 >= 4 && m.d3 >= 4, GENFIRE_ERR_ARG,
+             "cubic projector: interp3 'cubic' needs at least 4 nodes along every axis");
+    const int64_t total = cubic_form_floats(m.d1, m.d2, m.d3);
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
+    k_cubic_form<<<(unsigned)blocks, 256, 0, st>>>(m, d_dst, total);
+    GF_LAUNCH_CHECK();
+}
+
 void zdup_launch(const ModelSrc& m, float* d_dst, bool yfast, hipStream_t st)
 {
     if (yfast) {
@@ -541,6 +632,19 @@ void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipS
 
 int64_t zdup_floats(int d1, int d2, int d3) { return 2 * (int64_t)(d1 + 1) * (d2 + 1) * d3; }
 
+int64_t cubic_form_floats(int d1, int d2, int d3) { return (int64_t)(d1 + 2) * (d2 + 2) * (d3 + 2); }
+
+void volume_cubic_form(const float* d_vol, int d1, int d2, int d3, float* d_dst, hipStream_t st)
+{
+    cubic_launch(ModelSrc{d_vol, d1, d2, d3, 0, 0, 0, d1, d2, d3}, d_dst, st);
+}
+
+void crop_cubic_form(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3,
+                     hipStream_t st)
+{
+    cubic_launch(ModelSrc{d_src, n1, n2, n3, d1 / 2, d2 / 2, d3 / 2, d1, d2, d3}, d_dst, st);
+}
+
 void volume_zdup(const float* d_vol, int d1, int d2, int d3, float* d_dst, bool yfast, hipStream_t st)
 {
     zdup_launch(ModelSrc{d_vol, d1, d2, d3, 0, 0, 0, d1, d2, d3}, d_dst, yfast, st);
@@ -552,9 +656,13 @@ void crop_zdup(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1,
     zdup_launch(ModelSrc{d_src, n1, n2, n3, d1 / 2, d2 / 2, d3 / 2, d1, d2, d3}, d_dst, yfast, st);
 }
 
-ProjForms project_forms(const float* Rm, int P, int d2)
+ProjForms project_forms(const float* Rm, int P, int d2, bool cubic)
 {
     ProjForms f;
+    if (cubic) {
+        f.cubic = true;
+        return f;
+    }
     bool all_exact = P > 0;
     for (int p = 0; p < P; ++p) {
         const float* r = Rm + (size_t)kRStride * p;
@@ -582,8 +690,9 @@ void launch_ray(const ProjArgs& a, int tiles, hipStream_t st)
     // size (d1 + 1)(d2 + 1) below 2^24
     const bool wide = 4 * zdup_floats(a.d1, a.d2, a.d3) > (int64_t)UINT32_MAX ||
                       (int64_t)(a.d1 + 1) * (a.d2 + 1) >= (1ll << 24);
-    if (wide) k_project<true><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
-    else k_project<false><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
+    if (a.vcubic) k_project<true, true><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
+    else if (wide) k_project<true, false><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
+    else k_project<false, false><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
     GF_LAUNCH_CHECK();
 }
 
@@ -597,7 +706,7 @@ void project(const ProjArgs& a, hipStream_t st)
              "measured projection empty or larger than the volume");
     GF_CHECK((int64_t)a.d1 * a.d2 * a.d3 < (1ll << 31), GENFIRE_ERR_UNSUPPORTED, "projector volume above 2^31 voxels");
     const int tiles = project_blocks(a.m1, a.m2);
-    if (a.four_ray) {
+    if (a.four_ray && !a.vcubic) {
         GF_CHECK(a.vol_y && a.d2 % 4 == 0, GENFIRE_ERR_ARG, "four-ray projector form needs the y-fastest model");
         const int per = (a.P + kXcd - 1) / kXcd, group = std::min(kProjGroup, per);
         const int groups = (per + group - 1) / group;
@@ -610,7 +719,11 @@ void project(const ProjArgs& a, hipStream_t st)
         GF_LAUNCH_CHECK();
         return;
     }
-    GF_CHECK(a.zy || a.zx, GENFIRE_ERR_ARG, "projector needs a z-duplicated model form");
+    if (a.vcubic)
+        GF_CHECK(a.d1 >= 4 && a.d2 >= 4 && a.d3 >= 4, GENFIRE_ERR_ARG,
+                 "cubic projector: interp3 'cubic' needs at least 4 nodes along every axis");
+    else
+        GF_CHECK(a.zy || a.zx, GENFIRE_ERR_ARG, "projector needs a z-duplicated model form");
     launch_ray(a, tiles, st);
 }
 

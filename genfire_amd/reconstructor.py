@@ -92,6 +92,9 @@ class GENFIRE_Reconstructor:
         self.interpolationCutoffDistance = 0.7
         self.griddingMethod = 1
         self.allowMultipleGridMatches = 1
+        # the R-factor monitor's interpolant: "linear" (trilinear) or "cubic"
+        # (interp3 'cubic', src/calculate3Dprojection_RealSpaceinterp.m:45,51)
+        self.projectorInterpolation = "linear"
         self.calculate_Rfree = 0
         # Rfree properties reconstruct.m:47-58 reads (undeclared in the reference class)
         self.numBinsRfree = 10
@@ -249,6 +252,7 @@ class GENFIRE_Reconstructor:
         p.RandomSeed = int(self.RandomSeed) if self.RandomSeed is not None else 0
         p.griddingMethod = int(self.griddingMethod)
         p.allowMultipleGridMatches = int(bool(self.allowMultipleGridMatches))
+        p.projectorInterpolation = {"linear": _lib.INTERP_LINEAR, "cubic": _lib.INTERP_CUBIC}[self.projectorInterpolation]
         return p
 
     def _ensure_job(self):
@@ -370,11 +374,13 @@ class GENFIRE_Reconstructor:
             self._job = None
 
 
-def calculate3Dprojection_RealSpaceinterp(Vol, phi, theta, psi, ctx: _lib.Context | None = None):
-    """src/calculate3Dprojection_RealSpaceinterp.m (trilinear interpolant)."""
-    return (ctx or default_context()).project(Vol, phi, theta, psi)
+def calculate3Dprojection_RealSpaceinterp(Vol, phi, theta, psi, ctx: _lib.Context | None = None,
+                                          interp: str = "linear"):
+    """src/calculate3Dprojection_RealSpaceinterp.m: interp "linear" (trilinear)
+    or "cubic" (the reference's interp3 'cubic', :45,51)."""
+    return (ctx or default_context()).project(Vol, phi, theta, psi, interp=interp)
 
 
-def Tian_calc_Rfactor_realspace(model, Projs, Angles, ctx: _lib.Context | None = None):
+def Tian_calc_Rfactor_realspace(model, Projs, Angles, ctx: _lib.Context | None = None, interp: str = "linear"):
     """src/Tian_calc_Rfactor_realspace.m:1-24 -> (Rtotal, Rarr, simuProjs)."""
-    return (ctx or default_context()).rfactor(model, np.asarray(Projs), np.asarray(Angles))
+    return (ctx or default_context()).rfactor(model, np.asarray(Projs), np.asarray(Angles), interp=interp)

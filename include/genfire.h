@@ -44,6 +44,10 @@ extern "C" {
 #define GENFIRE_ALG_GENFIRE 0        /* reconstruct.m:117-186                    */
 #define GENFIRE_ALG_DR 1             /* Douglas-Rachford on the same projections */
 
+/* projectorInterpolation: the projector's interpolant */
+#define GENFIRE_INTERP_LINEAR 0   /* trilinear (the default)       */
+#define GENFIRE_INTERP_CUBIC 1    /* interp3 'cubic', as calculate3Dprojection_RealSpaceinterp.m:45,51 */
+
 #define GENFIRE_PREC_FP32 0          /* gridding sums in FP32                    */
 #define GENFIRE_PREC_SPLIT_BF16 1    /* gridding GEMM in split BF16 (hi+lo)      */
 #define GENFIRE_PREC_FP16 2          /* gridding GEMM in FP16, per-projection scale */
@@ -78,6 +82,7 @@ typedef struct genfire_params {
     int64_t RandomSeed;                 /* 0     seed of the withheld-point permutation (:52-55) */
     int32_t griddingMethod;             /* 1     (:45)  1 FFT gridding, 2 DFT gridding */
     int32_t allowMultipleGridMatches;   /* 1     (:46)  FFT gridding: match every voxel within the cutoff */
+    int32_t projectorInterpolation;     /* GENFIRE_INTERP_LINEAR: the R-factor monitor's interpolant */
 } genfire_params;
 
 /* Library version, major*10000 + minor*100 + patch. */
@@ -219,6 +224,17 @@ int genfire_project(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, 
 int genfire_rfactor(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
                     const float* projections, int32_t p1, int32_t p2, int32_t num_projs,
                     const double* angles, double* Rtotal, double* Rarr, float* simuProjs);
+
+/* The same two calls with the reference's own interpolant, interp3(...,'cubic',0)
+ * (src/calculate3Dprojection_RealSpaceinterp.m:45,51): cubic convolution with
+ * Keys' a = -1/2 kernel and Keys' boundary nodes f(-1) = 3f(0) - 3f(1) + f(2)
+ * (MATLAB's 'cubic' on a uniform grid), extrapolation value 0 outside the node
+ * box.  Every model extent must be >= 4. */
+int genfire_project_cubic(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3,
+                          double phi, double theta, double psi, float* proj);
+int genfire_rfactor_cubic(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
+                          const float* projections, int32_t p1, int32_t p2, int32_t num_projs,
+                          const double* angles, double* Rtotal, double* Rarr, float* simuProjs);
 
 /* genfire_rfactor on device-resident buffers, allocating nothing.  No
  * counterpart in the reference: for callers that cannot allocate device memory

@@ -423,6 +423,121 @@ void go_project(const float* vol, int d1, int d2, int d3, const double R[9], dou
         }
 }
 
+/* interp3(...,'cubic',0) (src/calculate3Dprojection_RealSpaceinterp.m:45,51):
+ * on a uniform grid MATLAB's 'cubic' is cubic convolution (Keys, IEEE Trans.
+ * ASSP 29, 1981) with a = -1/2, weights for the nodes i-1, i, i+1, i+2 of
+ * t = f - i in [0, 1]
+ *   ((-t + 2) t - 1) t / 2,  ((3t - 5) t t + 2) / 2,  ((-3t + 4) t + 1) t / 2,  (t - 1) t t / 2,
+ * and Keys' boundary nodes f(-1) = 3 f(0) - 3 f(1) + f(2), f(n) = 3 f(n-1) -
+ * 3 f(n-2) + f(n-3) on every axis (both exact for quadratics).  A query is 0
+ * unless each coordinate lies in [0, n-1]; the cell is min(floor(f), n-2).
+ * MATLAB is absent here, so this kernel and boundary rule are the published
+ * cubic-convolution ones, parity unpinned against MATLAB itself. */
+static void keys_weights(double t, double w[4])
+{
+    w[0] = ((-t + 2.0) * t - 1.0) * t * 0.5;
+    w[1] = ((3.0 * t - 5.0) * t * t + 2.0) * 0.5;
+    w[2] = ((-3.0 * t + 4.0) * t + 1.0) * t * 0.5;
+    w[3] = (t - 1.0) * t * t * 0.5;
+}
+
+/* (n+2) samples of one axis, index -1..n, from a strided row of n >= 4 */
+static double keys_node(const double* row, int64_t stride, int n, int i)
+{
+    if (i < 0) return 3.0 * row[0] - 3.0 * row[stride] + row[2 * stride];
+    if (i >= n) return 3.0 * row[(int64_t)(n - 1) * stride] - 3.0 * row[(int64_t)(n - 2) * stride] +
+                       row[(int64_t)(n - 3) * stride];
+    return row[(int64_t)i * stride];
+}
+
+/* The volume padded by Keys' boundary nodes, (d1+2) x (d2+2) x (d3+2),
+ * x fastest, extended along x, then y, then z. */
+static double* keys_pad(const float* vol, int d1, int d2, int d3)
+{
+    const int p1 = d1 + 2, p2 = d2 + 2, p3 = d3 + 2;
+    double* a = (double*)calloc((size_t)p1 * p2 * p3, sizeof(double));
+    double* row = (double*)malloc(sizeof(double) * (size_t)(d1 > d2 ? (d1 > d3 ? d1 : d3) : (d2 > d3 ? d2 : d3)));
+    for (int z = 0; z < d3; ++z)
+        for (int y = 0; y < d2; ++y) {
+            for (int x = 0; x < d1; ++x) row[x] = vol[x + (int64_t)d1 * (y + (int64_t)d2 * z)];
+            for (int x = -1; x <= d1; ++x)
+                a[(x + 1) + (int64_t)p1 * ((y + 1) + (int64_t)p2 * (z + 1))] = keys_node(row, 1, d1, x);
+        }
+    for (int z = 0; z < d3; ++z)
+        for (int x = 0; x < p1; ++x) {
+            for (int y = 0; y < d2; ++y) row[y] = a[x + (int64_t)p1 * ((y + 1) + (int64_t)p2 * (z + 1))];
+            for (int y = -1; y <= d2; ++y)
+                a[x + (int64_t)p1 * ((y + 1) + (int64_t)p2 * (z + 1))] = keys_node(row, 1, d2, y);
+        }
+    for (int y = 0; y < p2; ++y)
+        for (int x = 0; x < p1; ++x) {
+            for (int z = 0; z < d3; ++z) row[z] = a[x + (int64_t)p1 * (y + (int64_t)p2 * (z + 1))];
+            for (int z = -1; z <= d3; ++z) a[x + (int64_t)p1 * (y + (int64_t)p2 * (z + 1))] = keys_node(row, 1, d3, z);
+        }
+    free(row);
+    return a;
+}
+
+static int cubic_axis(double f, int n, int* i0, double w[4])
+{
+    if (!(f >= 0.0) || f > (double)(n - 1)) return 0;
+    int i = (int)floor(f);
+    if (i > n - 2) i = n - 2;
+    *i0 = i;
+    keys_weights(f - (double)i, w);
+    return 1;
+}
+
+static double cubic_at(const double* a, int d1, int d2, int d3, double fx, double fy, double fz, int* inside)
+{
+    int a0, b0, e0;
+    double wx[4], wy[4], wz[4];
+    *inside = cubic_axis(fx, d1, &a0, wx) && cubic_axis(fy, d2, &b0, wy) && cubic_axis(fz, d3, &e0, wz);
+    if (!*inside) return 0.0;
+    const int64_t p1 = d1 + 2, p2 = d2 + 2;
+    double s = 0.0;
+    for (int c = 0; c < 4; ++c)
+        for (int b = 0; b < 4; ++b)
+            for (int q = 0; q < 4; ++q)   /* padded index of node a0 - 1 + q is a0 + q */
+                s += wz[c] * wy[b] * wx[q] * a[(a0 + q) + p1 * ((b0 + b) + p2 * (int64_t)(e0 + c))];
+    return s;
+}
+
+/* the cubic interpolant at np points (x, y, z in node units), 0 outside */
+void go_interp_cubic(const float* vol, int d1, int d2, int d3, const double* pts, int64_t np, double* out)
+{
+    double* a = keys_pad(vol, d1, d2, d3);
+    for (int64_t q = 0; q < np; ++q) {
+        int in;
+        out[q] = cubic_at(a, d1, d2, d3, pts[3 * q], pts[3 * q + 1], pts[3 * q + 2], &in);
+    }
+    free(a);
+}
+
+/* calculate3Dprojection_RealSpaceinterp with interp3 'cubic' (see above) */
+void go_project_cubic(const float* vol, int d1, int d2, int d3, const double R[9], double* out)
+{
+    const int c1 = d1 / 2, c2 = d2 / 2, c3 = d3 / 2;
+    double* a = keys_pad(vol, d1, d2, d3);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 0; j < d2; ++j)
+        for (int i = 0; i < d1; ++i) {
+            double X = (double)(i - c1), Y = (double)(j - c2);
+            double acc = 0.0;
+            for (int k = 0; k < d3; ++k) {
+                double Z = (double)(k - c3);
+                double rx = (R[0] * X + R[3] * Y) + R[6] * Z;
+                double ry = (R[1] * X + R[4] * Y) + R[7] * Z;
+                double rz = (R[2] * X + R[5] * Y) + R[8] * Z;
+                int in;
+                const double v = cubic_at(a, d1, d2, d3, rx + c1, ry + c2, rz + c3, &in);
+                if (in) acc += v;
+            }
+            out[i + (int64_t)d1 * j] = acc;
+        }
+    free(a);
+}
+
 /* ------------------------------------------------------------------------- */
 /* Type-2 NUFFT interpolation (the CPU gridding route, nufft2d2 at            */
 /* fillInFourierGrid_DFT.m:249-254), for the CPU baseline                     */

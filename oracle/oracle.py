@@ -73,6 +73,10 @@ def lib():
         L.go_zyz.restype = None
         L.go_project.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
         L.go_project.restype = None
+        L.go_project_cubic.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
+        L.go_project_cubic.restype = None
+        L.go_interp_cubic.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int64, _f64p]
+        L.go_interp_cubic.restype = None
         L.go_nufft_interp.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_int64, _f32p, _f32p, _f64p]
         L.go_nufft_interp.restype = None
@@ -605,17 +609,31 @@ def zyz(phi: float, theta: float, psi: float) -> np.ndarray:
     return R.reshape(3, 3, order="F")
 
 
-def project(vol: np.ndarray, phi: float, theta: float, psi: float) -> np.ndarray:
-    """calculate3Dprojection_RealSpaceinterp(Vol,phi,theta,psi) with trilinear
-    interpolation and extrapolation value 0 (FP64).  The reference calls
-    interp3(...,'cubic',0) (:45,51); the trilinear interpolant is the one this
-    build computes, the cubic kernel is parity unpinned."""
+def project(vol: np.ndarray, phi: float, theta: float, psi: float, interp: str = "linear") -> np.ndarray:
+    """calculate3Dprojection_RealSpaceinterp(Vol,phi,theta,psi) with extrapolation
+    value 0 (FP64): trilinear interpolation (interp="linear", the library's
+    default), or interp3's 'cubic' (:45,51), cubic convolution with
+    Keys' a = -1/2 kernel and boundary nodes (interp="cubic", genfire_oracle.c
+    go_project_cubic; MATLAB is absent, so its agreement with MATLAB itself is
+    parity unpinned)."""
     d1, d2, d3 = vol.shape
     R = zyz(phi, theta, psi)
     out = np.zeros(d1 * d2, np.float64)
-    lib().go_project(np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F")),
-                     d1, d2, d3, np.ascontiguousarray(R.ravel(order="F")), out)
+    fn = {"linear": lib().go_project, "cubic": lib().go_project_cubic}[interp]
+    fn(np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F")),
+       d1, d2, d3, np.ascontiguousarray(R.ravel(order="F")), out)
     return out.reshape(d1, d2, order="F")
+
+
+def interp_cubic(vol: np.ndarray, pts: np.ndarray) -> np.ndarray:
+    """The cubic-convolution interpolant of vol at pts (k x 3, node units along
+    x, y, z), 0 outside the closed node box (genfire_oracle.c go_interp_cubic)."""
+    d1, d2, d3 = vol.shape
+    pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, 3))
+    out = np.zeros(len(pts), np.float64)
+    lib().go_interp_cubic(np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F")),
+                          d1, d2, d3, pts.ravel(), len(pts), out)
+    return out
 
 
 def quat_rot(vec, theta_deg: float) -> np.ndarray:
@@ -642,21 +660,22 @@ def project_general(vol: np.ndarray, phi: float, theta: float, psi: float, v1, v
     return out.reshape(d1, d2, order="F")
 
 
-def rfactors(model: np.ndarray, projections: np.ndarray, angles: np.ndarray):
+def rfactors(model: np.ndarray, projections: np.ndarray, angles: np.ndarray, interp: str = "linear"):
     """R-factor monitor body, reconstruct.m:171-178: per projection
-    R1 = sum|back-pj|/sum|pj| and R2 = ||back-pj||_F/||pj||_F."""
+    R1 = sum|back-pj|/sum|pj| and R2 = ||back-pj||_F/||pj||_F (``project``'s
+    interpolant ``interp``)."""
     P = projections.shape[2]
     R1 = np.zeros(P)
     R2 = np.zeros(P)
     for i in range(P):
         pj = np.asarray(projections[:, :, i], np.float64)
-        back = project(model, *angles[i])
+        back = project(model, *angles[i], interp=interp)
         R1[i] = np.sum(np.abs(back - pj)) / np.sum(np.abs(pj))
         R2[i] = np.linalg.norm(back - pj) / np.linalg.norm(pj)
     return R1, R2
 
 
-def rfactor_posthoc(model: np.ndarray, projections: np.ndarray, angles: np.ndarray):
+def rfactor_posthoc(model: np.ndarray, projections: np.ndarray, angles: np.ndarray, interp: str = "linear"):
     """src/Tian_calc_Rfactor_realspace.m:1-24 with the projector defined (the
     reference calls a missing ..._RealSpaceinterpZYX, :15): mean of
     sum||back|-|pj|| / sum|pj| after cropping the projection to the pj size."""
@@ -665,7 +684,7 @@ def rfactor_posthoc(model: np.ndarray, projections: np.ndarray, angles: np.ndarr
     sims = np.zeros(projections.shape, np.float32)
     for i in range(P):
         pj = np.asarray(projections[:, :, i], np.float64)
-        back = stripzero(project(model, *angles[i]), pj.shape)
+        back = stripzero(project(model, *angles[i], interp=interp), pj.shape)
         R[i] = np.sum(np.abs(np.abs(back) - np.abs(pj))) / np.sum(np.abs(pj))
         sims[:, :, i] = back
     return float(np.mean(R)), R, sims
@@ -734,7 +753,7 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
                 support_on: bool = True, algorithm: str = "genfire", precision: str = "fp32",
                 initial_object: np.ndarray | None = None, monitor_R: bool = False,
                 loop_length: int = 10, projections=None, angles=None, rfree=None,
-                save_every: int = 0) -> ReconResult:
+                save_every: int = 0, projector_interp: str = "linear") -> ReconResult:
     """reconstruct.m:28-194 (GENFIRE, algorithm='genfire').
 
     algorithm='dr' is the library's Douglas-Rachford variant composed of the same two
@@ -839,7 +858,7 @@ def reconstruct(measuredK: np.ndarray, mask: np.ndarray, support: np.ndarray, di
             model = np.fft.fftshift(pf)
             sl = tuple(slice(a - 1, b) for a, b in vol_ind)
             model = model[sl]
-            R1, R2 = rfactors(model, projections, angles)
+            R1, R2 = rfactors(model, projections, angles, interp=projector_interp)
             j = it // loop_length - 1
             errR[j] = np.mean(R1)
             Rrec[:, j] = R1
