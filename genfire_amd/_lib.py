@@ -36,7 +36,8 @@ EXPORTS = [
     "genfire_job_get_errK", "genfire_job_get_errR", "genfire_job_get_Rfree", "genfire_job_submit", "genfire_job_poll",
     "genfire_job_wait", "genfire_job_timing", "genfire_job_set_profiling",
     "genfire_job_kernel_stats", "genfire_job_graph_replays", "genfire_project", "genfire_rfactor", "genfire_fftn",
-    "genfire_project_general", "genfire_rfactor_general", "genfire_project_cubic", "genfire_rfactor_cubic", "genfire_rfactor_noalloc", "genfire_job_set_snapshots",
+    "genfire_project_general", "genfire_rfactor_general", "genfire_project_cubic", "genfire_rfactor_cubic",
+    "genfire_project_general_cubic", "genfire_rfactor_general_cubic", "genfire_rfactor_noalloc", "genfire_job_set_snapshots",
     "genfire_server_config_default", "genfire_server_create", "genfire_server_destroy",
     "genfire_server_submit", "genfire_server_status", "genfire_server_wait", "genfire_server_drain",
     "genfire_server_result", "genfire_server_release", "genfire_server_get_stats",
@@ -170,6 +171,9 @@ def load():
         "genfire_project_general": ([vp, pf, i32, i32, i32, f64, f64, f64, pd, pd, pd, pf], ctypes.c_int),
         "genfire_rfactor_general": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pd, pd, pd, pf],
                                     ctypes.c_int),
+        "genfire_project_general_cubic": ([vp, pf, i32, i32, i32, f64, f64, f64, pd, pd, pd, pf], ctypes.c_int),
+        "genfire_rfactor_general_cubic": ([vp, pf, i32, i32, i32, pf, i32, i32, i32, pd, pd, pd, pd, pd, pd, pf],
+                                          ctypes.c_int),
         "genfire_rfactor_noalloc": ([vp, vp, i32, i32, i32, vp, i32, i32, i32, pd, pd, pd, vp], ctypes.c_int),
         "genfire_server_config_default": ([P(ServerConfig)], ctypes.c_int),
         "genfire_server_create": ([vp, P(ServerConfig), P(vp)], ctypes.c_int),
@@ -244,19 +248,22 @@ class Context:
                                          float(theta), float(psi), _ptr(out, ctypes.c_float)), name)
         return out.reshape(d1, d2, order="F")
 
-    def project_general(self, vol: np.ndarray, phi: float, theta: float, psi: float, v1, v2, v3) -> np.ndarray:
-        """calculate3Dprojection_RealSpaceinterp_general (trilinear)."""
+    def project_general(self, vol: np.ndarray, phi: float, theta: float, psi: float, v1, v2, v3,
+                        interp: str = "linear") -> np.ndarray:
+        """calculate3Dprojection_RealSpaceinterp_general: trilinear, or interp3 'cubic'."""
         d1, d2, d3 = vol.shape
         v = np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F"))
         vec = [np.ascontiguousarray(np.asarray(x, np.float64).reshape(3)) for x in (v1, v2, v3)]
         out = np.zeros(d1 * d2, np.float32)
-        self.check(self.L.genfire_project_general(self.h, _ptr(v, ctypes.c_float), d1, d2, d3, float(phi),
-                                                  float(theta), float(psi), *[_ptr(x, ctypes.c_double) for x in vec],
-                                                  _ptr(out, ctypes.c_float)), "genfire_project_general")
+        name = {"linear": "genfire_project_general", "cubic": "genfire_project_general_cubic"}[interp]
+        self.check(getattr(self.L, name)(self.h, _ptr(v, ctypes.c_float), d1, d2, d3, float(phi),
+                                         float(theta), float(psi), *[_ptr(x, ctypes.c_double) for x in vec],
+                                         _ptr(out, ctypes.c_float)), name)
         return out.reshape(d1, d2, order="F")
 
-    def rfactor_general(self, model: np.ndarray, projections: np.ndarray, angles: np.ndarray, v1, v2, v3):
-        """Tian_calc_Rfactor_realspace with the general-axis projector."""
+    def rfactor_general(self, model: np.ndarray, projections: np.ndarray, angles: np.ndarray, v1, v2, v3,
+                        interp: str = "linear"):
+        """Tian_calc_Rfactor_realspace with the general-axis projector (trilinear or cubic)."""
         d1, d2, d3 = model.shape
         p1, p2, P = projections.shape
         v = np.ascontiguousarray(np.asarray(model, np.float32).ravel(order="F"))
@@ -266,11 +273,12 @@ class Context:
         rt = np.zeros(1)
         ra = np.zeros(P)
         sim = np.zeros(p1 * p2 * P, np.float32)
-        self.check(self.L.genfire_rfactor_general(self.h, _ptr(v, ctypes.c_float), d1, d2, d3,
-                                                  _ptr(m, ctypes.c_float), p1, p2, P, _ptr(ang, ctypes.c_double),
-                                                  *[_ptr(x, ctypes.c_double) for x in vec],
-                                                  _ptr(rt, ctypes.c_double), _ptr(ra, ctypes.c_double),
-                                                  _ptr(sim, ctypes.c_float)), "genfire_rfactor_general")
+        name = {"linear": "genfire_rfactor_general", "cubic": "genfire_rfactor_general_cubic"}[interp]
+        self.check(getattr(self.L, name)(self.h, _ptr(v, ctypes.c_float), d1, d2, d3,
+                                         _ptr(m, ctypes.c_float), p1, p2, P, _ptr(ang, ctypes.c_double),
+                                         *[_ptr(x, ctypes.c_double) for x in vec],
+                                         _ptr(rt, ctypes.c_double), _ptr(ra, ctypes.c_double),
+                                         _ptr(sim, ctypes.c_float)), name)
         return float(rt[0]), ra, sim.reshape(p1, p2, P, order="F")
 
     def rfactor(self, model: np.ndarray, projections: np.ndarray, angles: np.ndarray, interp: str = "linear"):

@@ -614,6 +614,31 @@ int genfire_rfactor_general(genfire_ctx* ctx, const float* model, int32_t d1, in
     });
 }
 
+int genfire_project_general_cubic(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3,
+                                  double phi, double theta, double psi, const double vector1[3],
+                                  const double vector2[3], const double vector3[3], float* proj)
+{
+    if (!ctx || !vol || !proj || !vector1 || !vector2 || !vector3) return GENFIRE_ERR_ARG;
+    const double ang[3] = {phi, theta, psi};
+    double vec[9];
+    for (int i = 0; i < 3; ++i) { vec[i] = vector1[i]; vec[3 + i] = vector2[i]; vec[6 + i] = vector3[i]; }
+    return guarded(ctx, [&] { project_one(ctx, vol, d1, d2, d3, ang, vec, proj, true); });
+}
+
+int genfire_rfactor_general_cubic(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
+                                  const float* projections, int32_t p1, int32_t p2, int32_t P, const double* angles,
+                                  const double vector1[3], const double vector2[3], const double vector3[3],
+                                  double* Rtotal, double* Rarr, float* simu)
+{
+    if (!ctx || !model || !projections || !angles || P < 1 || !vector1 || !vector2 || !vector3)
+        return GENFIRE_ERR_ARG;
+    double vec[9];
+    for (int i = 0; i < 3; ++i) { vec[i] = vector1[i]; vec[3 + i] = vector2[i]; vec[6 + i] = vector3[i]; }
+    return guarded(ctx, [&] {
+        rfactor_all(ctx, model, d1, d2, d3, projections, p1, p2, P, angles, vec, Rtotal, Rarr, simu, true);
+    });
+}
+
 int genfire_rfactor_noalloc(genfire_ctx* ctx, const float* d_model, int32_t d1, int32_t d2, int32_t d3,
                             const float* d_projections, int32_t p1, int32_t p2, int32_t P, const double* angles,
                             double* Rtotal, double* Rarr, float* d_simuProjs)

@@ -268,6 +268,17 @@ int genfire_rfactor_general(genfire_ctx* ctx, const float* model, int32_t d1, in
                             const double* angles, const double vector1[3], const double vector2[3],
                             const double vector3[3], double* Rtotal, double* Rarr, float* simuProjs);
 
+/* The two general-axis calls with the reference's own interp3 'cubic'
+ * (calculate3Dprojection_RealSpaceinterp_general.m:37,43), as
+ * genfire_project_cubic. */
+int genfire_project_general_cubic(genfire_ctx* ctx, const float* vol, int32_t d1, int32_t d2, int32_t d3,
+                                  double phi, double theta, double psi, const double vector1[3],
+                                  const double vector2[3], const double vector3[3], float* proj);
+int genfire_rfactor_general_cubic(genfire_ctx* ctx, const float* model, int32_t d1, int32_t d2, int32_t d3,
+                                  const float* projections, int32_t p1, int32_t p2, int32_t num_projs,
+                                  const double* angles, const double vector1[3], const double vector2[3],
+                                  const double vector3[3], double* Rtotal, double* Rarr, float* simuProjs);
+
 /* fftn (inverse = 0) or ifftn (inverse = 1, scaled by 1/N) of an n1 x n2 x n3
  * interleaved-complex array in place, with this library's 3D FFT (MATLAB's
  * fftn/ifftn at reconstruct.m:133,165 and src/my_ifft.m:11). */

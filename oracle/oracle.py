@@ -648,15 +648,17 @@ def quat_rot(vec, theta_deg: float) -> np.ndarray:
                      [2 * x * z + 2 * w * y, 2 * y * z - 2 * w * x, 1 - 2 * x * x - 2 * y * y]])
 
 
-def project_general(vol: np.ndarray, phi: float, theta: float, psi: float, v1, v2, v3) -> np.ndarray:
+def project_general(vol: np.ndarray, phi: float, theta: float, psi: float, v1, v2, v3,
+                    interp: str = "linear") -> np.ndarray:
     """src/calculate3Dprojection_RealSpaceinterp_general.m with R =
-    Q(v1,phi) Q(v2,theta) Q(v3,psi) (:3-7) and the trilinear interpolant of
-    ``project`` (FP64)."""
+    Q(v1,phi) Q(v2,theta) Q(v3,psi) (:3-7) and the interpolant ``interp`` of
+    ``project`` (FP64; the reference's is 'cubic', :37,43)."""
     d1, d2, d3 = vol.shape
     R = quat_rot(v1, phi) @ quat_rot(v2, theta) @ quat_rot(v3, psi)
     out = np.zeros(d1 * d2, np.float64)
-    lib().go_project(np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F")),
-                     d1, d2, d3, np.ascontiguousarray(R.ravel(order="F")), out)
+    fn = {"linear": lib().go_project, "cubic": lib().go_project_cubic}[interp]
+    fn(np.ascontiguousarray(np.asarray(vol, np.float32).ravel(order="F")),
+       d1, d2, d3, np.ascontiguousarray(R.ravel(order="F")), out)
     return out.reshape(d1, d2, order="F")
 
 

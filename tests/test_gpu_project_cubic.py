@@ -98,3 +98,27 @@ def test_monitor_cubic_matches_oracle(ctx, O, alg):
     ref_lin = O.reconstruct(mk, mk != 0, sup, 32, 32, iters, mode=1, algorithm=alg, precision="fp64",
                             monitor_R=True, loop_length=5, projections=projs, angles=ang)
     assert np.abs(R1 - ref_lin.Rarr_record).max() > 1e-4
+
+
+@pytest.mark.parametrize("vectors,angles", [
+    (((0, 0, 1), (0, 1, 0), (1, 0, 0)), (10.0, -40.0, 25.0)),
+    (((1, 1, 0), (0, 1, 1), (1, 0, 1)), (17.0, 33.0, -21.0)),
+])
+def test_project_general_cubic_matches_oracle(ctx, O, vectors, angles):
+    """calculate3Dprojection_RealSpaceinterp_general.m:37,43 ('cubic') with the
+    general rotation axes, and its post-hoc R-factor."""
+    print(f"[cubic] general {vectors} {angles}", flush=True)
+    rng = np.random.default_rng(6)
+    vol = rng.uniform(0, 1, (40, 36, 38)).astype(np.float32)
+    got = ctx.project_general(vol, *angles, *vectors, interp="cubic")
+    ref = O.project_general(vol, *angles, *vectors, interp="cubic")
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 2e-5
+    ang = np.array([angles, (angles[0], -angles[1], angles[2])])
+    projs = np.stack([O.project_general(vol, *a, *vectors, interp="cubic") for a in ang], 2)
+    projs = (projs * rng.uniform(0.9, 1.1, projs.shape)).astype(np.float32)
+    R, Rarr, _ = ctx.rfactor_general(vol, projs, ang, *vectors, interp="cubic")
+    for i, a in enumerate(ang):
+        back = O.project_general(vol, *a, *vectors, interp="cubic")
+        pj = projs[:, :, i].astype(np.float64)
+        ref_r = np.sum(np.abs(np.abs(back) - np.abs(pj))) / np.sum(np.abs(pj))
+        assert abs(Rarr[i] - ref_r) < 1e-5
