@@ -267,7 +267,6 @@ __global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
     const Tile g = tile_of(a.axis, it.n1, it.n2, it.n3, TP, blockIdx.x, a.yb, a.zb);
     const bool valid = p < g.valid;
     float2* __restrict__ K = (MODE == kPlain) ? a.data : it.K;
-    float2* row = lds + p * N;
     const int64_t pbase = g.base + p * g.ps;
     float2* twt = lds + TP * N;   // stage twiddle table past the tile
     reg_tw_fill<N>(twt, a.tw, t, NT);
@@ -281,23 +280,20 @@ __global__ __launch_bounds__(reg_threads<N>()) void k_fft_reg(PassArgs a)
             v[i * R0 + r] = valid ? pass_load<MODE>(a, K, pbase + e * g.es) : make_float2(0.0f, 0.0f);
         }
 
-    reg_fft<N, INV>(v, row, j, p, twt);
+    reg_fft<N, INV>(v, lds, p * N, j, p, twt);
 
     if (MODE == kFused) {
         // natural order into LDS, replace, back into the first-stage layout
-#pragma unroll
-        for (int i = 0; i < E / RL; ++i)
-#pragma unroll
-            for (int q = 0; q < RL; ++q) row[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
+        reg_store_x<N, E, RL>(lds, p * N, v, j, p);
         __syncthreads();
         const int tpr = (it.n1 + TP - 1) / TP;
         const int x0 = (int)(blockIdx.x % tpr) * TP, y = (int)(blockIdx.x / tpr);
         const double err = csr_replace<true>(lds, N, it, x0, y, g.valid, t, NT);
         wave_partial(err, it.err_part, (int64_t)blockIdx.x * (NT / 64) + (t >> 6));
         __syncthreads();
-        reg_read<N, E, R0>(row, v, j, p);
+        reg_read_x<N, E, R0>(lds, p * N, v, j, p);
         __syncthreads();
-        reg_fft<N, true>(v, row, j, p, twt);
+        reg_fft<N, true>(v, lds, p * N, j, p, twt);
     }
 
     if (!valid) return;

@@ -198,6 +198,38 @@ template <bool INV> struct Dft<16, INV> {
 // column of 16 different pencils (rows N*8 bytes apart would otherwise share a bank).
 __device__ __forceinline__ int swz(int col, int p) { return col ^ (((col >> 3) ^ p) & 15); }
 
+// The swizzle is linear over xor: swz(col, p) = swz_lin(col) ^ (p & 15).  So
+// for an element whose index splits into a lane part and a compile-time part
+// with no bits in common, e = lane_e | ce, the position is a per-lane value xor
+// a constant, swz_lin(lane_e) ^ (p & 15) ^ swz_lin(ce).  With the lane value
+// below 2^LB, the constant's bits from LB up add instead (an immediate offset of
+// the LDS instruction) and its low bits take one xor, shared by every access
+// with the same low bits.  The *_x forms below address LDS this way; they take
+// the kernel's dynamic LDS base and the row's element offset from it (a
+// multiple of 2^LB) and place every element exactly where swz does.
+__host__ __device__ constexpr int swz_lin(int c) { return c ^ ((c >> 3) & 15); }
+constexpr bool is_pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
+constexpr int ilog2(int x)
+{
+    int r = 0;
+    while ((1 << r) < x) ++r;
+    return r;
+}
+constexpr int lane_bits(int bound) { return ilog2(bound) < 4 ? 4 : ilog2(bound); }
+
+// Byte offset of the lane part: row offset rowoff (elements) plus the lane value.
+__device__ __forceinline__ uint32_t lane_bytes(int rowoff, int lane_e, int p)
+{
+    return 8u * (uint32_t)(rowoff + (swz_lin(lane_e) ^ (p & 15)));
+}
+
+template <int LB>
+__device__ __forceinline__ float2& lds_at(float2* lds, uint32_t lb, int ce)
+{
+    const int K = swz_lin(ce), klo = K & ((1 << LB) - 1), khi = K - klo;
+    return *reinterpret_cast<float2*>(reinterpret_cast<char*>(lds) + ((lb ^ (uint32_t)(8 * klo)) + (uint32_t)(8 * khi)));
+}
+
 // errK numerator (reconstruct.m:135) and Fourier replacement (:163) on the
 // natural-order spectrum of the tile's z-pencils [x0, x0+valid) at row y.  The
 // measured voxels come from the per-pencil list (meta = z | x<<11 | enforced<<31).
@@ -311,10 +343,70 @@ __device__ __forceinline__ void reg_read(const float2* row, float2 (&v)[E], int 
         for (int r = 0; r < R; ++r) v[i * R + r] = row[swz(j + T * i + r * (N / R), p)];
 }
 
-// In-register Stockham FFT of the pencils of a tile; twt is the stage table of
-// reg_tw_fill (normally in LDS).
+// reg_write and reg_read on the row at element offset rowoff from the dynamic
+// LDS base lds (rowoff = p N for pencil p).
+//   read: e = j | (T i + r N/R), T a power of two.
+//   write, R and LS powers of two with LS <= T: b = j + T i, k = j & (LS-1),
+//     e = (R (j & ~(LS-1)) + k) | (R T i + q LS);
+//   write with b = j + T i < LS always (T E/R <= LS): e = j | (T i + q LS).
+// Other stages (radix 3, 5, 6, 10) take reg_write's swz addressing.
+template <int N, int E, int R, int LS>
+__device__ __forceinline__ void reg_write_x(float2* lds, int rowoff, const float2 (&v)[E], int j, int p)
+{
+    constexpr int T = N / E;
+    if constexpr (is_pow2(R) && is_pow2(LS) && is_pow2(T) && LS <= T) {
+        constexpr int LB = lane_bits(R * T);
+        static_assert(N % (1 << LB) == 0, "row offsets p N are multiples of 2^LB");
+        const uint32_t lb = lane_bytes(rowoff, R * (j & ~(LS - 1)) + (j & (LS - 1)), p);
+#pragma unroll
+        for (int i = 0; i < E / R; ++i)
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds_at<LB>(lds, lb, R * T * i + q * LS) = v[i * R + q];
+    } else if constexpr (is_pow2(LS) && is_pow2(T) && T * (E / R) <= LS) {
+        constexpr int LB = lane_bits(T);
+        static_assert(N % (1 << LB) == 0, "row offsets p N are multiples of 2^LB");
+        const uint32_t lb = lane_bytes(rowoff, j, p);
+#pragma unroll
+        for (int i = 0; i < E / R; ++i)
+#pragma unroll
+            for (int q = 0; q < R; ++q) lds_at<LB>(lds, lb, T * i + q * LS) = v[i * R + q];
+    } else {
+        reg_write<N, E, R, LS>(lds + rowoff, v, j, p);
+    }
+}
+
+template <int N, int E, int R>
+__device__ __forceinline__ void reg_read_x(float2* lds, int rowoff, float2 (&v)[E], int j, int p)
+{
+    constexpr int T = N / E, LB = lane_bits(T);
+    static_assert(is_pow2(T) && N % (1 << LB) == 0, "lane part j, row offsets p N multiples of 2^LB");
+    const uint32_t lb = lane_bytes(rowoff, j, p);
+#pragma unroll
+    for (int i = 0; i < E / R; ++i)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[i * R + r] = lds_at<LB>(lds, lb, T * i + r * (N / R));
+}
+
+// The natural-order store of a transform's outputs, element j + T i + q N/RL
+// from register i RL + q (the output slots of the last stage of radix RL).
+template <int N, int E, int RL>
+__device__ __forceinline__ void reg_store_x(float2* lds, int rowoff, const float2 (&v)[E], int j, int p)
+{
+    constexpr int T = N / E, LB = lane_bits(T);
+    static_assert(is_pow2(T) && N % (1 << LB) == 0, "lane part j, row offsets p N multiples of 2^LB");
+    const uint32_t lb = lane_bytes(rowoff, j, p);
+#pragma unroll
+    for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) lds_at<LB>(lds, lb, T * i + q * (N / RL)) = v[i * RL + q];
+}
+
+// In-register Stockham FFT of the pencils of a tile: pencil p's row lies at
+// element offset rowoff = p N from the dynamic LDS base lds; twt is the stage
+// table of reg_tw_fill (normally in LDS).
 template <int N, bool INV>
-__device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row, int j, int p, const float2* twt)
+__device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* lds, int rowoff, int j, int p,
+                                        const float2* twt)
 {
     using PL = RegPlan<N>;
     constexpr int E = PL::E, R0 = PL::R0, R1 = PL::R1, R2 = PL::R2, R3 = PL::R3;
@@ -323,27 +415,26 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[RegPlan<N>::E], float2* row,
     static_assert(((R0 * R1) & (R0 * R1 - 1)) == 0 && (R3 == 1 || ((R0 * R1 * R2) & (R0 * R1 * R2 - 1)) == 0),
                   "the span of every stage but the last must be a power of two");
     reg_stage<N, E, R0, 1, INV>(v, j, twt);
-    reg_write<N, E, R0, 1>(row, v, j, p);
+    reg_write_x<N, E, R0, 1>(lds, rowoff, v, j, p);
     __syncthreads();
-    reg_read<N, E, R1>(row, v, j, p);
+    reg_read_x<N, E, R1>(lds, rowoff, v, j, p);
     __syncthreads();
     reg_stage<N, E, R1, R0, INV>(v, j, twt);
     if constexpr (R2 > 1) {
-        reg_write<N, E, R1, R0>(row, v, j, p);
+        reg_write_x<N, E, R1, R0>(lds, rowoff, v, j, p);
         __syncthreads();
-        reg_read<N, E, R2>(row, v, j, p);
+        reg_read_x<N, E, R2>(lds, rowoff, v, j, p);
         __syncthreads();
         reg_stage<N, E, R2, R0 * R1, INV>(v, j, twt + (R1 - 1) * R0);
     }
     if constexpr (R3 > 1) {
-        reg_write<N, E, R2, R0 * R1>(row, v, j, p);
+        reg_write_x<N, E, R2, R0 * R1>(lds, rowoff, v, j, p);
         __syncthreads();
-        reg_read<N, E, R3>(row, v, j, p);
+        reg_read_x<N, E, R3>(lds, rowoff, v, j, p);
         __syncthreads();
         reg_stage<N, E, R3, R0 * R1 * R2, INV>(v, j, twt + (R1 - 1) * R0 + (R2 - 1) * R0 * R1);
     }
 }
-
 
 // Orders the LDS accesses of one wave: a pencil whose T lanes lie in one wave
 // (T divides 64) exchanges its elements through LDS with no workgroup barrier.
@@ -359,28 +450,29 @@ __device__ __forceinline__ void wave_sync()
 // reg_fft for pencils that each lie in one wave: the stage exchanges are
 // ordered per wave, with no workgroup barrier.
 template <int N, bool INV>
-__device__ __forceinline__ void reg_fft_wave(float2 (&v)[RegPlan<N>::E], float2* row, int j, int p, const float2* twt)
+__device__ __forceinline__ void reg_fft_wave(float2 (&v)[RegPlan<N>::E], float2* lds, int rowoff, int j, int p,
+                                             const float2* twt)
 {
     using PL = RegPlan<N>;
     constexpr int E = PL::E, R0 = PL::R0, R1 = PL::R1, R2 = PL::R2, R3 = PL::R3;
     static_assert(64 % (N / E) == 0, "a pencil's lanes must lie in one wave");
     reg_stage<N, E, R0, 1, INV>(v, j, twt);
-    reg_write<N, E, R0, 1>(row, v, j, p);
+    reg_write_x<N, E, R0, 1>(lds, rowoff, v, j, p);
     wave_sync();
-    reg_read<N, E, R1>(row, v, j, p);
+    reg_read_x<N, E, R1>(lds, rowoff, v, j, p);
     wave_sync();
     reg_stage<N, E, R1, R0, INV>(v, j, twt);
     if constexpr (R2 > 1) {
-        reg_write<N, E, R1, R0>(row, v, j, p);
+        reg_write_x<N, E, R1, R0>(lds, rowoff, v, j, p);
         wave_sync();
-        reg_read<N, E, R2>(row, v, j, p);
+        reg_read_x<N, E, R2>(lds, rowoff, v, j, p);
         wave_sync();
         reg_stage<N, E, R2, R0 * R1, INV>(v, j, twt + (R1 - 1) * R0);
     }
     if constexpr (R3 > 1) {
-        reg_write<N, E, R2, R0 * R1>(row, v, j, p);
+        reg_write_x<N, E, R2, R0 * R1>(lds, rowoff, v, j, p);
         wave_sync();
-        reg_read<N, E, R3>(row, v, j, p);
+        reg_read_x<N, E, R3>(lds, rowoff, v, j, p);
         wave_sync();
         reg_stage<N, E, R3, R0 * R1 * R2, INV>(v, j, twt + (R1 - 1) * R0 + (R2 - 1) * R0 * R1);
     }

@@ -185,12 +185,8 @@ __device__ __forceinline__ void r2c_store(const HalfIterArgs& a, float2* lds, fl
     constexpr int E = PL::E, T = M / E, RL = reg_last_radix<M>();
     const int t = phase_tid(), j = t % T, p = t / T;
     const int n2 = a.n2, h = M + 1;
-    float2* rowl = lds + p * M;
-    reg_fft_wave<M, false>(v, rowl, j, p, twM);
-#pragma unroll
-    for (int i = 0; i < E / RL; ++i)
-#pragma unroll
-        for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (M / RL), p)] = v[i * RL + q];
+    reg_fft_wave<M, false>(v, lds, p * M, j, p, twM);
+    reg_store_x<M, E, RL>(lds, p * M, v, j, p);
     __syncthreads();
     constexpr int XR = xrows<M>();
     const int pp = t % XR, c = t / XR;
@@ -303,19 +299,13 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
             v[i * R0 + r] = (z_in && inb(y, N, a.by)) ? src[y] : make_float2(0.0f, 0.0f);
         }
     __syncthreads();   // the twiddle table; the pencils' transforms are wave-local
-    float2* rowl = lds + p * N;
-    reg_fft_wave<N, false>(v, rowl, j, p, twl);
-#pragma unroll
-    for (int i = 0; i < E / RL; ++i)
-#pragma unroll
-        for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
+    reg_fft_wave<N, false>(v, lds, p * N, j, p, twl);
+    reg_store_x<N, E, RL>(lds, p * N, v, j, p);
     __syncthreads();
     const int pp = t % kHP, c = t / kHP;
-    const float2* rp = lds + pp * N;
     float2* dst = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
     float2 w[N / T];
-#pragma unroll
-    for (int q = 0; q < N / T; ++q) w[q] = rp[swz(c + q * T, pp)];
+    reg_read_x<N, E, E>(lds, pp * N, w, c, pp);   // column c of pencil pp: elements c + q T
 #pragma unroll
     for (int q = 0; q < N / T; ++q) dst[(int64_t)n3 * (c + q * T)] = w[q];
 }
@@ -366,40 +356,45 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
             const int z = j + T * i + r * (N / R0);
             v[i * R0 + r] = (HALF ? half_slot<R0>(r) : inb(z, N, a.bz)) ? pen[z] : make_float2(0.0f, 0.0f);
         }
-    // The pencil's list range and each lane's first entry are loaded before the
-    // forward transform, which hides their latency (a pencil has fewer entries
-    // than lanes in most of the grid).
+    // The pencil's list range and each lane's first kPre entries (e0, e0 + T)
+    // are loaded before the forward transform, which hides their latency; most
+    // pencils have fewer entries than 2 T, and later rounds load as they go.
     const int64_t rp = (int64_t)(ky_0 + p) + (int64_t)n2 * kx;
     const int beg = (int)a.csr_off[rp], end = (int)a.csr_off[rp + 1];   // < 2^31 entries
     const int e0 = beg + j;
-    uint32_t m0 = 0;
-    float2 mg0 = make_float2(0.0f, 0.0f), V0 = mg0, d0 = mg0;
-    if (e0 < end) {
-        m0 = a.csr_meta[e0];
-        mg0 = a.csr_mag[e0];
-        V0 = a.csr_val[e0];
-        if (a.dmode) d0 = a.csr_d[e0];
+    constexpr int kPre = 2;
+    uint32_t mq[kPre];
+    float2 mgq[kPre], Vq[kPre], dq[kPre], dnq[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        mq[u] = 0;
+        mgq[u] = Vq[u] = dq[u] = dnq[u] = make_float2(0.0f, 0.0f);
+        const int e = e0 + u * T;
+        if (e < end) {
+            mq[u] = a.csr_meta[e];
+            mgq[u] = a.csr_mag[e];
+            Vq[u] = a.csr_val[e];
+            if (a.dmode) dq[u] = a.csr_d[e];
+        }
     }
     __syncthreads();   // the twiddle table
     float2* rowl = lds + p * N;
-    reg_fft_wave<N, false>(v, rowl, j, p, twl);
-#pragma unroll
-    for (int i = 0; i < E / RL; ++i)
-#pragma unroll
-        for (int q = 0; q < RL; ++q) rowl[swz(j + T * i + q * (N / RL), p)] = v[i * RL + q];
+    reg_fft_wave<N, false>(v, lds, p * N, j, p, twl);
+    reg_store_x<N, E, RL>(lds, p * N, v, j, p);
     wave_sync();
     // DR on the box: the transform of the box part minus D' of the last
     // iteration is the transform of the iterate's reflection.  The Rfree terms
     // read it from LDS; otherwise the errK loop subtracts in registers.
     if (a.dmode && a.rf_off) {
-        for (int e = e0; e < end; e += T) {
-            const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
+        auto sub_d = [&](uint32_t m, float2 dv) {
             float2* slot = rowl + swz((int)(m & 0x7FFu), p);
-            float2 dv = d0;   // by value: a select of two float2 lvalues selects addresses
-            if (e != e0) dv = a.csr_d[e];
             const float2 kv = *slot;
             *slot = make_float2(kv.x - dv.x, kv.y - dv.y);
-        }
+        };
+#pragma unroll
+        for (int u = 0; u < kPre; ++u)
+            if (e0 + u * T < end) sub_d(mq[u], dq[u]);
+        for (int e = e0 + kPre * T; e < end; e += T) sub_d(a.csr_meta[e], a.csr_d[e]);
         wave_sync();
     }
     // Rfree terms (reconstruct.m:137-147) on the transform before replacement
@@ -417,37 +412,37 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     // errK numerator (:135) and replacement (:163) from the half-spectrum lists;
     // the numerator is reduced here, per wave, so that it is not live through the
     // inverse transform
-    float2 dn0 = make_float2(0.0f, 0.0f);   // the new delta of the lane's first entry
     {
         float err = 0.0f;   // at most a few entries per lane
-        for (int e = e0; e < end; e += T) {
-            uint32_t m = m0;
-            float2 mg = mg0, V = V0;
-            if (e != e0) {
-                m = a.csr_meta[e];
-                mg = a.csr_mag[e];
-                V = a.csr_val[e];
-            }
+        // one entry: returns the new delta D = k' - k (DR)
+        auto entry = [&](int e, uint32_t m, float2 mg, float2 V, float2 dv) {
             float2* slot = rowl + swz((int)(m & 0x7FFu), p);
             float2 kv = *slot;
-            if (a.dmode && !a.rf_off) {
-                float2 dv = d0;
-                if (e != e0) dv = a.csr_d[e];
-                kv = make_float2(kv.x - dv.x, kv.y - dv.y);
-            }
+            if (a.dmode && !a.rf_off) kv = make_float2(kv.x - dv.x, kv.y - dv.y);
             const float ak = sqrtf(kv.x * kv.x + kv.y * kv.y);
             if (m & kHalfTermA) err += fabsf(ak - mg.x);
             if (m & kHalfTermB) err += fabsf(ak - mg.y);
             float2 kn = kv;
             if (m & kHalfEnforced)
                 kn = (m & kHalfAverage) ? make_float2(0.5f * (V.x + kv.x), 0.5f * (V.y + kv.y)) : V;
+            float2 dn = make_float2(0.0f, 0.0f);
             if (a.dmode) {
-                const float2 dn = make_float2(kn.x - kv.x, kn.y - kv.y);   // D = k' - k
+                dn = make_float2(kn.x - kv.x, kn.y - kv.y);
                 a.csr_d[e] = dn;
-                if (e == e0) dn0 = dn;
             } else if (m & kHalfEnforced) {
                 *slot = kn;
             }
+            return dn;
+        };
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) {
+            const int e = e0 + u * T;
+            if (e < end) dnq[u] = entry(e, mq[u], mgq[u], Vq[u], dq[u]);
+        }
+        for (int e = e0 + kPre * T; e < end; e += T) {
+            float2 dv = make_float2(0.0f, 0.0f);
+            if (a.dmode && !a.rf_off) dv = a.csr_d[e];
+            entry(e, a.csr_meta[e], a.csr_mag[e], a.csr_val[e], dv);
         }
         double de = (double)err;
         for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off, 64);
@@ -457,21 +452,18 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     if (a.dmode) {
         for (int i = j; i < N; i += T) rowl[i] = make_float2(0.0f, 0.0f);
         wave_sync();
-        for (int e = e0; e < end; e += T) {
-            const uint32_t m = (e == e0) ? m0 : a.csr_meta[e];
-            float2 dv = dn0;
-            if (e != e0) dv = a.csr_d[e];
-            rowl[swz((int)(m & 0x7FFu), p)] = dv;
-        }
+#pragma unroll
+        for (int u = 0; u < kPre; ++u)
+            if (e0 + u * T < end) rowl[swz((int)(mq[u] & 0x7FFu), p)] = dnq[u];
+        for (int e = e0 + kPre * T; e < end; e += T) rowl[swz((int)(a.csr_meta[e] & 0x7FFu), p)] = a.csr_d[e];
     }
     wave_sync();
     // the inverse transform takes fresh lane indices and its own address of the
     // twiddle table (opaque_zero) so that nothing of the forward one stays live
     const int t2 = phase_tid(), j2 = t2 % T, p2 = t2 / T;
-    float2* rowl2 = lds + p2 * N;
-    reg_read<N, E, R0>(rowl2, v, j2, p2);
+    reg_read_x<N, E, R0>(lds, p2 * N, v, j2, p2);
     wave_sync();
-    reg_fft_wave<N, true>(v, rowl2, j2, p2, lds + (kZP * N + opaque_zero()));
+    reg_fft_wave<N, true>(v, lds, p2 * N, j2, p2, lds + (kZP * N + opaque_zero()));
     const float sc = 1.0f / (float)N;
     float2* pen2 = a.B + (int64_t)n3 * ((ky_0 + p2) + (int64_t)n2 * kx);
 #pragma unroll
@@ -564,22 +556,19 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     reg_tw_fill<N>(twl, tw, t, half_threads<N>());
     {
         const int pp = t % kHP, c = t / kHP;
-        float2* rp = lds + pp * N;
         const float2* src = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
         const bool z_in = inb(i3_0 + pp, n3, a.bz);
         float2 w[N / T];
 #pragma unroll
         for (int q = 0; q < N / T; ++q) w[q] = z_in ? src[(int64_t)n3 * (c + q * T)] : make_float2(0.0f, 0.0f);
-#pragma unroll
-        for (int q = 0; q < N / T; ++q) rp[swz(c + q * T, pp)] = w[q];
+        reg_store_x<N, E, E>(lds, pp * N, w, c, pp);   // column c of pencil pp: elements c + q T
     }
     __syncthreads();
     const int t2 = phase_tid(), j2 = t2 % T, p2 = t2 / T;
-    float2* rowl = lds + p2 * N;
     float2 v[E];
-    reg_read<N, E, R0>(rowl, v, j2, p2);
+    reg_read_x<N, E, R0>(lds, p2 * N, v, j2, p2);
     wave_sync();
-    reg_fft_wave<N, true>(v, rowl, j2, p2, twl);
+    reg_fft_wave<N, true>(v, lds, p2 * N, j2, p2, twl);
     float2* dst = a.A + (int64_t)n2 * (kx + (int64_t)h * (i3_0 + p2));
     const float sc = 1.0f / (float)N;
     if (!inb(i3_0 + p2, n3, a.bz)) return;
@@ -612,32 +601,32 @@ __device__ __forceinline__ void load_c2r(const HalfIterArgs& a, float2* lds, flo
     const int n2 = a.n2, h = M + 1;
     {
         const int pp = t % XR, c = t / XR;
-        float2* rp = lds + pp * M;
         const float2* src = a.A + (i2_0 + pp) + (int64_t)n2 * h * i3;
         const bool y_in = inb(i2_0 + pp, n2, a.by);   // rows outside the box are not formed
         float2 w[M / T];
 #pragma unroll
         for (int q = 0; q < M / T; ++q) w[q] = y_in ? src[(int64_t)n2 * (c + q * T)] : make_float2(0.0f, 0.0f);
         if (c == 0) w[0].y = 0.0f;                 // Re of the kx = 0 bin
-#pragma unroll
-        for (int q = 0; q < M / T; ++q) rp[swz(c + q * T, pp)] = w[q];
+        reg_store_x<M, E, E>(lds, pp * M, w, c, pp);   // column c of pencil pp: elements c + q T
         if (c == 0) nyq[pp] = make_float2(y_in ? src[(int64_t)n2 * M].x : 0.0f, 0.0f);   // Re of kx = M
     }
     __syncthreads();
     float2* rowl = lds + p * M;
+    constexpr int LB = lane_bits(T);
+    const uint32_t lbj = lane_bytes(p * M, j, p);
 #pragma unroll
     for (int i = 0; i < E / R0; ++i)
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int e = j + T * i + r * (M / R0);
-            const float2 xk = rowl[swz(e, p)];
+            const float2 xk = lds_at<LB>(lds, lbj, T * i + r * (M / R0));   // element e
             const float2 xm = conjf2(e == 0 ? nyq[p] : rowl[swz(M - e, p)]);
             const float2 ev = make_float2(0.5f * (xk.x + xm.x), 0.5f * (xk.y + xm.y));
             const float2 od = cscale(cmul(csub(xk, xm), conjf2(twN[e])), 0.5f);
             v[i * R0 + r] = make_float2(ev.x - od.y, ev.y + od.x);   // ev + i*od
         }
     wave_sync();   // the row's lanes lie in one wave
-    reg_fft_wave<M, true>(v, rowl, j, p, twM);
+    reg_fft_wave<M, true>(v, lds, p * M, j, p, twM);
 }
 
 // The iterate update of the real pair at ad from pf = real(ifftn(k')) (:165):
