@@ -39,9 +39,9 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC = {512: 2 * 65853 * 1024 + 263168 * 1024}
 # HBM/MALL bytes per launch of the R-factor projector (same counters and
 # factors), keyed by (workload, three-Euler series); None where not measured.
-# tilt256 (three Euler angles): FETCH_SIZE 5,836,096 KiB (x2.00) + WRITE_SIZE
-# 2,992 KiB per launch, profiles/project_tilt256_traffic.md.
-PROJ_TRAFFIC = {("tilt256", True): 2 * 5836096 * 1024 + 2992 * 1024}
+# tilt256 (three Euler angles): FETCH_SIZE 5,510,784 KiB (x2.00) + WRITE_SIZE
+# 2,927 KiB per launch, profiles/project_tilt256_traffic.md.
+PROJ_TRAFFIC = {("tilt256", True): 2 * 5510784 * 1024 + 2927 * 1024}
 
 
 def log(msg):
