@@ -489,6 +489,7 @@ void project_one(genfire_ctx* ctx, const float* vol, int d1, int d2, int d3, con
     pa.R = r.as<float>();
     pa.P = 1;
     pa.out = o.as<float>();
+    pa.acc = pa.out;   // the slab sweep keeps its running ray sums in the output
     pa.m1 = d1; pa.m2 = d2;
     project(pa, st);
     GF_HIP(hipMemcpyAsync(proj, o.p, o.bytes, hipMemcpyDeviceToHost, st));
@@ -532,6 +533,7 @@ void rfactor_all(genfire_ctx* ctx, const float* model, int d1, int d2, int d3, c
     pa.R = r.as<float>();
     pa.P = P;
     pa.out = o.as<float>();
+    pa.acc = pa.out;   // the slab sweep keeps its running ray sums in the output
     pa.meas = m.as<float>();
     pa.m1 = p1; pa.m2 = p2;
     pa.sums = s.as<double>();
@@ -691,7 +693,7 @@ int genfire_rfactor_noalloc(genfire_ctx* ctx, const float* d_model, int32_t d1, 
         pa.d1 = d1; pa.d2 = d2; pa.d3 = d3;
         pa.R = dR;
         pa.P = P;
-        pa.out = d_simuProjs;
+        pa.out = d_simuProjs;   // acc stays null: one pass
         pa.meas = d_projections;
         pa.m1 = p1; pa.m2 = p2;
         pa.sums = ds;

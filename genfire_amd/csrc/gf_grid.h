@@ -279,6 +279,8 @@ struct ProjArgs {
     int m1, m2;        // measured size (crop of the back projection centre)
     double* sums;      // P x 4 x nblk partials: sum|b-m|, sum (b-m)^2, sum|m|, sum m^2
     int post_hoc;      // 1: sum ||b|-|m|| instead of |b-m| (Tian_calc_Rfactor_realspace)
+    float* acc;        // m1 x m2 x P ray sums between the passes of a slab sweep (may alias out);
+                       // nullable: one pass (project_slabs)
 };
 // The model forms a set of projector matrices (host copy, P x kRStride) reads.
 struct ProjForms {
@@ -297,6 +299,10 @@ void volume_cubic_form(const float* d_vol, int d1, int d2, int d3, float* d_dst,
 void crop_cubic_form(const float* d_src, int n1, int n2, int n3, float* d_dst, int d1, int d2, int d3,
                      hipStream_t st);
 int project_blocks(int m1, int m2);
+// Launches of the trilinear projector over a d1 x d2 x d3 model: 1, or the slab
+// count of a slab sweep when the z-duplicated form passes 128 MiB and
+// ProjArgs::acc is set.
+int project_slabs(int d1, int d2, int d3);
 // sets R[9] (y-fastest forms) and R[11] (lanes along pixel j) of a projection
 void project_orientation(float* R);
 bool project_y_exact(const float* R);

@@ -116,7 +116,7 @@ struct genfire_job {
     gf::DevBuf x, K, S, best, pfbuf, err_part, err_chunk, errK, errctr, den_tmp, den, recon;
     gf::DevBuf csr_cnt, csr_off, csr_coff, csr_meta, csr_val, csr_mag, csr_d, csr_mate;
     int64_t nmeas = 0;
-    gf::DevBuf vol, vol_y, zy, zx, vcubic, Rmat, rsums, rout;
+    gf::DevBuf vol, vol_y, zy, zx, vcubic, Rmat, rsums, rout, racc;
     // Rfree: withheld voxels per half pencil, their terms, shell order and denominators
     gf::DevBuf rf_off, rf_meta, rf_val, rf_terms, rf_order, rf_shell_off, rf_den, Rfree;
     int rf_bins = 0;

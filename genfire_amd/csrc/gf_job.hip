@@ -445,6 +445,7 @@ void run_rfactor(genfire_job* j, const float* d_model_fft, int slot)
     pa.m1 = j->dim1; pa.m2 = j->dim2;
     pa.sums = j->rsums.as<double>();
     pa.post_hoc = 0;
+    pa.acc = j->racc.as<float>();   // set when the model's forms take a slab sweep
     std::pair<hipEvent_t, hipEvent_t> ev;
     kstat_begin(j, GENFIRE_KSTAT_PROJECT, &ev);
     project(pa, st);
@@ -791,6 +792,8 @@ void recon_begin(genfire_job* j)
         if (f.zx) j->zx.alloc(sizeof(float) * zdup_floats(j->dim1, j->dim2, j->dim1));
         if (f.cubic) j->vcubic.alloc(sizeof(float) * cubic_form_floats(j->dim1, j->dim2, j->dim1));
         j->rsums.alloc(sizeof(double) * (size_t)j->P * 4 * project_blocks(j->dim1, j->dim2));
+        if ((f.zy || f.zx) && project_slabs(j->dim1, j->dim2, j->dim1) > 1)
+            j->racc.alloc(sizeof(float) * (size_t)j->P * j->dim1 * j->dim2);
         j->rout.alloc(sizeof(double) * (size_t)r.nR * j->P * 4);
         GF_HIP(hipStreamSynchronize(st));
     }
@@ -1259,6 +1262,7 @@ int64_t job_footprint(const genfire_params& p, int dim1, int dim2, int P)
     const int64_t meas = std::min<int64_t>(nf, 2 * pts);
     const int64_t iter = nf * (4 + 1 + 4 + 4 + 8) + spectrum + meas * 28 + n1 * n2 * 12 + box * 8 +
                          8 * zdup_floats(dim1, dim2, dim1) +    // the projector's two z-duplicated forms
+                         (project_slabs(dim1, dim2, dim1) > 1 ? (int64_t)4 * P * dim1 * dim2 : 0) +   // slab sums
                          (int64_t)P * 4 * 8 * (1 + p.numIterations / std::max(1, p.monitorR_loopLength));
     return inputs + std::max(grid, iter) + (64ll << 20);
 }

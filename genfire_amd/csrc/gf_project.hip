@@ -19,6 +19,7 @@
 // clamp at the top face: a coordinate equal to d - 1 takes cell d - 1 with
 // weight 0 on the zero row.
 #include <algorithm>
+#include <cstdlib>
 
 #include "gf_grid.h"
 
@@ -34,6 +35,10 @@ constexpr int kRows = 4;            // waves per workgroup along the other pixel
 constexpr int kRun = 32;            // one ray per lane: a wave's run along the fast pixel axis
 constexpr int kRunRows = 2;         // ... and its rows along the other (kRun x kRunRows = 64 lanes)
 constexpr float kRowAlign = 0.9f;   // lane rows when a pairing's |R| reaches this, else patches
+// z-duplicated forms above kSlabMinForm are projected in slabs of at most
+// kSlabForm bytes (project_slabs), half the 256 MiB last-level cache
+constexpr int64_t kSlabMinForm = 128ll << 20;
+constexpr int64_t kSlabForm = 128ll << 20;
 
 // Two (voxel, z + 1 neighbour) pairs in one 16-byte access (8-byte aligned).
 typedef float quad_t __attribute__((ext_vector_type(4), aligned(8)));
@@ -232,8 +237,9 @@ __device__ __forceinline__ void ray_range(const Ray& r, int d1, int d2, int d3, 
 // exchanged; 2, 8 x 8 patches, 16 x 16 per tile.  The two rows of a wave
 // sample cells about one voxel apart, so a row's second corner load (x = a0 +
 // 1 in the y-fastest form) meets the lines the other row's first load brought
-// into L1 at the same step: at tilt256 this takes the projector from 2.37 to
-// 2.15 ms per evaluation against one row of 64 j (16 j x 4 i: 2.39 ms).
+// into L1 at the same step: for 91 projections of a 256^3 model this takes the
+// projector from 2.37 to 2.15 ms per evaluation against one row of 64 j (16 j x
+// 4 i: 2.39 ms).
 __device__ __forceinline__ void ray_pixel(const ProjArgs& a, float form, int tile, int* i, int* j)
 {
     const int w = (int)threadIdx.x / kLanes, l = (int)threadIdx.x % kLanes;
@@ -284,13 +290,12 @@ __device__ __forceinline__ auto zdup_offset(int c, int r, int e0, uint32_t s1, u
 }
 
 // March the ray from Z to Z1 (Z = k - floor(d3/2) counts whole steps, exact in
-// FP32), summing its samples in ascending k.  v0 is the form's base, v1 the
-// next x row (YF) or y row.
+// FP32), adding its samples to acc in ascending k.  v0 is the form's base, v1
+// the next x row (YF) or y row.
 template <bool YF, bool WIDE>
 __device__ __forceinline__ float march(const char* __restrict__ v0, const char* __restrict__ v1, uint32_t s1,
-                                       uint32_t s12, const Ray& r, float Z, float Z1)
+                                       uint32_t s12, const Ray& r, float Z, float Z1, float acc)
 {
-    float acc = 0.0f;
     for (; Z <= Z1; Z += 1.0f) {
         const float fx = __builtin_fmaf(r.rx, Z, r.bx), fy = __builtin_fmaf(r.ry, Z, r.by);
         const float fz = __builtin_fmaf(r.rz, Z, r.bz);
@@ -366,6 +371,57 @@ __device__ __forceinline__ float march_cubic(const float* __restrict__ form, int
     return acc;
 }
 
+// Slab sweep (models whose z-duplicated form passes 128 MiB): the form is cut
+// into nslab slabs of slab_t z planes and every projection's rays cross one
+// slab per launch, so all projections share each slab's fills in the 256 MiB
+// last-level cache instead of each streaming the whole form (91 projections of
+// a 512^3 model, a 1.08 GB form: 26.6 -> 19.2 ms per evaluation in 9 slabs; of
+// a 256^3 model, 135 MB: 2.22 -> 2.16 ms in 2).  A ray's running sum
+// waits in acc[pixel] between launches; slabs are taken in the ray's own k
+// order (ascending z when rz >= 0, descending otherwise), so the sum adds the
+// same samples in the same order as one march and is bit for bit the same.
+//
+// The first k in [k0, k1 + 1] from which pred holds (pred monotonic, false
+// then true), starting from the estimate e and checked exactly; a wrong
+// estimate falls back to bisection.
+template <class Pred>
+__device__ __forceinline__ int first_true(int k0, int k1, float e, Pred pred)
+{
+    int r = (int)fminf((float)(k1 + 1), fmaxf((float)k0, e));
+    if ((r == k0 || !pred(r - 1)) && (r == k1 + 1 || pred(r))) return r;
+    int lo = k0, hi = k1 + 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pred(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Narrows the ray's steps [k0, k1] to those whose sample's z lies in
+// [zlo, zhi): fz(k) = fma(rz, Z, bz) is monotonic in k.
+__device__ __forceinline__ void slab_range(const Ray& r, int c3, float zlo, float zhi, int* k0, int* k1)
+{
+    const int a = *k0, b = *k1;
+    if (a > b) return;
+    auto fz = [&](int k) { return __builtin_fmaf(r.rz, (float)(k - c3), r.bz); };
+    if (r.rz == 0.0f) {
+        if (!(r.bz >= zlo && r.bz < zhi)) *k1 = a - 1;
+        return;
+    }
+    const float inv = 1.0f / r.rz, c = (float)c3;
+    int lo, hi;
+    if (r.rz > 0.0f) {
+        lo = first_true(a, b, ceilf((zlo - r.bz) * inv + c), [&](int k) { return fz(k) >= zlo; });
+        hi = first_true(a, b, ceilf((zhi - r.bz) * inv + c), [&](int k) { return fz(k) >= zhi; });
+    } else {
+        lo = first_true(a, b, floorf((zhi - r.bz) * inv + c) + 1.0f, [&](int k) { return fz(k) < zhi; });
+        hi = first_true(a, b, floorf((zlo - r.bz) * inv + c) + 1.0f, [&](int k) { return fz(k) < zlo; });
+    }
+    *k0 = lo;
+    *k1 = hi - 1;
+}
+
 // Every projection of a launch that is not all exact tilts about y: one ray
 // per lane on the z-duplicated forms, a wave's lanes in two runs of 32 along
 // the pixel axis R[11] picks (or 8 x 8 patches), and the block mapping of the
@@ -374,9 +430,11 @@ __device__ __forceinline__ float march_cubic(const float* __restrict__ form, int
 // tiles of groups of G neighbouring ones, so that the G workgroups of one tile
 // share the XCD's L2 fills.  With at most 8 projections each XCD takes an
 // eighth of every projection's tiles.
-template <bool WIDE, bool CUBIC>
-__global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles, int per, int group)
+template <bool WIDE, bool CUBIC, bool SLAB>
+__global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles, int per, int group, int slab,
+                                                           int nslab, int slab_t)
 {
+    static_assert(!(CUBIC && SLAB), "the slab sweep is for the trilinear forms");
     const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
     int p, tile;
     if (per == 1) {
@@ -397,24 +455,35 @@ __global__ __launch_bounds__(kLanes* kRows) void k_project(ProjArgs a, int tiles
     ray_pixel(a, R[11], tile, &i, &j);
     float acc = 0.0f;
     const bool valid = i < a.m1 && j < a.m2;
+    const int64_t o = (int64_t)i + (int64_t)a.m1 * (j + (int64_t)a.m2 * p);
     if (valid) {
         const Ray r = make_ray(a, R, i, j);
         int k0, k1;
         ray_range(r, d1, d2, d3, &k0, &k1);
+        if constexpr (SLAB) {
+            const int s = r.rz >= 0.0f ? slab : nslab - 1 - slab;
+            slab_range(r, c3, (float)(s * slab_t), (float)min(d3, (s + 1) * slab_t), &k0, &k1);
+            if (slab > 0) acc = a.acc[o];
+        }
         const float Z0 = (float)(k0 - c3), Z1 = (float)(k1 - c3);
         if constexpr (CUBIC) {
             acc = march_cubic(a.vcubic, d1, d2, d3, r, Z0, Z1);
         } else if (R[9] != 0.0f) {
             const char* v0 = reinterpret_cast<const char*>(a.zy);
-            acc = march<true, WIDE>(v0, v0 + 8 * (size_t)sy, sy, sxy, r, Z0, Z1);
+            acc = march<true, WIDE>(v0, v0 + 8 * (size_t)sy, sy, sxy, r, Z0, Z1, acc);
         } else {
             const char* v0 = reinterpret_cast<const char*>(a.zx);
-            acc = march<false, WIDE>(v0, v0 + 8 * (size_t)sx, sx, sxy, r, Z0, Z1);
+            acc = march<false, WIDE>(v0, v0 + 8 * (size_t)sx, sx, sxy, r, Z0, Z1, acc);
+        }
+    }
+    if constexpr (SLAB) {
+        if (slab < nslab - 1) {   // uniform: the sums follow the last slab
+            if (valid) a.acc[o] = acc;
+            return;
         }
     }
     double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
     if (valid) {
-        const int64_t o = (int64_t)i + (int64_t)a.m1 * (j + (int64_t)a.m2 * p);
         if (a.out) a.out[o] = acc;
         if (a.meas) {
             const double m = (double)a.meas[o];
@@ -643,6 +712,19 @@ void volume_yfast(const float* d_src, float* d_dst, int d1, int d2, int d3, hipS
 
 int64_t zdup_floats(int d1, int d2, int d3) { return 2 * (int64_t)(d1 + 1) * (d2 + 1) * d3; }
 
+int project_slabs(int d1, int d2, int d3)
+{
+    // GENFIRE_PROJECTOR_SLABS=n sets the slab count (1: one pass) for any model
+    // size: the tests compare the sweep with one pass on small models
+    if (const char* e = std::getenv("GENFIRE_PROJECTOR_SLABS")) {
+        const int n = std::atoi(e);
+        if (n >= 1) return std::min(n, d3);
+    }
+    const int64_t form = 4 * zdup_floats(d1, d2, d3);
+    if (form <= kSlabMinForm) return 1;
+    return (int)std::min<int64_t>(d3, (form + kSlabForm - 1) / kSlabForm);
+}
+
 int64_t cubic_form_floats(int d1, int d2, int d3) { return (int64_t)(d1 + 2) * (d2 + 2) * (d3 + 2); }
 
 void volume_cubic_form(const float* d_vol, int d1, int d2, int d3, float* d_dst, hipStream_t st)
@@ -701,9 +783,20 @@ void launch_ray(const ProjArgs& a, int tiles, hipStream_t st)
     // size (d1 + 1)(d2 + 1) below 2^24
     const bool wide = 4 * zdup_floats(a.d1, a.d2, a.d3) > (int64_t)UINT32_MAX ||
                       (int64_t)(a.d1 + 1) * (a.d2 + 1) >= (1ll << 24);
-    if (a.vcubic) k_project<true, true><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
-    else if (wide) k_project<true, false><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
-    else k_project<false, false><<<(unsigned)blocks, kLanes * kRows, 0, st>>>(a, tiles, per, group);
+    const dim3 g((unsigned)blocks), b(kLanes * kRows);
+    const int nslab = a.acc && !a.vcubic ? project_slabs(a.d1, a.d2, a.d3) : 1;
+    if (nslab > 1) {
+        const int t = (a.d3 + nslab - 1) / nslab, ns = (a.d3 + t - 1) / t;
+        for (int s = 0; s < ns; ++s) {
+            if (wide) k_project<true, false, true><<<g, b, 0, st>>>(a, tiles, per, group, s, ns, t);
+            else k_project<false, false, true><<<g, b, 0, st>>>(a, tiles, per, group, s, ns, t);
+            GF_LAUNCH_CHECK();
+        }
+        return;
+    }
+    if (a.vcubic) k_project<true, true, false><<<g, b, 0, st>>>(a, tiles, per, group, 0, 1, a.d3);
+    else if (wide) k_project<true, false, false><<<g, b, 0, st>>>(a, tiles, per, group, 0, 1, a.d3);
+    else k_project<false, false, false><<<g, b, 0, st>>>(a, tiles, per, group, 0, 1, a.d3);
     GF_LAUNCH_CHECK();
 }
 
