@@ -37,11 +37,12 @@ HBM_PEAK_GBS = 8000.0
 # on a known 1 GiB fill), in profiles/tilt256_hbm_traffic.md: 65853 KiB
 # fetched, 263168 KiB written.
 TRAFFIC = {512: 2 * 65853 * 1024 + 263168 * 1024}
-# HBM/MALL bytes per launch of the R-factor projector (same counters and
-# factors), keyed by (workload, three-Euler series); None where not measured.
-# tilt256 (three Euler angles): FETCH_SIZE 5,510,784 KiB (x2.00) + WRITE_SIZE
-# 2,927 KiB per launch, profiles/project_tilt256_traffic.md.
-PROJ_TRAFFIC = {("tilt256", True): 2 * 5510784 * 1024 + 2927 * 1024}
+# HBM/MALL bytes per evaluation of the R-factor projector (same counters and
+# factors; an evaluation is one launch per z slab), keyed by (workload,
+# three-Euler series); None where not measured.  tilt256 (three Euler angles,
+# two slabs): FETCH_SIZE 3,657,621 KiB (x2.00) + WRITE_SIZE 43,669 KiB per
+# evaluation, profiles/project_tilt256_traffic.md.
+PROJ_TRAFFIC = {("tilt256", True): 2 * 3657621 * 1024 + 43669 * 1024}
 
 
 def log(msg):
@@ -335,6 +336,19 @@ def extra_config(ctx, name, steps=1):
     except Exception as e:  # reported, never the target
         out["fp64_check"] = {"error": str(e)}
     return out
+
+
+def proj_slabs(d: int) -> int:
+    """Launches of the trilinear projector per evaluation of a d^3 model: its
+    z-duplicated form (8 (d+1)^2 d bytes) is swept in z slabs of at most 128 MiB
+    once it passes 128 MiB (the library's project_slabs, without the
+    GENFIRE_PROJECTOR_SLABS override)."""
+    form, cap = 8 * (d + 1) * (d + 1) * d, 128 << 20
+    if form <= cap:
+        return 1
+    n = min(d, -(-form // cap))
+    t = -(-d // n)
+    return -(-d // t)
 
 
 def box_rows(n: int, b: int, ts: int) -> int:
@@ -668,9 +682,10 @@ def main():
         "iteration_roofline_frac": (b_it / (it_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": proj_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": proj_achieved / HBM_PEAK_GBS, "traffic": proj_traffic,
-                     "kernel": f"k_project (R-factor projector, the largest kernel), {proj_bytes / 1e9:.3f} GB "
-                               f"algorithmic per launch (P (4 Dim^3 + 8 Dim^2)), {proj_ms:.3f} ms mean over "
-                               f"{npj} launches"},
+                     "kernel": f"k_project (R-factor projector, the largest kernel; {proj_slabs(w.dim)} z-slab "
+                               f"launch(es) per evaluation), {proj_bytes / 1e9:.3f} GB algorithmic per evaluation "
+                               f"(P (4 Dim^3 + 8 Dim^2)), {proj_ms:.3f} ms mean per evaluation over {npj} "
+                               f"evaluations"},
         "roofline_passes": {"bound": "hbm", "achieved": pass_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": pass_achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(n),
                             "kernel": f"k_half_y_fwd<{n}> (y-forward pass, box-pruned), {pass_bytes / 1e9:.3f} GB "
