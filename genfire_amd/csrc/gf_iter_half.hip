@@ -740,35 +740,47 @@ __global__ __launch_bounds__(x_threads<N / 2>()) void k_half_last_first(HalfBatc
     const int tu = phase_tid(), ju = tu % T, pu = tu / T;
     const int64_t row = (int64_t)N * ((i2_0 + pu) + (int64_t)n2 * i3) + 2 * ju;
     const bool row_in = inb(i2_0 + pu, n2, a.by);
-    float2 xv[E];
-    uint16_t s2[E];
-    int in[E];
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-        in[s] = pair_in(row_in, 2 * (ju + T * s), N, a.bx);
-        load_pair(a, row + 2 * T * s, in[s], a.dr != 0, xv[s], s2[s]);
-    }
+    // The update runs in two halves of E/2 pairs, each loading its iterate and
+    // support pairs just before use (the scheduling barrier keeps the second
+    // half's loads from being hoisted above the first half's arithmetic):
+    // 56 VGPRs instead of 70 at N = 512, four workgroups per CU instead of
+    // three, 110.5 -> 98.2 us per launch on 512^3 with the 256^3 box.
     float2 w[E];
+    constexpr int H = E / 2;
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int64_t ad = row + 2 * T * s;
-        float2 u = make_float2(0.0f, 0.0f);
-        if (in[s]) {
-            const float2 d = cscale(v[out_slot<M>(s)], sc);
-            if (a.dmode) {
-                // next transform input: 2 P_R(x_{k+1}) - P_R(x_k)
-                float2 rk;
-                const float2 xn = p5_dmode(a, d, xv[s], s2[s], ad, &rk);
-                const float2 rn = make_float2(prh(xn.x, (uint8_t)(s2[s] & 0xFF), a.positivity, a.support),
-                                              prh(xn.y, (uint8_t)(s2[s] >> 8), a.positivity, a.support));
-                if (a.best) *reinterpret_cast<float2*>(a.best + ad) = rn;
-                u = make_float2(2.0f * rn.x - rk.x, 2.0f * rn.y - rk.y);
-            } else {
-                const float2 xn = p5_update(a, d, xv[s], s2[s], ad, a.dr != 0);
-                u = p1_value(a, xn, s2[s], ad);
-            }
+    for (int hh = 0; hh < 2; ++hh) {
+        float2 xv[H];
+        uint16_t s2[H];
+        int in[H];
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            const int s = hh * H + q;
+            in[q] = pair_in(row_in, 2 * (ju + T * s), N, a.bx);
+            load_pair(a, row + 2 * T * s, in[q], a.dr != 0, xv[q], s2[q]);
         }
-        w[in_slot<M>(s)] = u;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            const int s = hh * H + q;
+            const int64_t ad = row + 2 * T * s;
+            float2 u = make_float2(0.0f, 0.0f);
+            if (in[q]) {
+                const float2 d = cscale(v[out_slot<M>(s)], sc);
+                if (a.dmode) {
+                    // next transform input: 2 P_R(x_{k+1}) - P_R(x_k)
+                    float2 rk;
+                    const float2 xn = p5_dmode(a, d, xv[q], s2[q], ad, &rk);
+                    const float2 rn = make_float2(prh(xn.x, (uint8_t)(s2[q] & 0xFF), a.positivity, a.support),
+                                                  prh(xn.y, (uint8_t)(s2[q] >> 8), a.positivity, a.support));
+                    if (a.best) *reinterpret_cast<float2*>(a.best + ad) = rn;
+                    u = make_float2(2.0f * rn.x - rk.x, 2.0f * rn.y - rk.y);
+                } else {
+                    const float2 xn = p5_update(a, d, xv[q], s2[q], ad, a.dr != 0);
+                    u = p1_value(a, xn, s2[q], ad);
+                }
+            }
+            w[in_slot<M>(s)] = u;
+        }
     }
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = w[s];
