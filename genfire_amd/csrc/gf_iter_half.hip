@@ -20,7 +20,9 @@
 // 4-KiB stride (no 2-MiB strides):
 //   P1 x: real rows -> P_R / DR reflection -> R2C (complex FFT of n1/2 + split)
 //         -> layout A  [y][kx][z]   (A[y + n2*(kx + h*z)])
-//   P2 y: A pencils (contiguous) -> FFT -> layout B  [z][ky][kx] (B[z + n3*(ky + n2*kx)])
+//   P2 y: A pencils (contiguous) -> FFT -> layout B  [kx][z/16][ky][z%16]
+//         (B[(z & 15) + 16*(ky + n2*((z >> 4) + (n3/16)*kx))]: a y tile's 16 z
+//         of every ky are one contiguous block, a z pencil 128-byte runs)
 //   P3 z: B pencils (contiguous) -> FFT -> errK + replacement -> IFFT -> B
 //   P4 y: B (strided) -> IFFT -> A
 //   P5 x: A (strided) -> Re of kx = 0, n1/2 -> C2R -> x (GENFIRE) or DR update
@@ -117,6 +119,20 @@ __device__ __forceinline__ bool y_tile(const HalfIterArgs& a, int cz, int& kx, i
     i3_0 = box_tile(tz, zt) * kHP;
     return true;
 }
+
+// Layout B of the y and z passes: plane kx holds n3/16 blocks of 16 z, each
+// block [ky][z % 16].  A y tile (one kx, 16 z from a multiple of 16) writes
+// and reads one contiguous block of n2 x 16 elements; a z pencil reads 16
+// contiguous z per block (128 bytes), and the 8 pencils of a z-pass tile
+// (consecutive ky) 1 KiB runs.  The z lengths with a register plan are
+// multiples of 16.
+static_assert(kHP == 16, "layout B blocks are a y tile's 16 z");
+__device__ __forceinline__ float2* b_block(const HalfIterArgs& a, int kx, int z0)
+{
+    return a.B + (int64_t)a.n2 * a.n3 * kx + (int64_t)16 * a.n2 * (z0 >> 4);
+}
+// element (ky, z) of a plane of B
+__device__ __forceinline__ int b_off(int ky, int z, int n2) { return (z & 15) + 16 * (ky + n2 * (z >> 4)); }
 
 // Which elements of the real pair (xc, xc+1) of a row lie in the box (bit 0, 1).
 __device__ __forceinline__ int pair_in(bool row_in, int xc, int n, int b)
@@ -303,11 +319,11 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_fwd(HalfBatch hb, 
     reg_store_x<N, E, RL>(lds, p * N, v, j, p);
     __syncthreads();
     const int pp = t % kHP, c = t / kHP;
-    float2* dst = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
+    float2* dst = b_block(a, kx, i3_0) + pp;   // the tile's contiguous block of B
     float2 w[N / T];
     reg_read_x<N, E, E>(lds, pp * N, w, c, pp);   // column c of pencil pp: elements c + q T
 #pragma unroll
-    for (int q = 0; q < N / T; ++q) dst[(int64_t)n3 * (c + q * T)] = w[q];
+    for (int q = 0; q < N / T; ++q) dst[kHP * (c + q * T)] = w[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -347,14 +363,15 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     // LDS: the tile, then the twiddle table
     float2* twl = lds + kZP * N;
     reg_tw_fill<N>(twl, tw, t, NT);
-    float2* pen = a.B + (int64_t)n3 * ((ky_0 + p) + (int64_t)n2 * kx);
+    const float2* bk = a.B + (int64_t)n2 * n3 * kx;   // plane kx of B
     float2 v[E];
 #pragma unroll
     for (int i = 0; i < E / R0; ++i)
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int z = j + T * i + r * (N / R0);
-            v[i * R0 + r] = (HALF ? half_slot<R0>(r) : inb(z, N, a.bz)) ? pen[z] : make_float2(0.0f, 0.0f);
+            v[i * R0 + r] = (HALF ? half_slot<R0>(r) : inb(z, N, a.bz)) ? bk[b_off(ky_0 + p, z, n2)]
+                                                                        : make_float2(0.0f, 0.0f);
         }
     // The pencil's list range and each lane's first kPre entries (e0, e0 + T)
     // are loaded before the forward transform, which hides their latency; most
@@ -465,13 +482,13 @@ __global__ __launch_bounds__(zfused_threads<N>()) void k_half_z_fused(HalfBatch 
     wave_sync();
     reg_fft_wave<N, true>(v, lds, p2 * N, j2, p2, lds + (kZP * N + opaque_zero()));
     const float sc = 1.0f / (float)N;
-    float2* pen2 = a.B + (int64_t)n3 * ((ky_0 + p2) + (int64_t)n2 * kx);
+    float2* bk2 = a.B + (int64_t)n2 * n3 * kx;
 #pragma unroll
     for (int i = 0; i < E / RL; ++i)
 #pragma unroll
         for (int q = 0; q < RL; ++q) {
             const int z = j2 + T * i + q * (N / RL);
-            if (HALF ? half_slot<RL>(q) : inb(z, N, a.bz)) pen2[z] = cscale(v[i * RL + q], sc);
+            if (HALF ? half_slot<RL>(q) : inb(z, N, a.bz)) bk2[b_off(ky_0 + p2, z, n2)] = cscale(v[i * RL + q], sc);
         }
 }
 
@@ -556,11 +573,11 @@ __global__ __launch_bounds__(half_threads<N>()) void k_half_y_inv(HalfBatch hb, 
     reg_tw_fill<N>(twl, tw, t, half_threads<N>());
     {
         const int pp = t % kHP, c = t / kHP;
-        const float2* src = a.B + (i3_0 + pp) + (int64_t)n3 * n2 * kx;
+        const float2* src = b_block(a, kx, i3_0) + pp;
         const bool z_in = inb(i3_0 + pp, n3, a.bz);
         float2 w[N / T];
 #pragma unroll
-        for (int q = 0; q < N / T; ++q) w[q] = z_in ? src[(int64_t)n3 * (c + q * T)] : make_float2(0.0f, 0.0f);
+        for (int q = 0; q < N / T; ++q) w[q] = z_in ? src[kHP * (c + q * T)] : make_float2(0.0f, 0.0f);
         reg_store_x<N, E, E>(lds, pp * N, w, c, pp);   // column c of pencil pp: elements c + q T
     }
     __syncthreads();
